@@ -1,0 +1,269 @@
+"""ctypes front-end of the CPU oracle (libhb_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker for the HIP path.  Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg import this module.
+
+Each function mirrors a reference routine (paths relative to the reference
+repo root):
+  hyperball()          src/centralities/hyper_ball.rs:702-736 and :754-874
+  centrality()         src/centralities/hyper_ball.rs:434-555
+  exact_closeness()    src/trails/bfs.rs:199-234, src/test_common.rs:217-227
+  mae/mape/spearman    src/utils.rs:57-100
+  dbg_mtx_csr()        gen_dbg_mtx.py:4-22 + parser_mtx/producers.rs:228-234
+                       + impl_sort.rs:10-51
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libhb_oracle.so")
+
+
+class Variant(C.Structure):
+    _fields_ = [
+        ("hash", C.c_int),
+        ("index", C.c_int),
+        ("lc_rule", C.c_int),
+        ("lc_calc", C.c_int),
+        ("sum_order", C.c_int),
+        ("alpha", C.c_int),
+        ("final_div", C.c_int),
+        ("bias", C.c_int),
+    ]
+
+    def astuple(self):
+        return tuple(getattr(self, f) for f, _ in self._fields_)
+
+    def __repr__(self):
+        return "Variant(" + ", ".join(f"{f}={getattr(self, f)}" for f, _ in self._fields_) + ")"
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int),
+        ("final_parity_latest", C.c_int),
+        ("increases", C.c_uint64 * 1024),
+    ]
+
+
+ITER_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.POINTER(C.c_uint8),
+                      C.POINTER(C.c_double), C.POINTER(C.c_double))
+
+_lib = None
+
+
+def build() -> str:
+    """Compiles the oracle with its Makefile (gcc)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        u64p = C.POINTER(C.c_uint64)
+        u32p = C.POINTER(C.c_uint32)
+        u8p = C.POINTER(C.c_uint8)
+        f64p = C.POINTER(C.c_double)
+        f32p = C.POINTER(C.c_float)
+        vp = C.POINTER(Variant)
+        L.hbo_siphash.restype = C.c_uint64
+        L.hbo_siphash.argtypes = [C.c_uint64, C.c_uint64, u8p, C.c_size_t, C.c_int, C.c_int]
+        L.hbo_hash_usize.restype = C.c_uint64
+        L.hbo_hash_usize.argtypes = [vp, C.c_uint64]
+        L.hbo_index_rank.restype = C.c_uint32
+        L.hbo_index_rank.argtypes = [vp, C.c_int, C.c_uint64, u8p]
+        L.hbo_estimate_u8.restype = C.c_float
+        L.hbo_estimate_u8.argtypes = [vp, C.c_int, u8p]
+        L.hbo_estimate_packed.restype = C.c_float
+        L.hbo_estimate_packed.argtypes = [vp, C.c_int, u32p]
+        L.hbo_knn_bias.restype = C.c_double
+        L.hbo_knn_bias.argtypes = [C.c_int, C.c_double]
+        L.hbo_pack6.argtypes = [C.c_int, u8p, u32p]
+        L.hbo_unpack6.argtypes = [C.c_int, u32p, u8p]
+        L.hbo_hyperball.restype = C.c_int
+        L.hbo_hyperball.argtypes = [u64p, u32p, C.c_uint64, C.c_int, vp, C.c_int, C.c_int,
+                                    C.c_int, u8p, u8p, f64p, f64p, C.POINTER(Stats),
+                                    ITER_CB, C.c_void_p]
+        L.hbo_estimates.argtypes = [vp, C.c_int, C.c_uint64, u8p, f32p]
+        L.hbo_centrality.argtypes = [C.c_int, C.c_int, C.c_uint64, f32p, f64p, f64p, f64p]
+        L.hbo_exact_closeness.argtypes = [u64p, u32p, C.c_uint64, C.c_uint64, C.c_uint64, f64p]
+        for f in ("hbo_mae", "hbo_mape", "hbo_spearman"):
+            getattr(L, f).restype = C.c_double
+            getattr(L, f).argtypes = [f64p, f64p, C.c_uint64]
+        L.hbo_ranks.argtypes = [f64p, C.c_uint64, u64p]
+        L.hbo_dbg_mtx_csr.restype = C.c_uint64
+        L.hbo_dbg_mtx_csr.argtypes = [C.c_int, C.c_int, u64p, u32p]
+        _lib = L
+    return _lib
+
+
+def _p(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def pinned() -> Variant:
+    return Variant.in_dll(lib(), "HBO_VARIANT_PINNED")
+
+
+def variant(**kw) -> Variant:
+    v = Variant()
+    base = pinned()
+    for f, _ in Variant._fields_:
+        setattr(v, f, kw.get(f, getattr(base, f)))
+    return v
+
+
+def siphash(msg: bytes, c_rounds=1, d_rounds=3, k0=0, k1=0) -> int:
+    buf = np.frombuffer(msg, dtype=np.uint8).copy() if msg else np.zeros(1, np.uint8)
+    return int(lib().hbo_siphash(k0, k1, _p(buf, C.c_uint8), len(msg), c_rounds, d_rounds))
+
+
+def index_rank(p: int, x: int, v: Variant | None = None):
+    v = v or pinned()
+    r = C.c_uint8()
+    idx = lib().hbo_index_rank(C.byref(v), p, x, C.byref(r))
+    return int(idx), int(r.value)
+
+
+def estimate(p: int, regs: np.ndarray, v: Variant | None = None) -> np.float32:
+    v = v or pinned()
+    regs = np.ascontiguousarray(regs, dtype=np.uint8)
+    return np.float32(lib().hbo_estimate_u8(C.byref(v), p, _p(regs, C.c_uint8)))
+
+
+def estimate_packed(p: int, words: np.ndarray, v: Variant | None = None) -> np.float32:
+    v = v or pinned()
+    words = np.ascontiguousarray(words, dtype=np.uint32)
+    return np.float32(lib().hbo_estimate_packed(C.byref(v), p, _p(words, C.c_uint32)))
+
+
+def pack6(p: int, regs: np.ndarray) -> np.ndarray:
+    m = 1 << p
+    out = np.zeros((m + 4) // 5, np.uint32)
+    regs = np.ascontiguousarray(regs, dtype=np.uint8)
+    lib().hbo_pack6(p, _p(regs, C.c_uint8), _p(out, C.c_uint32))
+    return out
+
+
+def estimates(p: int, regs: np.ndarray, v: Variant | None = None) -> np.ndarray:
+    v = v or pinned()
+    regs = np.ascontiguousarray(regs, dtype=np.uint8)
+    n = regs.size >> p
+    out = np.empty(n, np.float32)
+    lib().hbo_estimates(C.byref(v), p, n, _p(regs, C.c_uint8), _p(out, C.c_float))
+    return out
+
+
+@dataclass
+class HBResult:
+    regs_ref: np.ndarray      # the buffer the reference leaves in its counters file
+    regs_last: np.ndarray     # output of the last executed iteration
+    dist: np.ndarray
+    inv: np.ndarray
+    iterations: int
+    final_parity_latest: bool
+    increases: np.ndarray
+
+
+def hyperball(offsets: np.ndarray, nbrs: np.ndarray, p: int, *, v: Variant | None = None,
+              threads: int = 1, packed: bool = False, max_iters: int = 0,
+              on_iter=None) -> HBResult:
+    """Runs the restated HyperBall; on_iter(t, regs_out, dist, inv) gets copies."""
+    v = v or pinned()
+    off = np.ascontiguousarray(offsets, dtype=np.uint64)
+    nb = np.ascontiguousarray(nbrs, dtype=np.uint32)
+    n = off.size - 1
+    m = 1 << p
+    regs_ref = np.zeros(n * m, np.uint8)
+    regs_last = np.zeros(n * m, np.uint8)
+    dist = np.zeros(n, np.float64)
+    inv = np.zeros(n, np.float64)
+    st = Stats()
+    if on_iter is not None:
+        def _cb(_u, t, regs, d, i):
+            on_iter(t, np.ctypeslib.as_array(regs, shape=(n * m,)).copy(),
+                    np.ctypeslib.as_array(d, shape=(n,)).copy(),
+                    np.ctypeslib.as_array(i, shape=(n,)).copy())
+        cb = ITER_CB(_cb)
+    else:
+        cb = ITER_CB()
+    nbp = _p(nb, C.c_uint32) if nb.size else C.POINTER(C.c_uint32)()
+    rc = lib().hbo_hyperball(_p(off, C.c_uint64), nbp, n, p, C.byref(v), threads,
+                             int(packed), max_iters, _p(regs_ref, C.c_uint8),
+                             _p(regs_last, C.c_uint8), _p(dist, C.c_double),
+                             _p(inv, C.c_double), C.byref(st), cb, None)
+    if rc != 0:
+        raise RuntimeError(f"hbo_hyperball failed: {rc}")
+    it = st.iterations
+    return HBResult(regs_ref, regs_last, dist, inv, it, bool(st.final_parity_latest),
+                    np.array(st.increases[:min(it, 1024)], dtype=np.uint64))
+
+
+CLOSENESS, HARMONIC, LIN = 0, 1, 2
+
+
+def centrality(kind: int, normalize: bool, est: np.ndarray, dist: np.ndarray,
+               inv: np.ndarray) -> np.ndarray:
+    est = np.ascontiguousarray(est, dtype=np.float32)
+    dist = np.ascontiguousarray(dist, dtype=np.float64)
+    inv = np.ascontiguousarray(inv, dtype=np.float64)
+    out = np.empty(est.size, np.float64)
+    lib().hbo_centrality(kind, int(normalize), est.size, _p(est, C.c_float),
+                         _p(dist, C.c_double), _p(inv, C.c_double), _p(out, C.c_double))
+    return out
+
+
+def exact_closeness(offsets, nbrs, src_begin=0, src_end=None) -> np.ndarray:
+    off = np.ascontiguousarray(offsets, dtype=np.uint64)
+    nb = np.ascontiguousarray(nbrs, dtype=np.uint32)
+    n = off.size - 1
+    src_end = n if src_end is None else min(src_end, n)
+    out = np.zeros(max(src_end - src_begin, 0), np.float64)
+    lib().hbo_exact_closeness(_p(off, C.c_uint64), _p(nb, C.c_uint32), n, src_begin, src_end,
+                              _p(out, C.c_double))
+    return out
+
+
+def _metric(name, a, b):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    return float(getattr(lib(), name)(_p(a, C.c_double), _p(b, C.c_double), a.size))
+
+
+def mae(a, b):
+    return _metric("hbo_mae", a, b)
+
+
+def mape(a, b):
+    return _metric("hbo_mape", a, b)
+
+
+def spearman(a, b):
+    return _metric("hbo_spearman", a, b)
+
+
+def ranks(v) -> np.ndarray:
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    out = np.empty(v.size, np.uint64)
+    lib().hbo_ranks(_p(v, C.c_double), v.size, _p(out, C.c_uint64))
+    return out
+
+
+def dbg_mtx_csr(sigma: int, k: int):
+    """CSR of gen_dbg_mtx.py(sigma, k) as the reference's MTX reader builds it."""
+    n = sigma ** k
+    off = np.zeros(n + 1, np.uint64)
+    arcs = lib().hbo_dbg_mtx_csr(sigma, k, _p(off, C.c_uint64), C.POINTER(C.c_uint32)())
+    nb = np.zeros(arcs, np.uint32)
+    lib().hbo_dbg_mtx_csr(sigma, k, _p(off, C.c_uint64), _p(nb, C.c_uint32))
+    return off, nb
