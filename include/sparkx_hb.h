@@ -1,0 +1,144 @@
+/*
+ * sparkx_hb.h -- C ABI of the MI355X HyperBall engine (libsparkx_hb.so).
+ *
+ * Drop-in boundary for the reference's HyperBall path: HyperBallInner<'a,N,E,Ix,P,6>
+ * on &GraphMemoryMap<N,E,Ix> (src/centralities/hyper_ball.rs:81-108).  Plain
+ * pointers and sizes only; every call returns an hb_status (0 = ok, < 0 = error)
+ * and leaves a thread-local message for hb_last_error().  Host buffers are
+ * caller-owned; a context owns its device memory and one HIP stream and is not
+ * thread-safe (use one context per host thread).
+ */
+#ifndef SPARKX_HB_H
+#define SPARKX_HB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    HB_OK = 0,
+    HB_EINVAL = -1,   /* bad argument (null pointer, p out of 4..16, malformed CSR) */
+    HB_ENOMEM = -2,   /* device or pinned host allocation failed                */
+    HB_EHIP = -3,     /* a HIP runtime call failed                              */
+    HB_ESTATE = -4,   /* call out of order (e.g. hb_run before hb_stage_csr)   */
+    HB_ERANGE = -5,   /* node id / size beyond what the engine supports         */
+} hb_status;
+
+/* hb_run flags */
+#define HB_FLAG_DENSE 0x1u    /* recompute every node every iteration (the reference's
+                                 schedule); default skips nodes none of whose
+                                 neighbours changed, with identical results      */
+#define HB_FLAG_GRAPH 0x2u    /* capture one iteration in a hipGraph and replay it   */
+#define HB_FLAG_NF    0x4u    /* record the neighbourhood function N(t) each iteration */
+#define HB_FLAG_TIMING 0x8u   /* per-iteration device timing (adds a sync per iteration) */
+#define HB_FLAG_KTIMING 0x10u /* per-kernel device timing with events (implies TIMING;
+                                 not combinable with HB_FLAG_GRAPH)              */
+
+/* hb_read_registers / hb_read_estimates selector */
+#define HB_REGS_REFERENCE 0   /* the buffer the reference leaves in its counters file:
+                                 the last iteration's output if the iteration count is
+                                 even, its input if odd (hyper_ball.rs:840,865)      */
+#define HB_REGS_LATEST 1      /* output of the last executed iteration              */
+
+/* hb_centrality kinds */
+#define HB_CLOSENESS 0
+#define HB_HARMONIC 1
+#define HB_LIN 2
+
+#define HB_STATS_MAX_ITERS 4096
+/* kernel slots of one iteration: union over degree bins 0..2, hub chunks, hub merge */
+#define HB_NKERNELS 5
+
+typedef struct {
+    int32_t iterations;          /* executed, including the final zero-increase one   */
+    int32_t converged;           /* 1: stopped by the reference's rule; 0: truncated  */
+    int32_t final_parity_latest; /* 1 if HB_REGS_REFERENCE == HB_REGS_LATEST          */
+    int32_t p;
+    double ms_total;             /* device time of the iteration loop (events)         */
+    double ms_init;              /* device time of the register initialisation        */
+    uint64_t arcs_gathered;      /* sum over iterations of neighbour sketches gathered */
+    uint64_t nodes_recomputed;   /* sum over iterations of nodes recomputed            */
+    uint64_t increases[HB_STATS_MAX_ITERS];    /* nodes whose estimate increased, per t */
+    uint64_t iter_arcs[HB_STATS_MAX_ITERS];
+    uint64_t iter_nodes[HB_STATS_MAX_ITERS];
+    float iter_ms[HB_STATS_MAX_ITERS];         /* filled with HB_FLAG_TIMING          */
+    double nf[HB_STATS_MAX_ITERS];             /* N(t) = sum_u est_t(u), HB_FLAG_NF    */
+    /* per kernel slot: launches, device ms (HB_FLAG_KTIMING), work done, schedule */
+    uint64_t kernel_launches[HB_NKERNELS];
+    double kernel_ms[HB_NKERNELS];
+    uint64_t kernel_arcs[HB_NKERNELS];    /* neighbour (or partial) sketches gathered */
+    uint64_t kernel_nodes[HB_NKERNELS];   /* nodes recomputed                         */
+    uint32_t kernel_items[HB_NKERNELS];   /* nodes (bins, merge) or tasks (chunks) scheduled */
+    uint32_t kernel_team[HB_NKERNELS];    /* lanes per node / task                    */
+} hb_stats;
+
+typedef struct hb_ctx hb_ctx;
+
+/* Creates a context on HIP device `device_ordinal`.
+ * Replaces: HyperBallInner::new_no_compute_impl's resource set-up
+ * (src/centralities/hyper_ball.rs:684-700, :738-752). */
+int hb_create(hb_ctx **out, int device_ordinal);
+
+/* Releases the context and all its device memory.
+ * Replaces: dropping HyperBallInner (its mmaps). */
+void hb_destroy(hb_ctx *ctx);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *hb_last_error(void);
+
+/* Stages a CSR graph into HBM once, through pinned host buffers.  offsets has n+1
+ * entries and nbrs has arcs entries, each 8- or 4-byte unsigned (the reference's
+ * mmap'd index and edge files are 8-byte).  Neighbour ids must be < n and offsets
+ * non-decreasing with offsets[0] == 0 and offsets[n] == arcs; n < 2^32, arcs < 2^32.
+ * Replaces: the graph borrow g: &GraphMemoryMap and its CSR walk
+ * GraphMemoryMap::{size,width,index_node,neighbours} (src/graph/mod.rs:569-595,648-658). */
+int hb_stage_csr(hb_ctx *ctx, const void *offsets, int off_bytes, uint64_t n,
+                 const void *nbrs, int nbr_bytes, uint64_t arcs);
+
+/* Runs HyperBall with precision p (2^p registers per node, B = 6) on the staged
+ * graph.  max_iters == 0 follows the reference: iterate until no node's estimate
+ * increased; max_iters > 0 truncates (an extension: the reference has no cap).
+ * Replaces: HyperBallInner::new / new_with_conf and compute_with_proc_mem_impl
+ * (src/centralities/hyper_ball.rs:124-157, :702-736, :754-874). `stats` may be NULL. */
+int hb_run(hb_ctx *ctx, int p, int max_iters, uint32_t flags, hb_stats *stats);
+
+/* Copies n * 2^p u8 registers (node-major) of the selected buffer to dst.
+ * Replaces: the counters file HyperBallInner::counters (hyper_ball.rs:92). */
+int hb_read_registers(hb_ctx *ctx, int which, uint8_t *dst);
+
+/* f32 estimate of every node's sketch in the selected buffer.
+ * Replaces: HyperLogLog::estimate_cardinality() on counters (hyper_ball.rs:452,506,544). */
+int hb_read_estimates(hb_ctx *ctx, int which, float *dst);
+
+/* Copies the f64 distance and inverse-distance accumulators (either may be NULL).
+ * Replaces: geodesic_distances / harmonic_geodesic_distances (hyper_ball.rs:273-280). */
+int hb_read_accumulators(hb_ctx *ctx, double *dist, double *inv);
+
+/* Closeness / harmonic / Lin centrality from the last run, computed on the device.
+ * Replaces: compute_closeness_centrality / compute_harmonic_centrality /
+ * compute_lins_centrality (hyper_ball.rs:434-555), with the reference's formulas. */
+int hb_centrality(hb_ctx *ctx, int kind, int normalize, double *dst);
+
+/* N(t) for t = 1..min(iterations, cap) of the last run made with HB_FLAG_NF;
+ * returns the count written (>= 0) or an hb_status.  No reference counterpart. */
+int hb_neighbourhood_function(hb_ctx *ctx, double *dst, int cap);
+
+/* Scheduling knobs (results never depend on them).  Any 0 keeps the current value.
+ *   small_max_deg : nodes up to this degree use one 2^p/16-lane group per node
+ *   wave_max_deg  : nodes up to this degree use one wavefront per node
+ *   chunk         : neighbours per wavefront task for nodes above wave_max_deg
+ * No reference counterpart (the reference uses static contiguous thread blocks). */
+int hb_set_schedule(hb_ctx *ctx, uint32_t small_max_deg, uint32_t wave_max_deg,
+                    uint32_t chunk);
+
+/* Number of nodes / arcs staged (0 if none). */
+uint64_t hb_num_nodes(const hb_ctx *ctx);
+uint64_t hb_num_arcs(const hb_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

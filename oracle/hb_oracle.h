@@ -87,6 +87,8 @@ typedef struct {
     int final_parity_latest; /* 1 if the reference's counters file holds the last
                                 iteration's output, 0 if it holds its input        */
     uint64_t increases[1024];/* per-iteration count of nodes whose estimate increased */
+    double init_seconds;     /* wall time of the register initialisation */
+    double loop_seconds;     /* wall time of the iterations               */
 } hbo_stats;
 
 /*

@@ -49,6 +49,8 @@ class Stats(C.Structure):
         ("iterations", C.c_int),
         ("final_parity_latest", C.c_int),
         ("increases", C.c_uint64 * 1024),
+        ("init_seconds", C.c_double),
+        ("loop_seconds", C.c_double),
     ]
 
 
@@ -173,6 +175,8 @@ class HBResult:
     iterations: int
     final_parity_latest: bool
     increases: np.ndarray
+    init_seconds: float = 0.0
+    loop_seconds: float = 0.0
 
 
 def hyperball(offsets: np.ndarray, nbrs: np.ndarray, p: int, *, v: Variant | None = None,
@@ -206,7 +210,8 @@ def hyperball(offsets: np.ndarray, nbrs: np.ndarray, p: int, *, v: Variant | Non
         raise RuntimeError(f"hbo_hyperball failed: {rc}")
     it = st.iterations
     return HBResult(regs_ref, regs_last, dist, inv, it, bool(st.final_parity_latest),
-                    np.array(st.increases[:min(it, 1024)], dtype=np.uint64))
+                    np.array(st.increases[:min(it, 1024)], dtype=np.uint64),
+                    st.init_seconds, st.loop_seconds)
 
 
 CLOSENESS, HARMONIC, LIN = 0, 1, 2

@@ -1,0 +1,113 @@
+"""ctypes binding of libsparkx_hb.so (the C ABI in include/sparkx_hb.h and
+include/sparkx_hb_graphgen.h).  Loading fails loudly when the library is
+missing: there is no CPU fallback for the HyperBall engine."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsparkx_hb.so")
+
+HB_OK, HB_EINVAL, HB_ENOMEM, HB_EHIP, HB_ESTATE, HB_ERANGE = 0, -1, -2, -3, -4, -5
+HB_FLAG_DENSE, HB_FLAG_GRAPH, HB_FLAG_NF, HB_FLAG_TIMING, HB_FLAG_KTIMING = 0x1, 0x2, 0x4, 0x8, 0x10
+HB_REGS_REFERENCE, HB_REGS_LATEST = 0, 1
+HB_CLOSENESS, HB_HARMONIC, HB_LIN = 0, 1, 2
+HB_STATS_MAX_ITERS = 4096
+HB_NKERNELS = 5
+KERNEL_SLOTS = ("union_bin0", "union_bin1", "union_bin2", "hub_chunk", "hub_merge")
+
+# every extern "C" symbol the headers declare
+EXPORTS = (
+    "hb_create", "hb_destroy", "hb_last_error", "hb_stage_csr", "hb_run",
+    "hb_read_registers", "hb_read_estimates", "hb_read_accumulators", "hb_centrality",
+    "hb_neighbourhood_function", "hb_set_schedule", "hb_num_nodes", "hb_num_arcs",
+    "hb_graph_er", "hb_graph_rmat", "hb_graph_kmer", "hb_graph_nodes", "hb_graph_arcs",
+    "hb_graph_offsets", "hb_graph_neighbours", "hb_graph_free",
+)
+
+
+class HbStats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int32),
+        ("converged", C.c_int32),
+        ("final_parity_latest", C.c_int32),
+        ("p", C.c_int32),
+        ("ms_total", C.c_double),
+        ("ms_init", C.c_double),
+        ("arcs_gathered", C.c_uint64),
+        ("nodes_recomputed", C.c_uint64),
+        ("increases", C.c_uint64 * HB_STATS_MAX_ITERS),
+        ("iter_arcs", C.c_uint64 * HB_STATS_MAX_ITERS),
+        ("iter_nodes", C.c_uint64 * HB_STATS_MAX_ITERS),
+        ("iter_ms", C.c_float * HB_STATS_MAX_ITERS),
+        ("nf", C.c_double * HB_STATS_MAX_ITERS),
+        ("kernel_launches", C.c_uint64 * HB_NKERNELS),
+        ("kernel_ms", C.c_double * HB_NKERNELS),
+        ("kernel_arcs", C.c_uint64 * HB_NKERNELS),
+        ("kernel_nodes", C.c_uint64 * HB_NKERNELS),
+        ("kernel_items", C.c_uint32 * HB_NKERNELS),
+        ("kernel_team", C.c_uint32 * HB_NKERNELS),
+    ]
+
+
+class HyperBallError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m sparkx_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp, u8p, f32p, f64p = C.c_void_p, C.POINTER(C.c_uint8), C.POINTER(C.c_float), C.POINTER(C.c_double)
+    u32p = C.POINTER(C.c_uint32)
+    sig = {
+        "hb_create": (C.c_int, [C.POINTER(vp), C.c_int]),
+        "hb_destroy": (None, [vp]),
+        "hb_last_error": (C.c_char_p, []),
+        "hb_stage_csr": (C.c_int, [vp, C.c_void_p, C.c_int, C.c_uint64, C.c_void_p, C.c_int, C.c_uint64]),
+        "hb_run": (C.c_int, [vp, C.c_int, C.c_int, C.c_uint32, C.POINTER(HbStats)]),
+        "hb_read_registers": (C.c_int, [vp, C.c_int, u8p]),
+        "hb_read_estimates": (C.c_int, [vp, C.c_int, f32p]),
+        "hb_read_accumulators": (C.c_int, [vp, f64p, f64p]),
+        "hb_centrality": (C.c_int, [vp, C.c_int, C.c_int, f64p]),
+        "hb_neighbourhood_function": (C.c_int, [vp, f64p, C.c_int]),
+        "hb_set_schedule": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_uint32]),
+        "hb_num_nodes": (C.c_uint64, [vp]),
+        "hb_num_arcs": (C.c_uint64, [vp]),
+        "hb_graph_er": (C.c_int, [C.POINTER(vp), C.c_uint64, C.c_uint64, C.c_uint64, C.c_int]),
+        "hb_graph_rmat": (C.c_int, [C.POINTER(vp), C.c_int, C.c_uint64, C.c_double, C.c_double,
+                                    C.c_double, C.c_uint64, C.c_int]),
+        "hb_graph_kmer": (C.c_int, [C.POINTER(vp), C.c_uint64, C.c_int, C.c_double, C.c_int,
+                                    C.c_uint64, C.c_int]),
+        "hb_graph_nodes": (C.c_uint64, [vp]),
+        "hb_graph_arcs": (C.c_uint64, [vp]),
+        "hb_graph_offsets": (u32p, [vp]),
+        "hb_graph_neighbours": (u32p, [vp]),
+        "hb_graph_free": (None, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc: int) -> int:
+    if rc < 0:
+        msg = load().hb_last_error()
+        raise HyperBallError(rc, msg.decode() if msg else "")
+    return rc
+
+
+def ptr(a, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))

@@ -1,0 +1,690 @@
+// hb_api.hip -- C ABI of the MI355X HyperBall engine (declared in include/sparkx_hb.h).
+//
+// A context owns one HIP stream and the device-resident state of one graph:
+//   CSR (u32 offsets[n+1], u32 neighbours[arcs]) staged once through pinned
+//   host buffers; per run: two u8 register buffers [n][2^p], f32 estimates,
+//   f64 distance / inverse-distance accumulators, two changed-node bitmaps,
+//   the iteration state and per-iteration counters, and the degree-binned
+//   schedule (node lists per team width, hub chunk tasks and their scratch).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sparkx_hb.h"
+#include "hb_device.h"
+#include "hb_kernels.h"
+
+#define HB_BIAS_TABLE_QUAL static const
+#include "hll_bias_table.inc"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HB_TRY(call)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(e_ == hipErrorOutOfMemory ? HB_ENOMEM : HB_EHIP, "%s failed: %s", #call, \
+                        hipGetErrorString(e_));                                               \
+    } while (0)
+
+// HLL++ linear-counting thresholds (Heule, Nunkesser, Hall 2013), p = 4..16
+const double kLcThreshold[17] = {0, 0, 0, 0, 10, 20, 40, 80, 220, 400, 900,
+                                 1800, 3100, 6500, 11500, 20000, 50000};
+
+template <class T>
+struct DevBuf {
+    T *ptr = nullptr;
+    size_t count = 0;
+    hipError_t ensure(size_t n)
+    {
+        if (n <= count && ptr) return hipSuccess;
+        release();
+        hipError_t e = hipMalloc(&ptr, std::max<size_t>(n, 1) * sizeof(T));
+        if (e == hipSuccess) count = n;
+        else ptr = nullptr;
+        return e;
+    }
+    void release()
+    {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        count = 0;
+    }
+};
+
+}  // namespace
+
+struct hb_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t n = 0, arcs = 0;
+    DevBuf<uint32_t> off, nbr;
+    std::vector<uint32_t> h_off;  // host copy of the offsets, for scheduling
+
+    // schedule knobs
+    uint32_t small_max = 8, mid_max = 64, wave_max = 4096, chunk = 2048;
+
+    // run state
+    int p = 0;
+    bool ran = false;
+    DevBuf<uint8_t> regs0, regs1, scratch;
+    DevBuf<float> est, est_tmp, lc;
+    DevBuf<double> dist, inv, bias, cent;
+    DevBuf<uint32_t> bits0, bits1, sched_buf, chunk_flag;
+    DevBuf<HbIterState> st;
+    DevBuf<HbIterRec> rec;
+    uint32_t rec_cap = 0;
+    hb::HbSchedule sched;
+    HbDev dev{};
+    HbIterState h_st{};
+    std::vector<HbIterRec> h_rec;
+    uint32_t sched_p = 0, sched_key[4] = {};
+
+    void release_run()
+    {
+        regs0.release();
+        regs1.release();
+        scratch.release();
+        est.release();
+        est_tmp.release();
+        lc.release();
+        dist.release();
+        inv.release();
+        bias.release();
+        cent.release();
+        bits0.release();
+        bits1.release();
+        sched_buf.release();
+        chunk_flag.release();
+        st.release();
+        rec.release();
+        ran = false;
+        sched_p = 0;
+    }
+};
+
+static bool set_device(hb_ctx *c) { return hipSetDevice(c->device) == hipSuccess; }
+
+extern "C" {
+
+const char *hb_last_error(void) { return g_err.c_str(); }
+
+int hb_create(hb_ctx **out, int device_ordinal)
+{
+    if (!out) return fail(HB_EINVAL, "hb_create: out is NULL");
+    *out = nullptr;
+    int count = 0;
+    HB_TRY(hipGetDeviceCount(&count));
+    if (device_ordinal < 0 || device_ordinal >= count)
+        return fail(HB_EINVAL, "hb_create: device %d out of range (%d devices)", device_ordinal, count);
+    HB_TRY(hipSetDevice(device_ordinal));
+    hb_ctx *c = new hb_ctx();
+    c->device = device_ordinal;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(HB_EHIP, "hipStreamCreate failed: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    g_err.clear();
+    return HB_OK;
+}
+
+void hb_destroy(hb_ctx *c)
+{
+    if (!c) return;
+    set_device(c);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->release_run();
+    c->off.release();
+    c->nbr.release();
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+uint64_t hb_num_nodes(const hb_ctx *c) { return c ? c->n : 0; }
+uint64_t hb_num_arcs(const hb_ctx *c) { return c ? c->arcs : 0; }
+
+int hb_set_schedule(hb_ctx *c, uint32_t small_max_deg, uint32_t wave_max_deg, uint32_t chunk)
+{
+    if (!c) return fail(HB_EINVAL, "hb_set_schedule: ctx is NULL");
+    if (small_max_deg) c->small_max = small_max_deg;
+    if (wave_max_deg) c->wave_max = wave_max_deg;
+    if (chunk) c->chunk = chunk;
+    if (c->wave_max < c->small_max) c->wave_max = c->small_max;
+    c->mid_max = std::max(c->small_max, std::min<uint32_t>(c->wave_max, 8 * c->small_max));
+    c->sched_p = 0;  // rebuild on the next run
+    return HB_OK;
+}
+
+// Copies offsets/neighbours into HBM through two pinned staging buffers, converting
+// to u32 and validating on the way (every neighbour id < n, offsets monotone).
+int hb_stage_csr(hb_ctx *c, const void *offsets, int off_bytes, uint64_t n, const void *nbrs,
+                 int nbr_bytes, uint64_t arcs)
+{
+    if (!c) return fail(HB_EINVAL, "hb_stage_csr: ctx is NULL");
+    if (!offsets || (arcs && !nbrs)) return fail(HB_EINVAL, "hb_stage_csr: NULL buffer");
+    if ((off_bytes != 4 && off_bytes != 8) || (nbr_bytes != 4 && nbr_bytes != 8))
+        return fail(HB_EINVAL, "hb_stage_csr: element sizes must be 4 or 8 bytes");
+    if (n >= (1ull << 32) - 1 || arcs >= (1ull << 32))
+        return fail(HB_ERANGE, "hb_stage_csr: n=%llu arcs=%llu exceed the u32 device layout",
+                    (unsigned long long)n, (unsigned long long)arcs);
+    if (!set_device(c)) return fail(HB_EHIP, "hipSetDevice failed");
+    auto off_at = [&](uint64_t i) -> uint64_t {
+        return off_bytes == 8 ? ((const uint64_t *)offsets)[i] : ((const uint32_t *)offsets)[i];
+    };
+    if (off_at(0) != 0 || off_at(n) != arcs)
+        return fail(HB_EINVAL, "hb_stage_csr: offsets[0]=%llu offsets[n]=%llu, expected 0 and %llu",
+                    (unsigned long long)off_at(0), (unsigned long long)off_at(n),
+                    (unsigned long long)arcs);
+    std::vector<uint32_t> h_off(n + 1);
+    for (uint64_t i = 0; i <= n; ++i) {
+        const uint64_t o = off_at(i);
+        if (i && o < h_off[i - 1])
+            return fail(HB_EINVAL, "hb_stage_csr: offsets decrease at node %llu", (unsigned long long)i - 1);
+        h_off[i] = (uint32_t)o;
+    }
+    c->release_run();
+    c->n = 0;
+    c->arcs = 0;
+    HB_TRY(c->off.ensure(n + 1));
+    HB_TRY(c->nbr.ensure(arcs));
+    HB_TRY(hipMemcpyAsync(c->off.ptr, h_off.data(), (n + 1) * 4, hipMemcpyHostToDevice, c->stream));
+
+    const size_t kStage = size_t(16) << 20;  // u32 elements per staging buffer (64 MiB)
+    uint32_t *pinned[2] = {nullptr, nullptr};
+    hipEvent_t done[2] = {nullptr, nullptr};
+    int rc = HB_OK;
+    for (int b = 0; b < 2 && rc == HB_OK; ++b) {
+        if (hipHostMalloc((void **)&pinned[b], kStage * 4, hipHostMallocDefault) != hipSuccess)
+            rc = fail(HB_ENOMEM, "hipHostMalloc of %zu bytes failed", kStage * 4);
+        else if (hipEventCreateWithFlags(&done[b], hipEventDisableTiming) != hipSuccess)
+            rc = fail(HB_EHIP, "hipEventCreate failed");
+    }
+    uint64_t pos = 0;
+    int b = 0;
+    while (rc == HB_OK && pos < arcs) {
+        const uint64_t len = std::min<uint64_t>(kStage, arcs - pos);
+        if (hipEventSynchronize(done[b]) != hipSuccess) {
+            rc = fail(HB_EHIP, "hipEventSynchronize failed");
+            break;
+        }
+        uint32_t *dst = pinned[b];
+        uint64_t bad = 0, bad_at = 0;
+        if (nbr_bytes == 8) {
+            const uint64_t *src = (const uint64_t *)nbrs + pos;
+            for (uint64_t i = 0; i < len; ++i) {
+                const uint64_t v = src[i];
+                if (v >= n && !bad) { bad = v + 1; bad_at = pos + i; }
+                dst[i] = (uint32_t)v;
+            }
+        } else {
+            const uint32_t *src = (const uint32_t *)nbrs + pos;
+            for (uint64_t i = 0; i < len; ++i) {
+                const uint32_t v = src[i];
+                if (v >= n && !bad) { bad = (uint64_t)v + 1; bad_at = pos + i; }
+                dst[i] = v;
+            }
+        }
+        if (bad) {
+            rc = fail(HB_EINVAL, "hb_stage_csr: neighbour %llu at arc %llu is not < n=%llu",
+                      (unsigned long long)(bad - 1), (unsigned long long)bad_at, (unsigned long long)n);
+            break;
+        }
+        if (hipMemcpyAsync(c->nbr.ptr + pos, dst, len * 4, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+            hipEventRecord(done[b], c->stream) != hipSuccess) {
+            rc = fail(HB_EHIP, "staging copy failed");
+            break;
+        }
+        pos += len;
+        b ^= 1;
+    }
+    (void)hipStreamSynchronize(c->stream);
+    for (int i = 0; i < 2; ++i) {
+        if (pinned[i]) (void)hipHostFree(pinned[i]);
+        if (done[i]) (void)hipEventDestroy(done[i]);
+    }
+    if (rc != HB_OK) return rc;
+    c->h_off.swap(h_off);
+    c->n = n;
+    c->arcs = arcs;
+    g_err.clear();
+    return HB_OK;
+}
+
+}  // extern "C"
+
+// Degree-binned schedule: bins of team width G (deg <= small_max), 4G (<= mid_max)
+// and 64 (<= wave_max); higher-degree nodes are split into chunk-sized wavefront
+// tasks merged per hub.
+static int build_schedule(hb_ctx *c, int p)
+{
+    const uint32_t key[4] = {c->small_max, c->mid_max, c->wave_max, c->chunk};
+    if (c->sched_p == (uint32_t)p && !memcmp(key, c->sched_key, sizeof key)) return HB_OK;
+    const uint32_t G = std::min(64u, (1u << p) / 16u);
+    const uint32_t n = (uint32_t)c->n;
+    std::vector<uint32_t> bins[3], tnode, tbeg, tend, hnode, htask0, hntask;
+    for (uint32_t u = 0; u < n; ++u) {
+        const uint32_t d = c->h_off[u + 1] - c->h_off[u];
+        if (d <= c->small_max) bins[0].push_back(u);
+        else if (d <= c->mid_max) bins[1].push_back(u);
+        else if (d <= c->wave_max) bins[2].push_back(u);
+        else {
+            hnode.push_back(u);
+            htask0.push_back((uint32_t)tnode.size());
+            uint32_t cnt = 0;
+            for (uint32_t e = c->h_off[u]; e < c->h_off[u + 1]; e += c->chunk, ++cnt) {
+                tnode.push_back(u);
+                tbeg.push_back(e);
+                tend.push_back(std::min(e + c->chunk, c->h_off[u + 1]));
+            }
+            hntask.push_back(cnt);
+        }
+    }
+    const size_t total = bins[0].size() + bins[1].size() + bins[2].size() + 3 * tnode.size() +
+                         3 * hnode.size();
+    HB_TRY(c->sched_buf.ensure(total));
+    std::vector<uint32_t> host;
+    host.reserve(total);
+    hb::HbSchedule s;
+    size_t at = 0;
+    const int teams[3] = {(int)G, (int)std::min(64u, 4 * G), 64};
+    for (int b = 0; b < 3; ++b) {
+        s.bin_list[s.nbins] = c->sched_buf.ptr + at;
+        s.bin_count[s.nbins] = (uint32_t)bins[b].size();
+        s.bin_team[s.nbins] = teams[b];
+        ++s.nbins;
+        host.insert(host.end(), bins[b].begin(), bins[b].end());
+        at += bins[b].size();
+    }
+    auto put = [&](const std::vector<uint32_t> &v) -> const uint32_t * {
+        const uint32_t *ptr = c->sched_buf.ptr + at;
+        host.insert(host.end(), v.begin(), v.end());
+        at += v.size();
+        return ptr;
+    };
+    s.ntasks = (uint32_t)tnode.size();
+    s.task_node = put(tnode);
+    s.task_beg = put(tbeg);
+    s.task_end = put(tend);
+    s.nhubs = (uint32_t)hnode.size();
+    s.hub_node = put(hnode);
+    s.hub_task0 = put(htask0);
+    s.hub_ntask = put(hntask);
+    if (!host.empty())
+        HB_TRY(hipMemcpyAsync(c->sched_buf.ptr, host.data(), host.size() * 4, hipMemcpyHostToDevice,
+                              c->stream));
+    HB_TRY(c->scratch.ensure((size_t)s.ntasks << p));
+    HB_TRY(c->chunk_flag.ensure(s.ntasks));
+    HB_TRY(hipStreamSynchronize(c->stream));
+    c->sched = s;
+    c->sched_p = (uint32_t)p;
+    memcpy(c->sched_key, key, sizeof key);
+    return HB_OK;
+}
+
+static float alpha_f32(uint32_t m)
+{
+    if (m == 16) return 0.673f;
+    if (m == 32) return 0.697f;
+    if (m == 64) return 0.709f;
+    return 0.7213f / (1.0f + 1.079f / (float)m);
+}
+
+extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats *stats)
+{
+    if (!c) return fail(HB_EINVAL, "hb_run: ctx is NULL");
+    if (p < 4 || p > 16) return fail(HB_EINVAL, "hb_run: precision p=%d outside 4..16", p);
+    if (p > 10) return fail(HB_ERANGE, "hb_run: precision p=%d not supported by this build (4..10)", p);
+    if (max_iters < 0) return fail(HB_EINVAL, "hb_run: max_iters < 0");
+    if (!c->off.ptr) return fail(HB_ESTATE, "hb_run: no graph staged");
+    if (!set_device(c)) return fail(HB_EHIP, "hipSetDevice failed");
+    const uint32_t n = (uint32_t)c->n;
+    const uint32_t m = 1u << p;
+    const uint64_t words = (c->n + 31) / 32;
+    c->ran = false;
+    if (n == 0) {
+        // no nodes: the reference's loop runs one pass with no increase and stops
+        c->h_st = HbIterState{};
+        c->h_st.executed = 1;
+        c->h_st.converged = 1;
+        c->h_rec.assign(2, HbIterRec{});
+        c->p = p;
+        c->ran = true;
+        if (stats) {
+            memset(stats, 0, sizeof *stats);
+            stats->iterations = 1;
+            stats->converged = 1;
+            stats->p = p;
+        }
+        g_err.clear();
+        return HB_OK;
+    }
+
+    HB_TRY(c->regs0.ensure((size_t)c->n << p));
+    HB_TRY(c->regs1.ensure((size_t)c->n << p));
+    HB_TRY(c->est.ensure(c->n));
+    HB_TRY(c->dist.ensure(c->n));
+    HB_TRY(c->inv.ensure(c->n));
+    HB_TRY(c->bits0.ensure(words));
+    HB_TRY(c->bits1.ensure(words));
+    HB_TRY(c->st.ensure(1));
+    const uint32_t cap = max_iters > 0 ? (uint32_t)max_iters + 2
+                                       : (uint32_t)std::min<uint64_t>(c->n + 3, 1u << 16);
+    HB_TRY(c->rec.ensure(cap));
+    c->rec_cap = cap;
+    HB_TRY(c->lc.ensure(m + 1));
+    HB_TRY(c->bias.ensure(2 * HB_BIAS_K));
+    {
+        int rc = build_schedule(c, p);
+        if (rc != HB_OK) return rc;
+    }
+
+    // host-computed constants: LC table (f64 then f32, as m ln(m/z)), alpha m^2, bias points
+    std::vector<float> lc(m + 1, 0.0f);
+    for (uint32_t z = 1; z <= m; ++z) lc[z] = (float)((double)m * std::log((double)m / (double)z));
+    HB_TRY(hipMemcpyAsync(c->lc.ptr, lc.data(), (m + 1) * 4, hipMemcpyHostToDevice, c->stream));
+    HB_TRY(hipMemcpyAsync(c->bias.ptr, HB_BIAS_RAW[p - HB_BIAS_PMIN], HB_BIAS_K * 8,
+                          hipMemcpyHostToDevice, c->stream));
+    HB_TRY(hipMemcpyAsync(c->bias.ptr + HB_BIAS_K, HB_BIAS_VAL[p - HB_BIAS_PMIN], HB_BIAS_K * 8,
+                          hipMemcpyHostToDevice, c->stream));
+
+    HbDev &d = c->dev;
+    d = HbDev{};
+    d.off = c->off.ptr;
+    d.nbr = c->nbr.ptr;
+    d.regs0 = c->regs0.ptr;
+    d.regs1 = c->regs1.ptr;
+    d.est = c->est.ptr;
+    d.dist = c->dist.ptr;
+    d.inv = c->inv.ptr;
+    d.bits0 = c->bits0.ptr;
+    d.bits1 = c->bits1.ptr;
+    d.st = c->st.ptr;
+    d.rec = c->rec.ptr;
+    d.lc = c->lc.ptr;
+    d.bias_raw = c->bias.ptr;
+    d.bias_val = c->bias.ptr + HB_BIAS_K;
+    d.scratch = c->scratch.ptr;
+    d.chunk_flag = c->chunk_flag.ptr;
+    d.n = n;
+    d.p = (uint32_t)p;
+    const float mm = (float)m * (float)m;
+    d.am = alpha_f32(m) * mm;
+    d.five_m = 5.0f * (float)m;
+    d.lc_thr = kLcThreshold[p];
+    d.frontier = (flags & HB_FLAG_DENSE) ? 0 : 1;
+
+    HbIterState st0{};
+    st0.iter = 1;
+    st0.max_iters = (uint32_t)max_iters;
+    st0.t = 1.0;
+    st0.inv_t = 1.0;
+    HB_TRY(hipMemcpyAsync(c->st.ptr, &st0, sizeof st0, hipMemcpyHostToDevice, c->stream));
+    HB_TRY(hipMemsetAsync(c->rec.ptr, 0, (size_t)cap * sizeof(HbIterRec), c->stream));
+    HB_TRY(hb::launch_fill(c->bits0.ptr, words, 0u, c->stream));
+    HB_TRY(hb::launch_fill(c->bits1.ptr, words, 0u, c->stream));
+
+    hipEvent_t e0, e1, e2;
+    HB_TRY(hipEventCreate(&e0));
+    HB_TRY(hipEventCreate(&e1));
+    HB_TRY(hipEventCreate(&e2));
+    std::vector<float> iter_ms;
+    int rc = HB_OK;
+    hipGraphExec_t gexec = nullptr;
+    hipGraph_t graph = nullptr;
+    const bool nf = (flags & HB_FLAG_NF) != 0;
+    const bool ktiming = (flags & HB_FLAG_KTIMING) != 0;
+    const bool timing = ktiming || (flags & HB_FLAG_TIMING) != 0;
+    if (ktiming && (flags & HB_FLAG_GRAPH)) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipEventDestroy(e2);
+        return fail(HB_EINVAL, "hb_run: HB_FLAG_KTIMING cannot be combined with HB_FLAG_GRAPH");
+    }
+    hipEvent_t kev[2 * HB_NSLOT] = {};
+    double kms[HB_NSLOT] = {};
+    uint64_t klaunch[HB_NSLOT] = {};
+    if (ktiming)
+        for (auto &ev : kev) (void)hipEventCreate(&ev);
+    do {
+        if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fail(HB_EHIP, "event record failed"); break; }
+        hipError_t e = hb::launch_init(d, c->stream);
+        if (e != hipSuccess) { rc = fail(HB_EHIP, "init kernel: %s", hipGetErrorString(e)); break; }
+        if (hipEventRecord(e1, c->stream) != hipSuccess) { rc = fail(HB_EHIP, "event record failed"); break; }
+        if (flags & HB_FLAG_GRAPH) {
+            if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+                rc = fail(HB_EHIP, "hipStreamBeginCapture failed");
+                break;
+            }
+            e = hb::launch_iteration(d, c->sched, nf, c->stream);
+            hipError_t e2c = hipStreamEndCapture(c->stream, &graph);
+            if (e != hipSuccess || e2c != hipSuccess) {
+                rc = fail(HB_EHIP, "graph capture failed: %s", hipGetErrorString(e != hipSuccess ? e : e2c));
+                break;
+            }
+            if (hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0) != hipSuccess) {
+                rc = fail(HB_EHIP, "hipGraphInstantiate failed");
+                break;
+            }
+        }
+        // iterations: launches after convergence return at once (done flag), so a
+        // batch can run ahead of the host's check
+        uint32_t launched = 0;
+        HbIterState hs{};
+        for (;;) {
+            const uint32_t batch = timing ? 1u : (launched < 8 ? 2u : 8u);
+            const uint32_t room = cap - 2 - launched;
+            if (room == 0) { rc = fail(HB_ERANGE, "hb_run: more than %u iterations", cap - 2); break; }
+            const uint32_t k = std::min(batch, room);
+            hipEvent_t ib = nullptr, ie = nullptr;
+            if (timing) {
+                (void)hipEventCreate(&ib);
+                (void)hipEventCreate(&ie);
+                (void)hipEventRecord(ib, c->stream);
+            }
+            for (uint32_t i = 0; i < k && rc == HB_OK; ++i) {
+                e = gexec ? hipGraphLaunch(gexec, c->stream)
+                          : hb::launch_iteration(d, c->sched, nf, c->stream, ktiming ? kev : nullptr);
+                if (e != hipSuccess) rc = fail(HB_EHIP, "iteration launch: %s", hipGetErrorString(e));
+            }
+            launched += k;
+            if (timing) {
+                (void)hipEventRecord(ie, c->stream);
+            }
+            if (rc != HB_OK) break;
+            if (hipMemcpyAsync(&hs, c->st.ptr, sizeof hs, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+                hipStreamSynchronize(c->stream) != hipSuccess) {
+                rc = fail(HB_EHIP, "iteration failed: %s", hipGetErrorString(hipGetLastError()));
+                break;
+            }
+            if (timing) {
+                float ms = 0.0f;
+                (void)hipEventElapsedTime(&ms, ib, ie);
+                const bool ran_now = !hs.done || hs.executed == launched;
+                if (ran_now) iter_ms.push_back(ms);
+                (void)hipEventDestroy(ib);
+                (void)hipEventDestroy(ie);
+                if (ktiming && ran_now) {
+                    const uint32_t counts[HB_NSLOT] = {c->sched.bin_count[0], c->sched.bin_count[1],
+                                                       c->sched.bin_count[2], c->sched.ntasks, c->sched.nhubs};
+                    for (int sl = 0; sl < HB_NSLOT; ++sl) {
+                        if (!counts[sl]) continue;
+                        float kms1 = 0.0f;
+                        (void)hipEventElapsedTime(&kms1, kev[2 * sl], kev[2 * sl + 1]);
+                        kms[sl] += kms1;
+                        klaunch[sl] += 1;
+                    }
+                }
+            }
+            if (hs.done) break;
+        }
+        if (rc != HB_OK) break;
+        if (hipEventRecord(e2, c->stream) != hipSuccess || hipEventSynchronize(e2) != hipSuccess) {
+            rc = fail(HB_EHIP, "event sync failed");
+            break;
+        }
+        c->h_st = hs;
+    } while (0);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (ktiming)
+        for (auto &ev : kev) (void)hipEventDestroy(ev);
+    float ms_init = 0.0f, ms_loop = 0.0f;
+    if (rc == HB_OK) {
+        (void)hipEventElapsedTime(&ms_init, e0, e1);
+        (void)hipEventElapsedTime(&ms_loop, e1, e2);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    if (rc != HB_OK) return rc;
+
+    const uint32_t executed = c->h_st.executed;
+    c->h_rec.assign(executed + 1, HbIterRec{});
+    HB_TRY(hipMemcpy(c->h_rec.data(), c->rec.ptr, (executed + 1) * sizeof(HbIterRec), hipMemcpyDeviceToHost));
+    c->p = p;
+    c->ran = true;
+    if (stats) {
+        memset(stats, 0, sizeof *stats);
+        stats->iterations = (int32_t)executed;
+        stats->converged = (int32_t)c->h_st.converged;
+        // the reference's counters file is buffer 0; the last iteration wrote buffer parity^1
+        stats->final_parity_latest = (c->h_st.parity ^ 1u) == 0u;
+        stats->p = p;
+        stats->ms_total = ms_loop;
+        stats->ms_init = ms_init;
+        const uint32_t items[HB_NSLOT] = {c->sched.bin_count[0], c->sched.bin_count[1],
+                                          c->sched.bin_count[2], c->sched.ntasks, c->sched.nhubs};
+        for (int sl = 0; sl < HB_NSLOT; ++sl) {
+            stats->kernel_launches[sl] = ktiming ? klaunch[sl] : (items[sl] ? executed : 0);
+            stats->kernel_ms[sl] = kms[sl];
+            stats->kernel_items[sl] = items[sl];
+            stats->kernel_team[sl] = sl < 3 ? (uint32_t)c->sched.bin_team[sl] : 64u;
+        }
+        for (uint32_t t = 1; t <= executed; ++t) {
+            const HbIterRec &r = c->h_rec[t];
+            uint64_t inc = 0, arcs = 0, nodes = 0;
+            for (int sl = 0; sl < HB_NSLOT; ++sl) {
+                inc += r.inc[sl];
+                arcs += r.arcs[sl];
+                nodes += r.nodes[sl];
+                stats->kernel_arcs[sl] += r.arcs[sl];
+                stats->kernel_nodes[sl] += r.nodes[sl];
+            }
+            stats->arcs_gathered += arcs;
+            stats->nodes_recomputed += nodes;
+            if (t <= HB_STATS_MAX_ITERS) {
+                stats->increases[t - 1] = inc;
+                stats->iter_arcs[t - 1] = arcs;
+                stats->iter_nodes[t - 1] = nodes;
+                if (t - 1 < iter_ms.size()) stats->iter_ms[t - 1] = iter_ms[t - 1];
+                if (nf) {
+                    const unsigned __int128 v = ((unsigned __int128)r.nf_hi << 64) | r.nf_lo;
+                    stats->nf[t - 1] = std::ldexp((double)v, -40);
+                }
+            }
+        }
+    }
+    g_err.clear();
+    return HB_OK;
+}
+
+static const uint8_t *select_regs(hb_ctx *c, int which)
+{
+    if (which == HB_REGS_REFERENCE) return c->regs0.ptr;
+    return (c->h_st.parity ^ 1u) ? c->regs1.ptr : c->regs0.ptr;
+}
+
+extern "C" int hb_read_registers(hb_ctx *c, int which, uint8_t *dst)
+{
+    if (!c || !dst) return fail(HB_EINVAL, "hb_read_registers: NULL argument");
+    if (!c->ran) return fail(HB_ESTATE, "hb_read_registers: no completed run");
+    if (which != HB_REGS_REFERENCE && which != HB_REGS_LATEST)
+        return fail(HB_EINVAL, "hb_read_registers: bad selector %d", which);
+    if (c->n == 0) return HB_OK;
+    if (!set_device(c)) return fail(HB_EHIP, "hipSetDevice failed");
+    HB_TRY(hipMemcpy(dst, select_regs(c, which), (size_t)c->n << c->p, hipMemcpyDeviceToHost));
+    g_err.clear();
+    return HB_OK;
+}
+
+extern "C" int hb_read_estimates(hb_ctx *c, int which, float *dst)
+{
+    if (!c || !dst) return fail(HB_EINVAL, "hb_read_estimates: NULL argument");
+    if (!c->ran) return fail(HB_ESTATE, "hb_read_estimates: no completed run");
+    if (which != HB_REGS_REFERENCE && which != HB_REGS_LATEST)
+        return fail(HB_EINVAL, "hb_read_estimates: bad selector %d", which);
+    if (c->n == 0) return HB_OK;
+    if (!set_device(c)) return fail(HB_EHIP, "hipSetDevice failed");
+    HB_TRY(c->est_tmp.ensure(c->n));
+    HB_TRY(hb::launch_estimates(c->dev, select_regs(c, which), c->est_tmp.ptr, c->stream));
+    HB_TRY(hipMemcpyAsync(dst, c->est_tmp.ptr, c->n * 4, hipMemcpyDeviceToHost, c->stream));
+    HB_TRY(hipStreamSynchronize(c->stream));
+    g_err.clear();
+    return HB_OK;
+}
+
+extern "C" int hb_read_accumulators(hb_ctx *c, double *dist, double *inv)
+{
+    if (!c) return fail(HB_EINVAL, "hb_read_accumulators: ctx is NULL");
+    if (!c->ran) return fail(HB_ESTATE, "hb_read_accumulators: no completed run");
+    if (c->n == 0) return HB_OK;
+    if (!set_device(c)) return fail(HB_EHIP, "hipSetDevice failed");
+    if (dist) HB_TRY(hipMemcpy(dist, c->dist.ptr, c->n * 8, hipMemcpyDeviceToHost));
+    if (inv) HB_TRY(hipMemcpy(inv, c->inv.ptr, c->n * 8, hipMemcpyDeviceToHost));
+    g_err.clear();
+    return HB_OK;
+}
+
+extern "C" int hb_centrality(hb_ctx *c, int kind, int normalize, double *dst)
+{
+    if (!c || !dst) return fail(HB_EINVAL, "hb_centrality: NULL argument");
+    if (kind < HB_CLOSENESS || kind > HB_LIN) return fail(HB_EINVAL, "hb_centrality: bad kind %d", kind);
+    if (!c->ran) return fail(HB_ESTATE, "hb_centrality: no completed run");
+    if (c->n == 0) return HB_OK;
+    if (!set_device(c)) return fail(HB_EHIP, "hipSetDevice failed");
+    HB_TRY(c->est_tmp.ensure(c->n));
+    HB_TRY(c->cent.ensure(c->n));
+    // the epilogues re-estimate the reference's counters buffer (hyper_ball.rs:452,506,544)
+    HB_TRY(hb::launch_estimates(c->dev, c->regs0.ptr, c->est_tmp.ptr, c->stream));
+    HB_TRY(hb::launch_centrality(c->dev, c->est_tmp.ptr, kind, normalize ? 1 : 0, c->cent.ptr, c->stream));
+    HB_TRY(hipMemcpyAsync(dst, c->cent.ptr, c->n * 8, hipMemcpyDeviceToHost, c->stream));
+    HB_TRY(hipStreamSynchronize(c->stream));
+    g_err.clear();
+    return HB_OK;
+}
+
+extern "C" int hb_neighbourhood_function(hb_ctx *c, double *dst, int cap)
+{
+    if (!c || (!dst && cap > 0)) return fail(HB_EINVAL, "hb_neighbourhood_function: NULL argument");
+    if (!c->ran) return fail(HB_ESTATE, "hb_neighbourhood_function: no completed run");
+    const int it = (int)c->h_st.executed;
+    const int k = std::min(it, cap);
+    for (int t = 1; t <= k; ++t) {
+        const HbIterRec &r = c->h_rec[t];
+        const unsigned __int128 v = ((unsigned __int128)r.nf_hi << 64) | r.nf_lo;
+        dst[t - 1] = std::ldexp((double)v, -40);
+    }
+    g_err.clear();
+    return k;
+}

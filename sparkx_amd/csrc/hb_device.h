@@ -1,0 +1,284 @@
+// hb_device.h -- device-side HyperLogLog arithmetic and shared kernel state for
+// the MI355X HyperBall engine (gfx950, wave64).
+//
+// Register layout in HBM: node-major u8 registers, node u's 2^p registers at
+// regs + u * 2^p, register j at byte j.  A "copy" of a sketch is held by
+// G = 2^p / 16 lanes, each lane owning 16 consecutive registers in one uint4
+// (p <= 10).  Every estimate is bit-identical to the reference's f32 value
+// for the same registers (see estimate_copy below).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define HB_WAVE 64
+
+// ---------------------------------------------------------------------------
+// Device state that survives across the launches of one run
+// ---------------------------------------------------------------------------
+struct HbIterState {
+    uint32_t iter;       // iteration t being executed (1-based)
+    uint32_t done;       // set by the finalize kernel; later launches return at once
+    uint32_t executed;   // iterations executed when done
+    uint32_t max_iters;  // 0: run to the reference's convergence rule
+    uint32_t parity;     // registers read from regs[parity], written to regs[parity^1]
+    uint32_t converged;
+    double t, inv_t;     // t and 1/t as f64 (hyper_ball.rs:768-769,842-843)
+};
+
+// kernel slots of one iteration: union bins 0..2, hub chunks, hub merge
+#define HB_NSLOT 5
+#define HB_SLOT_CHUNK 3
+#define HB_SLOT_MERGE 4
+
+struct HbIterRec {  // per-iteration counters, per kernel slot
+    unsigned long long inc[HB_NSLOT];    // nodes whose estimate increased
+    unsigned long long arcs[HB_NSLOT];   // neighbour sketches gathered
+    unsigned long long nodes[HB_NSLOT];  // nodes recomputed
+    unsigned long long nf_lo, nf_hi;     // N(t) * 2^40 as a 128-bit integer
+};
+
+struct HbDev {
+    const uint32_t *off;   // n+1
+    const uint32_t *nbr;   // arcs
+    uint8_t *regs0, *regs1;
+    float *est;            // estimate of each node's current sketch
+    double *dist, *inv;    // accumulators
+    uint32_t *bits0, *bits1;  // changed-node bitmaps (bit set: registers changed)
+    HbIterState *st;
+    HbIterRec *rec;        // indexed by t
+    const float *lc;       // lc[z] = (float)(m ln(m/z)), z = 1..m
+    const double *bias_raw, *bias_val;  // HLL++ points for this p
+    uint8_t *scratch;      // partial sketches of hub chunks
+    uint32_t *chunk_flag;  // 1 if the chunk was recomputed this iteration
+    uint32_t n;
+    uint32_t p;
+    float am;              // alpha_m * m * m, rounded to f32
+    float five_m;          // 5m
+    double lc_thr;         // HLL++ linear-counting threshold for p
+    int frontier;
+};
+
+// ---------------------------------------------------------------------------
+// SipHash-1-3 of an 8-byte little-endian message, key (0,0)
+// (Aumasson & Bernstein; Rust's `usize: Hash` writes the value's 8 LE bytes)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t hb_rotl(uint64_t x, int b) { return (x << b) | (x >> (64 - b)); }
+
+__device__ __forceinline__ uint64_t hb_siphash13_u64(uint64_t msg)
+{
+    uint64_t v0 = 0x736f6d6570736575ULL, v1 = 0x646f72616e646f6dULL;
+    uint64_t v2 = 0x6c7967656e657261ULL, v3 = 0x7465646279746573ULL;
+#define HB_SIPROUND                                                              \
+    v0 += v1; v1 = hb_rotl(v1, 13); v1 ^= v0; v0 = hb_rotl(v0, 32);             \
+    v2 += v3; v3 = hb_rotl(v3, 16); v3 ^= v2;                                    \
+    v0 += v3; v3 = hb_rotl(v3, 21); v3 ^= v0;                                    \
+    v2 += v1; v1 = hb_rotl(v1, 17); v1 ^= v2; v2 = hb_rotl(v2, 32);
+    v3 ^= msg;
+    HB_SIPROUND
+    v0 ^= msg;
+    const uint64_t b = 8ULL << 56;  // length byte, no tail bytes
+    v3 ^= b;
+    HB_SIPROUND
+    v0 ^= b;
+    v2 ^= 0xff;
+    HB_SIPROUND HB_SIPROUND HB_SIPROUND
+#undef HB_SIPROUND
+    return v0 ^ v1 ^ v2 ^ v3;
+}
+
+// Register set by insert(x): index = top p bits of the hash, rank = 1 + leading
+// zeros of (hash << p) with a sentinel bit at p-1 (rank <= 65-p < 64).
+__device__ __forceinline__ void hb_index_rank(uint64_t x, uint32_t p, uint32_t *idx, uint32_t *rank)
+{
+    const uint64_t h = hb_siphash13_u64(x);
+    *idx = (uint32_t)(h >> (64 - p));
+    const uint64_t w = (h << p) | (1ULL << (p - 1));
+    *rank = (uint32_t)__clzll((long long)w) + 1;
+}
+
+// ---------------------------------------------------------------------------
+// Bytewise max of packed u8 registers with v_pk_max_u16: the even bytes and the
+// odd bytes of a word are maxed as two 16-bit lanes each (the other byte of every
+// lane masked to zero), then recombined.
+// ---------------------------------------------------------------------------
+typedef unsigned short hb_u16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ uint32_t hb_pkmax(uint32_t a, uint32_t b)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(hb_u16x2, a),
+                                                                   __builtin_bit_cast(hb_u16x2, b)));
+}
+
+__device__ __forceinline__ uint32_t hb_bmax(uint32_t a, uint32_t b)
+{
+    return hb_pkmax(a & 0x00ff00ffu, b & 0x00ff00ffu) | hb_pkmax(a & 0xff00ff00u, b & 0xff00ff00u);
+}
+
+__device__ __forceinline__ uint4 hb_bmax4(uint4 a, uint4 b)
+{
+    return make_uint4(hb_bmax(a.x, b.x), hb_bmax(a.y, b.y), hb_bmax(a.z, b.z), hb_bmax(a.w, b.w));
+}
+
+// Running register-wise max over many sketches, kept as even-byte and odd-byte
+// planes so that each gathered word costs two masks and two v_pk_max_u16.
+struct HbAcc {
+    uint32_t e[4], o[4];
+};
+
+__device__ __forceinline__ HbAcc hb_acc(uint4 x)
+{
+    HbAcc a;
+    a.e[0] = x.x & 0x00ff00ffu; a.o[0] = x.x & 0xff00ff00u;
+    a.e[1] = x.y & 0x00ff00ffu; a.o[1] = x.y & 0xff00ff00u;
+    a.e[2] = x.z & 0x00ff00ffu; a.o[2] = x.z & 0xff00ff00u;
+    a.e[3] = x.w & 0x00ff00ffu; a.o[3] = x.w & 0xff00ff00u;
+    return a;
+}
+
+__device__ __forceinline__ void hb_acc_max(HbAcc &a, uint4 x)
+{
+    a.e[0] = hb_pkmax(a.e[0], x.x & 0x00ff00ffu); a.o[0] = hb_pkmax(a.o[0], x.x & 0xff00ff00u);
+    a.e[1] = hb_pkmax(a.e[1], x.y & 0x00ff00ffu); a.o[1] = hb_pkmax(a.o[1], x.y & 0xff00ff00u);
+    a.e[2] = hb_pkmax(a.e[2], x.z & 0x00ff00ffu); a.o[2] = hb_pkmax(a.o[2], x.z & 0xff00ff00u);
+    a.e[3] = hb_pkmax(a.e[3], x.w & 0x00ff00ffu); a.o[3] = hb_pkmax(a.o[3], x.w & 0xff00ff00u);
+}
+
+__device__ __forceinline__ uint4 hb_acc_get(const HbAcc &a)
+{
+    return make_uint4(a.e[0] | a.o[0], a.e[1] | a.o[1], a.e[2] | a.o[2], a.e[3] | a.o[3]);
+}
+
+template <int W>
+__device__ __forceinline__ uint4 hb_shfl_xor4(uint4 a, int mask)
+{
+    return make_uint4(__shfl_xor(a.x, mask, W), __shfl_xor(a.y, mask, W),
+                      __shfl_xor(a.z, mask, W), __shfl_xor(a.w, mask, W));
+}
+
+// number of zero bytes (register values are < 128)
+__device__ __forceinline__ uint32_t hb_zero_bytes(uint32_t x)
+{
+    const uint32_t nz = ((x | 0x80808080u) - 0x01010101u) & 0x80808080u;
+    return 4u - (uint32_t)__popc(nz);
+}
+
+__device__ __forceinline__ uint32_t hb_max_byte(uint32_t x)
+{
+    x = hb_bmax(x, x >> 16);
+    x = hb_bmax(x, x >> 8);
+    return x & 0xffu;
+}
+
+// sum over the 4 bytes r of 2^(mx - r), each term clamped to 2^24
+__device__ __forceinline__ uint32_t hb_scaled_terms(uint32_t x, uint32_t mx)
+{
+    uint32_t s = 0;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const uint32_t sh = mx - ((x >> (8 * b)) & 0xffu);
+        s += 1u << (sh < 24u ? sh : 24u);
+    }
+    return s;
+}
+
+__device__ __forceinline__ float hb_pow2neg(uint32_t r) { return __uint_as_float((127u - r) << 23); }
+
+// reductions over the G lanes of one sketch copy (G a power of two <= 64)
+template <int G>
+__device__ __forceinline__ uint32_t hb_copy_sum(uint32_t v)
+{
+#pragma unroll
+    for (int s = 1; s < G; s <<= 1) v += __shfl_xor(v, s, G);
+    return v;
+}
+template <int G>
+__device__ __forceinline__ uint32_t hb_copy_max(uint32_t v)
+{
+#pragma unroll
+    for (int s = 1; s < G; s <<= 1) v = max(v, (uint32_t)__shfl_xor(v, s, G));
+    return v;
+}
+template <int G>
+__device__ __forceinline__ uint32_t hb_copy_sum_sat24(uint32_t v)
+{
+    v = min(v, 1u << 24);
+#pragma unroll
+    for (int s = 1; s < G; s <<= 1) v = min(v + (uint32_t)__shfl_xor(v, s, G), 1u << 24);
+    return v;
+}
+template <int G>
+__device__ __forceinline__ uint32_t hb_copy_or(uint32_t v)
+{
+#pragma unroll
+    for (int s = 1; s < G; s <<= 1) v |= __shfl_xor(v, s, G);
+    return v;
+}
+
+// HLL++ EstimateBias: mean of the biases of the 6 table points nearest to e
+// (ties to the lower point), summed in ascending order in f64.
+__device__ __forceinline__ double hb_knn_bias(const double *__restrict__ raw,
+                                              const double *__restrict__ val, double e, int K)
+{
+    int lo = 0, hi = K;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (raw[mid] > e) hi = mid;
+        else lo = mid + 1;
+    }
+    hi = lo;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        if (lo > 0 && (hi == K || e - raw[lo - 1] <= raw[hi] - e)) --lo;
+        else ++hi;
+    }
+    double s = 0.0;
+    for (int i = lo; i < hi; ++i) s = __dadd_rn(s, val[i]);
+    return s / 6.0;
+}
+
+// Estimate of the sketch held by a G-lane copy (each lane: 16 registers in acc).
+// All lanes of the copy return the same value.
+//   1. zeros > 0 and LC = m ln(m/zeros) <= threshold(p): return LC (HLL++).
+//   2. raw = alpha m^2 / S with S = sum_j 2^-M_j summed in register order in f32.
+//      S is computed exactly in integers: with mx = max_j M_j,
+//      k = sum_j 2^(mx - M_j); when k < 2^24 every partial sum of the sequential
+//      f32 loop is a multiple of 2^-mx below 2^(24-mx), hence exact, and
+//      S = k * 2^-mx.  Otherwise the sequential loop is replayed.
+//   3. raw <= 5m: subtract the HLL++ bias (k = 6 nearest neighbours).
+// f32 products and quotients are formed in f64 and rounded once, which gives the
+// correctly rounded f32 result (53 >= 2*24 + 2).
+template <int G>
+__device__ __forceinline__ float hb_estimate_copy(const HbDev &d, uint4 acc)
+{
+    uint32_t z = hb_zero_bytes(acc.x) + hb_zero_bytes(acc.y) + hb_zero_bytes(acc.z) +
+                 hb_zero_bytes(acc.w);
+    z = hb_copy_sum<G>(z);
+    if (z > 0) {
+        const float lc = d.lc[z];
+        if ((double)lc <= d.lc_thr) return lc;
+    }
+    uint32_t mx = hb_max_byte(hb_bmax(hb_bmax(acc.x, acc.y), hb_bmax(acc.z, acc.w)));
+    mx = hb_copy_max<G>(mx);
+    uint32_t k = hb_scaled_terms(acc.x, mx) + hb_scaled_terms(acc.y, mx) +
+                 hb_scaled_terms(acc.z, mx) + hb_scaled_terms(acc.w, mx);
+    k = hb_copy_sum_sat24<G>(k);
+    float S;
+    if (k < (1u << 24)) {
+        S = ldexpf((float)k, -(int)mx);
+    } else {
+        // replay the reference's sequential f32 loop over registers 0..m-1
+        S = 0.0f;
+        for (int l = 0; l < G; ++l) {
+            const uint32_t w[4] = {(uint32_t)__shfl(acc.x, l, G), (uint32_t)__shfl(acc.y, l, G),
+                                   (uint32_t)__shfl(acc.z, l, G), (uint32_t)__shfl(acc.w, l, G)};
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) S = __fadd_rn(S, hb_pow2neg((w[q] >> (8 * b)) & 0xffu));
+        }
+    }
+    float est = (float)((double)d.am / (double)S);
+    if (est <= d.five_m) est = (float)((double)est - hb_knn_bias(d.bias_raw, d.bias_val, (double)est, 200));
+    return est;
+}

@@ -1,0 +1,33 @@
+// hb_kernels.h -- launchers of the HyperBall kernels (hb_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "hb_device.h"
+
+namespace hb {
+
+// Degree-binned schedule of one run (device pointers).
+struct HbSchedule {
+    int nbins = 0;
+    const uint32_t *bin_list[4] = {};
+    uint32_t bin_count[4] = {};
+    int bin_team[4] = {};              // lanes per node in this bin
+    const uint32_t *task_node = nullptr, *task_beg = nullptr, *task_end = nullptr;
+    uint32_t ntasks = 0;               // hub neighbour chunks
+    const uint32_t *hub_node = nullptr, *hub_task0 = nullptr, *hub_ntask = nullptr;
+    uint32_t nhubs = 0;
+    unsigned max_blocks = 8192;        // grid cap of the persistent loops
+};
+
+hipError_t launch_init(const HbDev &d, hipStream_t s);
+hipError_t launch_fill(uint32_t *p, uint64_t count, uint32_t value, hipStream_t s);
+// ev (nullable): 2 * HB_NSLOT events recorded around each kernel slot that launches
+hipError_t launch_iteration(const HbDev &d, const HbSchedule &sc, bool nf, hipStream_t s,
+                            hipEvent_t *ev = nullptr);
+hipError_t launch_estimates(const HbDev &d, const uint8_t *regs, float *out, hipStream_t s);
+hipError_t launch_centrality(const HbDev &d, const float *est, int kind, int normalize, double *out,
+                             hipStream_t s);
+
+}  // namespace hb

@@ -1,0 +1,504 @@
+// hb_kernels.hip -- HyperBall iteration kernels for gfx950 (no MFMA: the union
+// is a bytewise max, HBM-bound).
+//
+// One iteration t (hyper_ball.rs:754-874, restated per node u):
+//     a = c_{t-1}(u); for v in N(u): a |= c_{t-1}(v)       (register-wise max)
+//     diff = f64(est(a) - est(c_{t-1}(u)))  (f32 subtraction)
+//     if diff > 0 { dist[u] += t*diff; inv[u] += (1/t)*diff; increases++ }
+//     c_t(u) = a
+// Launch sequence per iteration, all reading the device-resident HbIterState so
+// that one captured hipGraph serves every iteration:
+//     k_begin (clear next changed-bitmap)  ->  k_union<P,T> per degree bin
+//     ->  k_chunk<P> (hub neighbour chunks) -> k_merge<P> (hubs) -> [k_nf] -> k_finalize
+// Frontier: if no neighbour of u changed at t-1 then c_t(u) = c_{t-1}(u) (union
+// of unchanged sets), so u is skipped; when u itself changed at t-1 its row is
+// copied into the output buffer so both buffers stay complete.  Results are
+// identical to recomputing every node.
+#include "hb_device.h"
+#include "hb_kernels.h"
+
+namespace hb {
+
+static constexpr int kBlock = 256;
+
+__device__ __forceinline__ bool bit_test(const uint32_t *bits, uint32_t v)
+{
+    return (bits[v >> 5] >> (v & 31u)) & 1u;
+}
+
+// block-wide sum of three per-lane counters, one atomic per counter per block
+__device__ __forceinline__ void block_add3(unsigned long long a, unsigned long long b,
+                                           unsigned long long c, HbIterRec *rec, int slot)
+{
+    __shared__ unsigned long long red[3][kBlock / HB_WAVE];
+#pragma unroll
+    for (int s = 1; s < HB_WAVE; s <<= 1) {
+        a += __shfl_xor(a, s);
+        b += __shfl_xor(b, s);
+        c += __shfl_xor(c, s);
+    }
+    const int w = threadIdx.x / HB_WAVE;
+    if ((threadIdx.x & (HB_WAVE - 1)) == 0) {
+        red[0][w] = a;
+        red[1][w] = b;
+        red[2][w] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long s0 = 0, s1 = 0, s2 = 0;
+        for (int i = 0; i < (int)(blockDim.x / HB_WAVE); ++i) {
+            s0 += red[0][i];
+            s1 += red[1][i];
+            s2 += red[2][i];
+        }
+        if (s0) atomicAdd(&rec->inc[slot], s0);
+        if (s1) atomicAdd(&rec->arcs[slot], s1);
+        if (s2) atomicAdd(&rec->nodes[slot], s2);
+    }
+}
+
+// f64 accumulation without contraction (hyper_ball.rs:823-825)
+__device__ __forceinline__ bool accumulate(const HbDev &d, uint32_t u, float prev, float curr,
+                                           double t, double inv_t)
+{
+    const float d32 = __fsub_rn(curr, prev);
+    const double diff = (double)d32;
+    if (diff > 0.0) {
+        d.dist[u] = __dadd_rn(d.dist[u], __dmul_rn(t, diff));
+        d.inv[u] = __dadd_rn(d.inv[u], __dmul_rn(inv_t, diff));
+        return true;
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------------------
+// init: c_0(u) = {insert(u)} (hyper_ball.rs:716-721); est, accumulators
+// ---------------------------------------------------------------------------
+template <int P>
+__global__ __launch_bounds__(kBlock) void k_init(HbDev d)
+{
+    constexpr uint32_t M = 1u << P, G = M / 16;
+    const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t u = (uint32_t)(gtid / G), cl = (uint32_t)(gtid % G);
+    if (u >= d.n) return;  // whole copies leave together (G divides the block)
+    uint32_t idx, rank;
+    hb_index_rank(u, P, &idx, &rank);
+    uint32_t w[4] = {0, 0, 0, 0};
+    if (idx / 16 == cl) w[(idx % 16) / 4] = rank << (8 * (idx % 4));
+    const uint4 r = make_uint4(w[0], w[1], w[2], w[3]);
+    *(uint4 *)(d.regs0 + ((uint64_t)u << P) + cl * 16) = r;
+    const float e = hb_estimate_copy<G>(d, r);
+    if (cl == 0) {
+        d.est[u] = e;
+        d.dist[u] = 0.0;
+        d.inv[u] = 0.0;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t *p, uint64_t count, uint32_t value)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = value;
+}
+
+// start of iteration t: clear the bitmap iteration t writes (bits[t & 1])
+__global__ __launch_bounds__(kBlock) void k_begin(HbDev d, uint32_t words)
+{
+    if (d.st->done) return;
+    uint32_t *bw = (d.st->iter & 1u) ? d.bits1 : d.bits0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += gridDim.x * blockDim.x)
+        bw[i] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// union over the neighbours of listed nodes; a team of T lanes per node
+// (T = G: one sketch copy; T > G: S = T/G copies split the neighbours)
+// ---------------------------------------------------------------------------
+template <int P, int T>
+__global__ __launch_bounds__(kBlock) void k_union(HbDev d, const uint32_t *__restrict__ list,
+                                                  uint32_t count, int slot_id)
+{
+    constexpr uint32_t M = 1u << P, G = M / 16, S = T / G;
+    static_assert(T >= (int)G && T <= HB_WAVE, "team width");
+    const HbIterState *st = d.st;
+    if (st->done) return;
+    const uint32_t it = st->iter;
+    const double t = st->t, inv_t = st->inv_t;
+    const uint8_t *__restrict__ in = st->parity ? d.regs1 : d.regs0;
+    uint8_t *__restrict__ out = st->parity ? d.regs0 : d.regs1;
+    const uint32_t *__restrict__ bitsR = (it & 1u) ? d.bits0 : d.bits1;
+    uint32_t *bitsW = (it & 1u) ? d.bits1 : d.bits0;
+    const bool use_frontier = d.frontier && it > 1;
+
+    const uint32_t tl = threadIdx.x % T, cl = tl % G, slot = tl / G;
+    const uint32_t teams_per_grid = gridDim.x * (kBlock / T);
+    unsigned long long c_inc = 0, c_arcs = 0, c_nodes = 0;
+
+    for (uint32_t team = blockIdx.x * (kBlock / T) + threadIdx.x / T; team < count;
+         team += teams_per_grid) {
+        const uint32_t u = list[team];
+        const uint32_t e0 = d.off[u], deg = d.off[u + 1] - e0;
+        const uint32_t *__restrict__ nb = d.nbr + e0;
+        bool active = true;
+        if (use_frontier) {
+            uint32_t any = 0;
+            for (uint32_t j = tl; j < deg; j += T) any |= bit_test(bitsR, nb[j]);
+            active = hb_copy_or<T>(any) != 0;
+        }
+        const uint4 own = *(const uint4 *)(in + ((uint64_t)u << P) + cl * 16);
+        if (!active) {
+            if (slot == 0 && bit_test(bitsR, u))
+                *(uint4 *)(out + ((uint64_t)u << P) + cl * 16) = own;
+            continue;
+        }
+        HbAcc pa = hb_acc(own);
+        uint32_t j = slot;
+        for (; j + 3 * S < deg; j += 4 * S) {
+            const uint32_t v0 = nb[j], v1 = nb[j + S], v2 = nb[j + 2 * S], v3 = nb[j + 3 * S];
+            const uint4 x0 = *(const uint4 *)(in + ((uint64_t)v0 << P) + cl * 16);
+            const uint4 x1 = *(const uint4 *)(in + ((uint64_t)v1 << P) + cl * 16);
+            const uint4 x2 = *(const uint4 *)(in + ((uint64_t)v2 << P) + cl * 16);
+            const uint4 x3 = *(const uint4 *)(in + ((uint64_t)v3 << P) + cl * 16);
+            hb_acc_max(pa, x0);
+            hb_acc_max(pa, x1);
+            hb_acc_max(pa, x2);
+            hb_acc_max(pa, x3);
+        }
+        for (; j < deg; j += S) hb_acc_max(pa, *(const uint4 *)(in + ((uint64_t)nb[j] << P) + cl * 16));
+        uint4 acc = hb_acc_get(pa);
+#pragma unroll
+        for (uint32_t s = G; s < T; s <<= 1) acc = hb_bmax4(acc, hb_shfl_xor4<T>(acc, s));
+
+        const uint32_t diffw = (acc.x != own.x) | (acc.y != own.y) | (acc.z != own.z) | (acc.w != own.w);
+        const bool changed = hb_copy_or<G>(diffw) != 0;
+        const float curr = hb_estimate_copy<G>(d, acc);
+        if (slot == 0) *(uint4 *)(out + ((uint64_t)u << P) + cl * 16) = acc;
+        if (tl == 0) {
+            const float prev = d.est[u];
+            if (accumulate(d, u, prev, curr, t, inv_t)) c_inc += 1;
+            d.est[u] = curr;
+            if (changed) atomicOr(bitsW + (u >> 5), 1u << (u & 31u));
+            c_arcs += deg;
+            c_nodes += 1;
+        }
+    }
+    block_add3(c_inc, c_arcs, c_nodes, d.rec + it, slot_id);
+}
+
+// ---------------------------------------------------------------------------
+// hubs, phase 1: one wavefront per chunk of a hub's neighbour list writes the
+// chunk's partial union to scratch (not the hub's own registers)
+// ---------------------------------------------------------------------------
+template <int P>
+__global__ __launch_bounds__(kBlock) void k_chunk(HbDev d, const uint32_t *__restrict__ tnode,
+                                                  const uint32_t *__restrict__ tbeg,
+                                                  const uint32_t *__restrict__ tend, uint32_t ntasks)
+{
+    constexpr uint32_t M = 1u << P, G = M / 16, T = HB_WAVE, S = T / G;
+    const HbIterState *st = d.st;
+    if (st->done) return;
+    const uint32_t it = st->iter;
+    const uint8_t *__restrict__ in = st->parity ? d.regs1 : d.regs0;
+    const uint32_t *__restrict__ bitsR = (it & 1u) ? d.bits0 : d.bits1;
+    const bool use_frontier = d.frontier && it > 1;
+    const uint32_t tl = threadIdx.x % T, cl = tl % G, slot = tl / G;
+    const uint32_t waves_per_grid = gridDim.x * (kBlock / T);
+    unsigned long long c_arcs = 0;
+
+    for (uint32_t task = blockIdx.x * (kBlock / T) + threadIdx.x / T; task < ntasks;
+         task += waves_per_grid) {
+        (void)tnode;
+        const uint32_t eb = tbeg[task], ee = tend[task];
+        const uint32_t *__restrict__ nb = d.nbr + eb;
+        const uint32_t len = ee - eb;
+        bool active = true;
+        if (use_frontier) {
+            uint32_t any = 0;
+            for (uint32_t j = tl; j < len; j += T) any |= bit_test(bitsR, nb[j]);
+            active = hb_copy_or<T>(any) != 0;
+        }
+        if (tl == 0) d.chunk_flag[task] = active ? 1u : 0u;
+        if (!active) continue;
+        HbAcc pa = hb_acc(make_uint4(0, 0, 0, 0));
+        uint32_t j = slot;
+        for (; j + 3 * S < len; j += 4 * S) {
+            const uint32_t v0 = nb[j], v1 = nb[j + S], v2 = nb[j + 2 * S], v3 = nb[j + 3 * S];
+            const uint4 x0 = *(const uint4 *)(in + ((uint64_t)v0 << P) + cl * 16);
+            const uint4 x1 = *(const uint4 *)(in + ((uint64_t)v1 << P) + cl * 16);
+            const uint4 x2 = *(const uint4 *)(in + ((uint64_t)v2 << P) + cl * 16);
+            const uint4 x3 = *(const uint4 *)(in + ((uint64_t)v3 << P) + cl * 16);
+            hb_acc_max(pa, x0);
+            hb_acc_max(pa, x1);
+            hb_acc_max(pa, x2);
+            hb_acc_max(pa, x3);
+        }
+        for (; j < len; j += S) hb_acc_max(pa, *(const uint4 *)(in + ((uint64_t)nb[j] << P) + cl * 16));
+        uint4 acc = hb_acc_get(pa);
+#pragma unroll
+        for (uint32_t s = G; s < T; s <<= 1) acc = hb_bmax4(acc, hb_shfl_xor4<T>(acc, s));
+        if (slot == 0) *(uint4 *)(d.scratch + ((uint64_t)task << P) + cl * 16) = acc;
+        if (tl == 0) c_arcs += len;
+    }
+    block_add3(0, c_arcs, 0, d.rec + it, HB_SLOT_CHUNK);
+}
+
+// hubs, phase 2: one wavefront per hub merges its own registers with the
+// partials of its recomputed chunks (a chunk that was not recomputed has no
+// changed neighbour, so its union is already contained in c_{t-1}(u))
+template <int P>
+__global__ __launch_bounds__(kBlock) void k_merge(HbDev d, const uint32_t *__restrict__ hnode,
+                                                  const uint32_t *__restrict__ htask0,
+                                                  const uint32_t *__restrict__ hntask, uint32_t nhubs)
+{
+    constexpr uint32_t M = 1u << P, G = M / 16, T = HB_WAVE, S = T / G;
+    const HbIterState *st = d.st;
+    if (st->done) return;
+    const uint32_t it = st->iter;
+    const double t = st->t, inv_t = st->inv_t;
+    const uint8_t *__restrict__ in = st->parity ? d.regs1 : d.regs0;
+    uint8_t *__restrict__ out = st->parity ? d.regs0 : d.regs1;
+    const uint32_t *__restrict__ bitsR = (it & 1u) ? d.bits0 : d.bits1;
+    uint32_t *bitsW = (it & 1u) ? d.bits1 : d.bits0;
+    const uint32_t tl = threadIdx.x % T, cl = tl % G, slot = tl / G;
+    const uint32_t waves_per_grid = gridDim.x * (kBlock / T);
+    unsigned long long c_inc = 0, c_nodes = 0;
+
+    for (uint32_t h = blockIdx.x * (kBlock / T) + threadIdx.x / T; h < nhubs; h += waves_per_grid) {
+        const uint32_t u = hnode[h], t0 = htask0[h], nt = hntask[h];
+        uint32_t any = 0;
+        for (uint32_t j = tl; j < nt; j += T) any |= d.chunk_flag[t0 + j];
+        const bool active = hb_copy_or<T>(any) != 0;
+        const uint4 own = *(const uint4 *)(in + ((uint64_t)u << P) + cl * 16);
+        if (!active) {
+            if (slot == 0 && bit_test(bitsR, u))
+                *(uint4 *)(out + ((uint64_t)u << P) + cl * 16) = own;
+            continue;
+        }
+        uint4 acc = own;
+        for (uint32_t j = slot; j < nt; j += S)
+            if (d.chunk_flag[t0 + j])
+                acc = hb_bmax4(acc, *(const uint4 *)(d.scratch + ((uint64_t)(t0 + j) << P) + cl * 16));
+#pragma unroll
+        for (uint32_t s = G; s < T; s <<= 1) acc = hb_bmax4(acc, hb_shfl_xor4<T>(acc, s));
+        const uint32_t diffw = (acc.x != own.x) | (acc.y != own.y) | (acc.z != own.z) | (acc.w != own.w);
+        const bool changed = hb_copy_or<G>(diffw) != 0;
+        const float curr = hb_estimate_copy<G>(d, acc);
+        if (slot == 0) *(uint4 *)(out + ((uint64_t)u << P) + cl * 16) = acc;
+        if (tl == 0) {
+            const float prev = d.est[u];
+            if (accumulate(d, u, prev, curr, t, inv_t)) c_inc += 1;
+            d.est[u] = curr;
+            if (changed) atomicOr(bitsW + (u >> 5), 1u << (u & 31u));
+            c_nodes += 1;
+        }
+    }
+    block_add3(c_inc, 0, c_nodes, d.rec + it, HB_SLOT_MERGE);
+}
+
+// ---------------------------------------------------------------------------
+// neighbourhood function N(t) = sum_u est_t(u), summed exactly in 128-bit fixed
+// point with 40 fractional bits (every estimate is >= 1, whose f32 ulp >= 2^-23)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_nf(HbDev d)
+{
+    const HbIterState *st = d.st;
+    if (st->done) return;
+    unsigned long long lo = 0, hi = 0;
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < d.n; u += gridDim.x * blockDim.x) {
+        const uint32_t b = __float_as_uint(d.est[u]);
+        const int e = (int)((b >> 23) & 0xffu);
+        const unsigned long long mant = (e ? (1ull << 23) : 0ull) | (b & 0x7fffffu);
+        const int sh = (e ? e : 1) - 127 - 23 + 40;  // value * 2^40 = mant * 2^sh
+        unsigned long long vlo, vhi;
+        if (sh >= 64) { vlo = 0; vhi = mant << (sh - 64); }
+        else if (sh > 0) { vlo = mant << sh; vhi = sh > 40 ? (mant >> (64 - sh)) : 0ull; }
+        else { vlo = mant >> (-sh); vhi = 0; }
+        const unsigned long long nlo = lo + vlo;
+        hi += vhi + (nlo < lo);
+        lo = nlo;
+    }
+    // wave reduction with carries
+#pragma unroll
+    for (int s = 1; s < HB_WAVE; s <<= 1) {
+        const unsigned long long olo = __shfl_xor(lo, s), ohi = __shfl_xor(hi, s);
+        const unsigned long long nlo = lo + olo;
+        hi = hi + ohi + (nlo < lo);
+        lo = nlo;
+    }
+    if ((threadIdx.x & (HB_WAVE - 1)) == 0) {
+        HbIterRec *r = d.rec + st->iter;
+        const unsigned long long old = atomicAdd(&r->nf_lo, lo);
+        atomicAdd(&r->nf_hi, hi + ((old + lo) < old ? 1ull : 0ull));
+    }
+}
+
+// end of iteration t: termination rule (hyper_ball.rs:833-849) and buffer swap
+__global__ void k_finalize(HbDev d)
+{
+    HbIterState *st = d.st;
+    if (st->done || threadIdx.x != 0 || blockIdx.x != 0) return;
+    const uint32_t it = st->iter;
+    unsigned long long inc = 0;
+    for (int k = 0; k < HB_NSLOT; ++k) inc += d.rec[it].inc[k];
+    if (inc == 0 || (st->max_iters && it >= st->max_iters)) {
+        st->done = 1;
+        st->executed = it;
+        st->converged = inc == 0;
+        return;
+    }
+    st->iter = it + 1;
+    st->parity ^= 1u;
+    st->t = st->t + 1.0;
+    st->inv_t = __ddiv_rn(1.0, st->t);
+}
+
+// ---------------------------------------------------------------------------
+// epilogues
+// ---------------------------------------------------------------------------
+template <int P>
+__global__ __launch_bounds__(kBlock) void k_estimates(HbDev d, const uint8_t *__restrict__ regs,
+                                                      float *__restrict__ out)
+{
+    constexpr uint32_t M = 1u << P, G = M / 16;
+    const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t u = (uint32_t)(gtid / G), cl = (uint32_t)(gtid % G);
+    if (u >= d.n) return;
+    const uint4 r = *(const uint4 *)(regs + ((uint64_t)u << P) + cl * 16);
+    const float e = hb_estimate_copy<G>(d, r);
+    if (cl == 0) out[u] = e;
+}
+
+// hyper_ball.rs:434-555: closeness (:447-466), harmonic (:500-518, guard on dist),
+// Lin (:540-547, 1 when dist is not normal)
+__global__ __launch_bounds__(kBlock) void k_centrality(HbDev d, const float *__restrict__ est,
+                                                       int kind, int normalize, double *__restrict__ out)
+{
+    const double nf = (double)d.n - 1.0;
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < d.n; u += gridDim.x * blockDim.x) {
+        const double dd = d.dist[u];
+        const double e = (double)est[u];
+        const bool normal = isnormal(dd);
+        double r;
+        if (kind == 0) r = !normal ? 0.0 : (normalize ? __ddiv_rn(__dsub_rn(e, 1.0), dd) : __ddiv_rn(nf, dd));
+        else if (kind == 1) r = !normal ? 0.0 : (normalize ? __ddiv_rn(d.inv[u], __dsub_rn(e, 1.0)) : __ddiv_rn(d.inv[u], nf));
+        else {
+            const double x = __dsub_rn(e, 1.0);
+            r = !normal ? 1.0 : __ddiv_rn(__dmul_rn(x, x), dd);
+        }
+        out[u] = r;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers
+// ---------------------------------------------------------------------------
+static unsigned grid_for(uint64_t threads, unsigned cap)
+{
+    const uint64_t b = (threads + kBlock - 1) / kBlock;
+    return (unsigned)(b < 1 ? 1 : (b > cap ? cap : b));
+}
+
+template <int P>
+static hipError_t launch_union_T(const HbDev &d, const uint32_t *list, uint32_t count, int T,
+                                 unsigned max_blocks, int slot, hipStream_t s)
+{
+    constexpr int G = (1 << P) / 16;
+    const unsigned grid = grid_for((uint64_t)count * T, max_blocks);
+    switch (T) {
+#define HB_CASE(TT)                                                                        \
+    case TT:                                                                               \
+        if constexpr (TT >= G) {                                                           \
+            k_union<P, TT><<<grid, kBlock, 0, s>>>(d, list, count, slot);                  \
+            return hipGetLastError();                                                      \
+        }                                                                                  \
+        break;
+        HB_CASE(1) HB_CASE(2) HB_CASE(4) HB_CASE(8) HB_CASE(16) HB_CASE(32) HB_CASE(64)
+#undef HB_CASE
+    }
+    return hipErrorInvalidValue;
+}
+
+#define HB_DISPATCH_P(p, ...)                              \
+    switch (p) {                                           \
+    case 4: { constexpr int P = 4; __VA_ARGS__; } break;   \
+    case 5: { constexpr int P = 5; __VA_ARGS__; } break;   \
+    case 6: { constexpr int P = 6; __VA_ARGS__; } break;   \
+    case 7: { constexpr int P = 7; __VA_ARGS__; } break;   \
+    case 8: { constexpr int P = 8; __VA_ARGS__; } break;   \
+    case 9: { constexpr int P = 9; __VA_ARGS__; } break;   \
+    case 10: { constexpr int P = 10; __VA_ARGS__; } break; \
+    default: return hipErrorInvalidValue;                  \
+    }
+
+hipError_t launch_init(const HbDev &d, hipStream_t s)
+{
+    HB_DISPATCH_P(d.p, {
+        const uint64_t threads = (uint64_t)d.n * ((1 << P) / 16);
+        k_init<P><<<(unsigned)((threads + kBlock - 1) / kBlock), kBlock, 0, s>>>(d);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(uint32_t *p, uint64_t count, uint32_t value, hipStream_t s)
+{
+    if (count == 0) return hipSuccess;
+    k_fill_u32<<<grid_for(count, 4096), kBlock, 0, s>>>(p, count, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_iteration(const HbDev &d, const HbSchedule &sc, bool nf, hipStream_t s,
+                            hipEvent_t *ev)
+{
+    const uint32_t words = (d.n + 31) / 32;
+    k_begin<<<grid_for(words, 2048), kBlock, 0, s>>>(d, words);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    for (int b = 0; b < sc.nbins; ++b) {
+        if (sc.bin_count[b] == 0) continue;
+        if (ev) (void)hipEventRecord(ev[2 * b], s);
+        HB_DISPATCH_P(d.p, e = launch_union_T<P>(d, sc.bin_list[b], sc.bin_count[b], sc.bin_team[b],
+                                                 sc.max_blocks, b, s));
+        if (ev) (void)hipEventRecord(ev[2 * b + 1], s);
+        if (e != hipSuccess) return e;
+    }
+    if (sc.ntasks) {
+        HB_DISPATCH_P(d.p, {
+            if (ev) (void)hipEventRecord(ev[2 * HB_SLOT_CHUNK], s);
+            k_chunk<P><<<grid_for((uint64_t)sc.ntasks * HB_WAVE, sc.max_blocks), kBlock, 0, s>>>(
+                d, sc.task_node, sc.task_beg, sc.task_end, sc.ntasks);
+            if (ev) (void)hipEventRecord(ev[2 * HB_SLOT_CHUNK + 1], s);
+            if (ev) (void)hipEventRecord(ev[2 * HB_SLOT_MERGE], s);
+            k_merge<P><<<grid_for((uint64_t)sc.nhubs * HB_WAVE, sc.max_blocks), kBlock, 0, s>>>(
+                d, sc.hub_node, sc.hub_task0, sc.hub_ntask, sc.nhubs);
+            if (ev) (void)hipEventRecord(ev[2 * HB_SLOT_MERGE + 1], s);
+        });
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (nf) {
+        k_nf<<<grid_for(d.n, 1024), kBlock, 0, s>>>(d);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    k_finalize<<<1, 64, 0, s>>>(d);
+    return hipGetLastError();
+}
+
+hipError_t launch_estimates(const HbDev &d, const uint8_t *regs, float *out, hipStream_t s)
+{
+    HB_DISPATCH_P(d.p, {
+        const uint64_t threads = (uint64_t)d.n * ((1 << P) / 16);
+        k_estimates<P><<<(unsigned)((threads + kBlock - 1) / kBlock), kBlock, 0, s>>>(d, regs, out);
+    });
+    return hipGetLastError();
+}
+
+hipError_t launch_centrality(const HbDev &d, const float *est, int kind, int normalize, double *out,
+                             hipStream_t s)
+{
+    k_centrality<<<grid_for(d.n, 4096), kBlock, 0, s>>>(d, est, kind, normalize, out);
+    return hipGetLastError();
+}
+
+}  // namespace hb

@@ -1,0 +1,223 @@
+"""HyperBall on MI355X behind the reference's HyperBallInner interface.
+
+Mirrors `HyperBallInner<'a, N, E, Ix, P, B = 6>` (src/centralities/hyper_ball.rs:81-108)
+and its aliases HyperBall4..HyperBall16 (:46-58): same constructor names,
+argument meaning, accessors, epilogues and error behaviour, over the C ABI of
+libsparkx_hb.so.  All numerics run in the HIP engine; this module only moves
+host buffers and keeps the reference's caching rules for centralities.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from .graph import Graph
+
+
+class HyperBall:
+    """Register-exact HyperBall (HLL++ counters with 2^p six-bit registers).
+
+    new(g) / new_with_conf(g, max_depth, threads) run the algorithm at once,
+    like the reference (hyper_ball.rs:124-157).  `max_depth` and `threads` are
+    accepted for signature parity; the reference ignores max_depth (:710-712 is
+    dead code) and threads only partition its CPU loop, so neither changes
+    results.  `max_iters` (an extension) truncates the run.
+    """
+
+    PRECISION = 8  # GraphMemoryMap::hyperball() uses HyperBall8 (impl_algos.rs:45-48)
+
+    def __init__(self, g: Graph, p: int, *, max_iters: int = 0, dense: bool = False,
+                 graph_capture: bool = False, timing: bool = False, nf: bool = False,
+                 ktiming: bool = False,
+                 device: int = 0, schedule: Optional[tuple] = None, _run: bool = True):
+        if not 4 <= p <= 16:
+            raise ValueError(f"precision p={p} outside 4..16")
+        self.g = g
+        self.p = p
+        self._L = _lib.load()
+        self._ctx = C.c_void_p()
+        _lib.check(self._L.hb_create(C.byref(self._ctx), device))
+        self._closeness = [None, None]
+        self._harmonic = [None, None]
+        self._lin = None
+        self.stats = None
+        try:
+            if schedule is not None:
+                _lib.check(self._L.hb_set_schedule(self._ctx, *[int(x) for x in schedule]))
+            self._stage()
+            if _run:
+                self.run(max_iters=max_iters, dense=dense, graph_capture=graph_capture, timing=timing,
+                         nf=nf, ktiming=ktiming)
+        except Exception:
+            self.close()
+            raise
+
+    # ---- constructors (hyper_ball.rs:124-201) ---------------------------------
+    @classmethod
+    def new(cls, g: Graph, p: Optional[int] = None, **kw) -> "HyperBall":
+        return cls(g, cls.PRECISION if p is None else p, **kw)
+
+    @classmethod
+    def new_with_conf(cls, g: Graph, max_depth: Optional[int], threads: int,
+                      p: Optional[int] = None, **kw) -> "HyperBall":
+        del max_depth, threads  # no effect on results in the reference either
+        return cls(g, cls.PRECISION if p is None else p, **kw)
+
+    # ---- engine ---------------------------------------------------------------
+    def _stage(self):
+        g = self.g
+        off, nb = g.offsets, g.neighbours
+        nbp = nb.ctypes.data if nb.size else None
+        _lib.check(self._L.hb_stage_csr(self._ctx, off.ctypes.data, off.itemsize, g.size(),
+                                        nbp, nb.itemsize, g.width()))
+
+    def run(self, *, max_iters: int = 0, dense: bool = False, graph_capture: bool = False,
+            timing: bool = False, nf: bool = False, ktiming: bool = False) -> "_lib.HbStats":
+        flags = ((_lib.HB_FLAG_DENSE if dense else 0) | (_lib.HB_FLAG_GRAPH if graph_capture else 0)
+                 | (_lib.HB_FLAG_TIMING if timing else 0) | (_lib.HB_FLAG_NF if nf else 0)
+                 | (_lib.HB_FLAG_KTIMING if ktiming else 0))
+        st = _lib.HbStats()
+        _lib.check(self._L.hb_run(self._ctx, self.p, max_iters, flags, C.byref(st)))
+        self.stats = st
+        self._closeness = [None, None]
+        self._harmonic = [None, None]
+        self._lin = None
+        return st
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._L.hb_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- results --------------------------------------------------------------
+    def get_iters(self) -> int:
+        """Iterations executed, including the final one with no increase (hyper_ball.rs:557-560)."""
+        return int(self.stats.iterations)
+
+    def registers(self, which: int = _lib.HB_REGS_REFERENCE) -> np.ndarray:
+        out = np.empty((self.g.size(), 1 << self.p), np.uint8)
+        _lib.check(self._L.hb_read_registers(self._ctx, which, _lib.ptr(out, C.c_uint8)))
+        return out
+
+    def estimates(self, which: int = _lib.HB_REGS_REFERENCE) -> np.ndarray:
+        out = np.empty(self.g.size(), np.float32)
+        _lib.check(self._L.hb_read_estimates(self._ctx, which, _lib.ptr(out, C.c_float)))
+        return out
+
+    def _accumulators(self):
+        n = self.g.size()
+        dist = np.empty(n, np.float64)
+        inv = np.empty(n, np.float64)
+        _lib.check(self._L.hb_read_accumulators(self._ctx, _lib.ptr(dist, C.c_double), _lib.ptr(inv, C.c_double)))
+        return dist, inv
+
+    # hyper_ball.rs:272-280
+    def geodesic_distances(self) -> np.ndarray:
+        return self._accumulators()[0]
+
+    def harmonic_geodesic_distances(self) -> np.ndarray:
+        return self._accumulators()[1]
+
+    # hyper_ball.rs:260-270: assert!(node_id < self.g.size())
+    def get_geodesic_distance(self, node_id: int) -> float:
+        if not 0 <= node_id < self.g.size():
+            raise AssertionError(f"node_id {node_id} >= |V| = {self.g.size()}")
+        return float(self.geodesic_distances()[node_id])
+
+    def get_harmonic_geodesic_distance(self, node_id: int) -> float:
+        if not 0 <= node_id < self.g.size():
+            raise AssertionError(f"node_id {node_id} >= |V| = {self.g.size()}")
+        return float(self.harmonic_geodesic_distances()[node_id])
+
+    def _centrality(self, kind: int, normalize: bool) -> np.ndarray:
+        out = np.empty(self.g.size(), np.float64)
+        _lib.check(self._L.hb_centrality(self._ctx, kind, 1 if normalize else 0, _lib.ptr(out, C.c_double)))
+        return out
+
+    # hyper_ball.rs:434-477
+    def compute_closeness_centrality(self, normalize: bool) -> np.ndarray:
+        self._closeness[int(normalize)] = self._centrality(_lib.HB_CLOSENESS, normalize)
+        return self._closeness[int(normalize)]
+
+    # hyper_ball.rs:487-529
+    def compute_harmonic_centrality(self, normalize: bool) -> np.ndarray:
+        self._harmonic[int(normalize)] = self._centrality(_lib.HB_HARMONIC, normalize)
+        return self._harmonic[int(normalize)]
+
+    # hyper_ball.rs:532-555
+    def compute_lins_centrality(self) -> np.ndarray:
+        self._lin = self._centrality(_lib.HB_LIN, False)
+        return self._lin
+
+    # hyper_ball.rs:290-299, :347-356, :397-402: Err when not computed
+    def get_closeness_centrality(self, normalization: bool) -> np.ndarray:
+        v = self._closeness[int(normalization)]
+        if v is None:
+            raise LookupError("error no such cached centrality")
+        return v
+
+    def get_hamonic_centrality(self, normalization: bool) -> np.ndarray:
+        v = self._harmonic[int(normalization)]
+        if v is None:
+            raise LookupError("no such cached centrality")
+        return v
+
+    get_harmonic_centrality = get_hamonic_centrality
+
+    def get_lins_centrality(self) -> np.ndarray:
+        if self._lin is None:
+            raise LookupError("no such cached centrality")
+        return self._lin
+
+    # hyper_ball.rs:327-337, :384-394, :418-424
+    def get_or_compute_closeness_centrality(self, normalization: bool) -> np.ndarray:
+        if self._closeness[int(normalization)] is None:
+            return self.compute_closeness_centrality(normalization)
+        return self._closeness[int(normalization)]
+
+    def get_or_compute_harmonic_centrality(self, normalization: bool) -> np.ndarray:
+        if self._harmonic[int(normalization)] is None:
+            return self.compute_harmonic_centrality(normalization)
+        return self._harmonic[int(normalization)]
+
+    def get_or_compute_lins_centrality(self) -> np.ndarray:
+        if self._lin is None:
+            return self.compute_lins_centrality()
+        return self._lin
+
+    def neighbourhood_function(self) -> np.ndarray:
+        """N(t) = sum_u |B(u, t)| estimates, t = 1..iterations (needs nf=True)."""
+        it = self.get_iters()
+        out = np.empty(max(it, 1), np.float64)
+        k = _lib.check(self._L.hb_neighbourhood_function(self._ctx, _lib.ptr(out, C.c_double), it))
+        return out[:k]
+
+
+def _alias(p: int):
+    class _HB(HyperBall):
+        PRECISION = p
+
+    _HB.__name__ = _HB.__qualname__ = f"HyperBall{p}"
+    return _HB
+
+
+# hyper_ball.rs:46-58
+HyperBall4, HyperBall5, HyperBall6, HyperBall7, HyperBall8, HyperBall9, HyperBall10 = (
+    _alias(p) for p in range(4, 11))
+HyperBall11, HyperBall12, HyperBall13, HyperBall14, HyperBall15, HyperBall16 = (
+    _alias(p) for p in range(11, 17))

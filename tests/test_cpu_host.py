@@ -1,0 +1,116 @@
+"""CPU tests of the host side: the C ABI library loads and exports every
+declared symbol, the generators build canonical CSRs, and the Python mirror of
+HyperBallInner validates its arguments.  No GPU compute calls."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_functions():
+    names = set()
+    for hdr in ("sparkx_hb.h", "sparkx_hb_graphgen.h"):
+        with open(os.path.join(ROOT, "include", hdr)) as fh:
+            text = re.sub(r"/\*.*?\*/", "", fh.read(), flags=re.S)
+        names |= set(re.findall(r"\b(hb_[a-z_0-9]+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    from sparkx_amd import _lib
+    L = _lib.load()
+    declared = _declared_functions()
+    assert declared == set(_lib.EXPORTS)
+    for name in declared:
+        assert hasattr(L, name), name
+    nm = os.popen(f"nm -D --defined-only {_lib.LIB_PATH}").read()
+    exported = set(re.findall(r" T (hb_[a-z_0-9]+)", nm))
+    assert declared <= exported
+
+
+def test_create_reports_errors_without_crashing():
+    from sparkx_amd import _lib
+    L = _lib.load()
+    assert L.hb_create(None, 0) == _lib.HB_EINVAL
+    assert b"NULL" in L.hb_last_error()
+    assert L.hb_run(None, 6, 0, 0, None) == _lib.HB_EINVAL
+    assert L.hb_stage_csr(None, None, 8, 0, None, 8, 0) == _lib.HB_EINVAL
+    assert L.hb_read_registers(None, 0, None) == _lib.HB_EINVAL
+    assert L.hb_num_nodes(None) == 0
+    L.hb_destroy(None)
+
+
+def _check_canonical(g):
+    off = g.offsets.astype(np.int64)
+    nb = g.neighbours.astype(np.int64)
+    n = g.size()
+    assert off[0] == 0 and off[-1] == nb.size and np.all(np.diff(off) >= 0)
+    src = np.repeat(np.arange(n), np.diff(off))
+    assert np.all(nb < n) and not np.any(src == nb)
+    key = src * n + nb
+    assert np.all(np.diff(key) > 0)  # sorted per node, no duplicates
+    assert np.array_equal(np.sort(nb * n + src), key)  # symmetric
+
+
+def test_erdos_renyi_config_graph():
+    from sparkx_amd import Graph
+    g = Graph.erdos_renyi(65536, 1 << 20, seed=1)
+    assert g.size() == 65536
+    assert g.width() == 2096656  # recorded arc count of the ER config (seed 1)
+    _check_canonical(g)
+    g1 = Graph.erdos_renyi(65536, 1 << 20, seed=1, threads=1)
+    assert np.array_equal(g.offsets, g1.offsets) and np.array_equal(g.neighbours, g1.neighbours)
+    g2 = Graph.erdos_renyi(65536, 1 << 20, seed=2)
+    assert not np.array_equal(g.neighbours[:1000], g2.neighbours[:1000])
+
+
+def test_rmat_graph_shape():
+    from sparkx_amd import Graph
+    g = Graph.rmat(14, 16, seed=5)
+    assert g.size() == 1 << 14
+    _check_canonical(g)
+    d = g.degrees()
+    assert d.max() > 50 * d.mean()  # heavy-tailed hubs
+    g1 = Graph.rmat(14, 16, seed=5, threads=1)
+    assert np.array_equal(g.neighbours, g1.neighbours)
+
+
+def test_kmer_graph_shape():
+    from sparkx_amd import Graph
+    g = Graph.kmer_dbg(50_000, 8, 0.005, 21, seed=3)
+    _check_canonical(g)
+    d = g.degrees()
+    assert d.max() <= 8 and np.bincount(d).argmax() == 2
+    assert 50_000 < g.size() < 2 * 50_000
+
+
+def test_generators_reject_bad_arguments():
+    from sparkx_amd import Graph, HyperBallError
+    with pytest.raises(HyperBallError):
+        Graph.rmat(0)
+    with pytest.raises(HyperBallError):
+        Graph.kmer_dbg(10, 8, 0.005, 21)
+
+
+def test_graph_mirror_bounds():
+    from sparkx_amd import Graph
+    g = Graph(np.array([0, 1, 2], np.uint64), np.array([1, 0], np.uint64))
+    assert g.size() == 2 and g.width() == 2
+    assert list(g.neighbours_of(0)) == [1]
+    with pytest.raises(IndexError):
+        g.neighbours_of(2)
+    with pytest.raises(TypeError):
+        Graph(np.array([0, 1], np.int32), np.array([0], np.uint32))
+
+
+def test_hyperball_rejects_bad_precision():
+    from sparkx_amd import Graph, HyperBall
+    g = Graph(np.array([0, 0], np.uint64), np.zeros(0, np.uint64))
+    with pytest.raises(ValueError):
+        HyperBall(g, 3)
+    with pytest.raises(ValueError):
+        HyperBall(g, 17)

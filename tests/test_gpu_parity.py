@@ -1,0 +1,252 @@
+"""GPU parity tests: the HIP engine (through the C ABI) against the CPU oracle
+on the same seeded inputs.  Registers, f32 estimates, f64 accumulators,
+iteration counts and per-iteration increase counts must be bit-identical."""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+THREADS = min(8, os.cpu_count() or 1)
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def hb():
+    from sparkx_amd import _lib
+    _lib.load()
+    import sparkx_amd
+    return sparkx_amd
+
+
+def _oracle_run(oracle, g, p, **kw):
+    return oracle.hyperball(g.offsets.astype(np.uint64), g.neighbours.astype(np.uint32), p,
+                            threads=THREADS, **kw)
+
+
+def _assert_same(hbobj, ref, oracle, p, ctx=""):
+    from sparkx_amd import _lib
+    st = hbobj.stats
+    assert st.iterations == ref.iterations, ctx
+    assert list(st.increases[:st.iterations]) == list(ref.increases), ctx
+    assert bool(st.final_parity_latest) == ref.final_parity_latest, ctx
+    n = hbobj.g.size()
+    assert np.array_equal(hbobj.registers(_lib.HB_REGS_REFERENCE).ravel(), ref.regs_ref), ctx
+    assert np.array_equal(hbobj.registers(_lib.HB_REGS_LATEST).ravel(), ref.regs_last), ctx
+    dist, inv = hbobj._accumulators()
+    assert np.array_equal(dist, ref.dist), ctx
+    assert np.array_equal(inv, ref.inv), ctx
+    est = hbobj.estimates(_lib.HB_REGS_REFERENCE)
+    assert np.array_equal(est, oracle.estimates(p, ref.regs_ref)), ctx
+    assert est.size == n
+
+
+@pytest.mark.parametrize("p", [4, 5, 6, 7, 8, 9, 10])
+def test_er_small_all_precisions(hb, oracle, p):
+    g = hb.Graph.erdos_renyi(3000, 24000, seed=10 + p)
+    say("er_small", p, g.size(), g.width())
+    ref = _oracle_run(oracle, g, p)
+    with hb.HyperBall(g, p) as h:
+        _assert_same(h, ref, oracle, p, f"p={p}")
+
+
+def test_er_config_every_iteration(hb, oracle):
+    """ER n=65,536, m=2^20 samples, p=6: registers and accumulators after every
+    iteration t equal the oracle's (GPU truncated at t vs oracle trace)."""
+    g = hb.Graph.erdos_renyi(65536, 1 << 20, seed=1)
+    p = 6
+    say("er_config", g.size(), g.width())
+    trace = {}
+    ref = _oracle_run(oracle, g, p, on_iter=lambda t, r, d, i: trace.__setitem__(t, (r, d, i)))
+    from sparkx_amd import _lib
+    with hb.HyperBall(g, p) as h:
+        _assert_same(h, ref, oracle, p, "full run")
+        for t in range(1, ref.iterations + 1):
+            say("er_config iteration", t)
+            st = h.run(max_iters=t)
+            assert st.iterations == t
+            regs, dist, inv = trace[t]
+            assert np.array_equal(h.registers(_lib.HB_REGS_LATEST).ravel(), regs), t
+            d, i = h._accumulators()
+            assert np.array_equal(d, dist) and np.array_equal(i, inv), t
+
+
+@pytest.mark.parametrize("sigma,k,fixture", [(20, 2, "exact_closeness_dbg_sigma20_k2.f64"),
+                                             (4, 7, "exact_closeness_dbg_sigma4_k7.f64")])
+@pytest.mark.parametrize("p", [8, 9, 10])
+def test_fixture_graph_closeness_and_metrics(hb, oracle, sigma, k, fixture, p):
+    """The reference's hyperball_profile (main.rs:855-944) on the de Bruijn
+    fixture graphs, whose CSR keeps MTX duplicates and self-loops."""
+    off, nb = oracle.dbg_mtx_csr(sigma, k)
+    g = hb.Graph(off, nb.astype(np.uint64), f"dbg{sigma}_{k}")
+    say("dbg", sigma, k, p)
+    ref = oracle.hyperball(off, nb, p, threads=THREADS)
+    ex = np.fromfile(os.path.join(GOLDEN, fixture), dtype="<f8")
+    with hb.HyperBall(g, p) as h:
+        _assert_same(h, ref, oracle, p)
+        c = h.compute_closeness_centrality(True)
+        c_ref = oracle.centrality(oracle.CLOSENESS, True, oracle.estimates(p, ref.regs_ref), ref.dist, ref.inv)
+        assert np.array_equal(c, c_ref)
+        assert h.get_closeness_centrality(True) is c
+        metrics = (oracle.mae(c, ex), oracle.mape(c, ex), oracle.spearman(c, ex))
+        if sigma == 20 and p == 10:
+            from tests.golden.pin_hll_variant import PUBLISHED
+            assert metrics == (PUBLISHED[10]["mae"][0], PUBLISHED[10]["mape"][0], PUBLISHED[10]["rho"][0])
+
+
+def test_schedules_frontier_dense_and_graph_capture_agree(hb, oracle):
+    """Scheduling never changes results: dense vs frontier, hipGraph replay, and
+    degree-bin / hub-chunk settings that route nodes through every kernel."""
+    g = hb.Graph.rmat(13, 16, seed=9)
+    p = 7
+    say("rmat13", g.size(), g.width(), int(g.degrees().max()))
+    ref = _oracle_run(oracle, g, p)
+    variants = [dict(), dict(dense=True), dict(graph_capture=True), dict(ktiming=True),
+                dict(schedule=(2, 16, 64)), dict(schedule=(1, 4, 16), dense=True),
+                dict(schedule=(64, 64, 4096)), dict(schedule=(4, 100, 7), graph_capture=True)]
+    for kw in variants:
+        say("variant", kw)
+        with hb.HyperBall(g, p, **kw) as h:
+            _assert_same(h, ref, oracle, p, str(kw))
+            if "schedule" in kw and kw["schedule"][1] < g.degrees().max():
+                assert h.stats.kernel_items[3] > 0  # hub chunks were used
+
+
+def test_kmer_graph_low_degree_bins(hb, oracle):
+    g = hb.Graph.kmer_dbg(60_000, 8, 0.005, 21, seed=4)
+    p = 6
+    say("kmer", g.size(), g.width())
+    ref = _oracle_run(oracle, g, p, max_iters=40)
+    with hb.HyperBall(g, p, max_iters=40) as h:
+        _assert_same(h, ref, oracle, p)
+
+
+def _csr(n, edges):
+    adj = [[] for _ in range(n)]
+    for a, b in edges:
+        adj[a].append(b)
+    off = np.zeros(n + 1, np.uint64)
+    off[1:] = np.cumsum([len(x) for x in adj])
+    nb = np.array([v for x in adj for v in x], np.uint64)
+    return off, nb
+
+
+@pytest.mark.parametrize("name", ["single_node", "isolated", "one_edge", "star", "path", "directed_cycle",
+                                  "self_loops"])
+def test_edge_case_graphs(hb, oracle, name):
+    if name == "single_node":
+        off, nb = _csr(1, [])
+    elif name == "isolated":
+        off, nb = _csr(50, [(3, 4), (4, 3)])
+    elif name == "one_edge":
+        off, nb = _csr(2, [(0, 1), (1, 0)])
+    elif name == "star":
+        n = 5000
+        off, nb = _csr(n, [(0, i) for i in range(1, n)] + [(i, 0) for i in range(1, n)])
+    elif name == "path":
+        n = 300
+        off, nb = _csr(n, [(i, i + 1) for i in range(n - 1)] + [(i + 1, i) for i in range(n - 1)])
+    elif name == "directed_cycle":
+        n = 64
+        off, nb = _csr(n, [(i, (i + 1) % n) for i in range(n)])
+    else:
+        off, nb = _csr(4, [(0, 0), (0, 1), (1, 0), (1, 1), (2, 2), (2, 3), (3, 2), (3, 3), (3, 3)])
+    g = hb.Graph(off, nb, name)
+    for p in (4, 7):
+        say("edge case", name, p)
+        ref = oracle.hyperball(off, nb.astype(np.uint32), p)
+        with hb.HyperBall(g, p, schedule=(2, 8, 16)) as h:
+            _assert_same(h, ref, oracle, p, name)
+
+
+def test_empty_graph(hb):
+    g = hb.Graph(np.zeros(1, np.uint64), np.zeros(0, np.uint64))
+    say("empty graph")
+    with hb.HyperBall(g, 6) as h:
+        assert h.get_iters() == 1
+        assert h.registers().size == 0
+        assert h.compute_closeness_centrality(True).size == 0
+
+
+def test_truncation_and_reference_parity(hb, oracle):
+    g = hb.Graph.erdos_renyi(5000, 12000, seed=3)
+    p = 6
+    with hb.HyperBall(g, p, _run=False) as h:
+        for it in (1, 2, 3, 4, 5):
+            say("truncate", it)
+            ref = _oracle_run(oracle, g, p, max_iters=it)
+            h.run(max_iters=it)
+            _assert_same(h, ref, oracle, p, f"max_iters={it}")
+            assert h.stats.converged == (ref.increases[-1] == 0)
+
+
+def test_centralities_match_oracle(hb, oracle):
+    g = hb.Graph.erdos_renyi(4000, 10000, seed=21)  # sparse: isolated and small components
+    p = 8
+    say("centralities")
+    ref = _oracle_run(oracle, g, p)
+    est = oracle.estimates(p, ref.regs_ref)
+    with hb.HyperBall.new(g) as h:  # HyperBall8 default (GraphMemoryMap::hyperball)
+        assert h.p == 8
+        for kind, norm, fn in ((oracle.CLOSENESS, False, h.compute_closeness_centrality),
+                               (oracle.HARMONIC, True, h.compute_harmonic_centrality),
+                               (oracle.HARMONIC, False, h.compute_harmonic_centrality)):
+            assert np.array_equal(fn(norm), oracle.centrality(kind, norm, est, ref.dist, ref.inv))
+        assert np.array_equal(h.compute_lins_centrality(),
+                              oracle.centrality(oracle.LIN, False, est, ref.dist, ref.inv))
+        assert h.get_or_compute_lins_centrality() is h.get_lins_centrality()
+        with pytest.raises(LookupError):
+            h.get_closeness_centrality(True)
+        with pytest.raises(AssertionError):
+            h.get_geodesic_distance(g.size())
+        assert h.get_geodesic_distance(5) == ref.dist[5]
+
+
+def test_neighbourhood_function_exact(hb, oracle):
+    g = hb.Graph.erdos_renyi(20000, 50000, seed=8)
+    p = 7
+    say("nf")
+    trace = {}
+    _oracle_run(oracle, g, p, on_iter=lambda t, r, d, i: trace.__setitem__(t, r))
+    with hb.HyperBall(g, p, nf=True) as h:
+        nf = h.neighbourhood_function()
+        assert nf.size == len(trace)
+        for t in range(1, nf.size + 1):
+            est = oracle.estimates(p, trace[t]).astype(np.float64)
+            exact = sum(int(x * 2**40) for x in est)  # every f32 estimate >= 1 is exact in 2^-40
+            assert nf[t - 1] == float(exact) / 2**40, t
+
+
+def test_abi_error_paths(hb):
+    from sparkx_amd import _lib
+    L = _lib.load()
+    ctx = C.c_void_p()
+    assert L.hb_create(C.byref(ctx), 0) == 0
+    try:
+        assert L.hb_run(ctx, 6, 0, 0, None) == _lib.HB_ESTATE
+        assert L.hb_read_registers(ctx, 0, (C.c_uint8 * 4)()) == _lib.HB_ESTATE
+        off = np.array([0, 1, 2], np.uint64)
+        bad = np.array([1, 2], np.uint64)  # 2 >= n
+        assert L.hb_stage_csr(ctx, off.ctypes.data, 8, 2, bad.ctypes.data, 8, 2) == _lib.HB_EINVAL
+        assert b"not < n" in L.hb_last_error()
+        dec = np.array([0, 2, 1], np.uint64)
+        assert L.hb_stage_csr(ctx, dec.ctypes.data, 8, 2, bad.ctypes.data, 8, 1) == _lib.HB_EINVAL
+        good = np.array([1, 0], np.uint32)
+        assert L.hb_stage_csr(ctx, off.ctypes.data, 8, 2, good.ctypes.data, 4, 2) == 0
+        assert L.hb_run(ctx, 3, 0, 0, None) == _lib.HB_EINVAL
+        assert L.hb_run(ctx, 6, -1, 0, None) == _lib.HB_EINVAL
+        assert L.hb_run(ctx, 6, 0, _lib.HB_FLAG_KTIMING | _lib.HB_FLAG_GRAPH, None) == _lib.HB_EINVAL
+        assert L.hb_run(ctx, 6, 0, 0, None) == 0
+        assert L.hb_centrality(ctx, 7, 0, (C.c_double * 2)()) == _lib.HB_EINVAL
+        assert L.hb_read_registers(ctx, 5, (C.c_uint8 * 128)()) == _lib.HB_EINVAL
+    finally:
+        L.hb_destroy(ctx)
