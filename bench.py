@@ -38,7 +38,20 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def make_graph(spec):
+def make_graph(spec, cache=None):
+    import numpy as np
+
+    from sparkx_amd import Graph
+    if cache and os.path.exists(cache + ".off.npy"):
+        return Graph(np.load(cache + ".off.npy"), np.load(cache + ".nbr.npy", mmap_mode="r"), "cached")
+    g = _generate(spec)
+    if cache:
+        np.save(cache + ".off.npy", g.offsets)
+        np.save(cache + ".nbr.npy", g.neighbours)
+    return g
+
+
+def _generate(spec):
     from sparkx_amd import Graph
     kind = spec["kind"]
     if kind == "rmat":
@@ -61,6 +74,29 @@ def algorithmic_bytes(slot, arcs, nodes, items, p):
     return b
 
 
+def kernel_name(slot, p, team):
+    if slot < 3:
+        return f"hb::k_union<{p}, {team}>"
+    return f"hb::k_chunk<{p}>" if slot == 3 else f"hb::k_merge<{p}>"
+
+
+def measured_traffic(kernel, workload, p):
+    """HBM bytes per launch of `kernel` from a committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (profiles/*_traffic.txt, copy-calibrated), or None."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_traffic.txt"))):
+        kv = {}
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("#") or ":" not in line:
+                    continue
+                k, v = line.split(":", 1)
+                kv[k.strip()] = v.strip()
+        if kv.get("kernel") == kernel and kv.get("workload") == workload and kv.get("p") == str(p):
+            return int(kv["traffic_bytes_per_launch"]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def cpu_baseline(g, p, threads, iters):
     import numpy as np
 
@@ -78,6 +114,7 @@ def main():
     ap.add_argument("--config", default="rmat24", choices=sorted(CONFIGS))
     ap.add_argument("--cpu-iters", type=int, default=1, help="iterations in the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--graph-cache", default=None, help="path prefix to save/load the generated CSR (.npy)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -88,7 +125,7 @@ def main():
 
     spec, p, max_iters = CONFIGS[args.config]
     t0 = time.perf_counter()
-    g = make_graph(spec)
+    g = make_graph(spec, args.graph_cache)
     log(f"graph {args.config}: n={g.size()} arcs={g.width()} max_deg={int(g.degrees().max())} "
         f"built in {time.perf_counter() - t0:.1f}s")
     t0 = time.perf_counter()
@@ -133,6 +170,8 @@ def main():
                 "team": int(st.kernel_team[s]), "items": int(st.kernel_items[s]),
                 "alg_bytes_per_launch": int(slot_bytes[s] / slot_launch[s]),
                 "GBps": round(slot_bytes[s] / (slot_ms[s] * 1e-3) / 1e9, 1) if slot_ms[s] else None}
+    dom_kernel = kernel_name(dom, p, int(st.kernel_team[dom]))
+    traffic, traffic_src = measured_traffic(dom_kernel, args.config, p)
     all_ms = sum(slot_ms)
     iter_gbs = sum(slot_bytes) / (all_ms * 1e-3) / 1e9 if all_ms else 0.0
 
@@ -171,9 +210,11 @@ def main():
         "iteration_GBps_alg": round(iter_gbs, 1),
         "per_iteration_ms": per_iter_ms,
         "kernels": kernels,
-        "roofline": {"bound": "hbm", "kernel": _lib.KERNEL_SLOTS[dom], "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm", "kernel": dom_kernel, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None},
+                     "traffic": traffic, "traffic_source": traffic_src,
+                     "alg_bytes_per_launch": int(dom_bytes_per_launch),
+                     "avg_launch_ms": round(dom_avg_ms, 4)},
         "cpu_baseline": cpu,
         "speedup_vs_cpu": round(value / cpu["value"], 2) if cpu else None,
     }
