@@ -98,6 +98,7 @@ struct hb_ctx {
     HbIterState h_st{};
     std::vector<HbIterRec> h_rec;
     uint32_t sched_p = 0, sched_key[4] = {};
+    uint32_t n_linked = 0;
 
     void release_run()
     {
@@ -282,10 +283,14 @@ static int build_schedule(hb_ctx *c, int p)
     if (c->sched_p == (uint32_t)p && !memcmp(key, c->sched_key, sizeof key)) return HB_OK;
     const uint32_t G = std::min(64u, (1u << p) / 16u);
     const uint32_t n = (uint32_t)c->n;
+    c->n_linked = 0;
     std::vector<uint32_t> bins[3], tnode, tbeg, tend, hnode, htask0, hntask;
     for (uint32_t u = 0; u < n; ++u) {
         const uint32_t d = c->h_off[u + 1] - c->h_off[u];
-        if (d <= c->small_max) bins[0].push_back(u);
+        if (d == 0) continue;  // keeps c_0; written to both buffers at init
+        ++c->n_linked;
+        if (p > 10) bins[2].push_back(u);  // one wavefront per node walks the register row
+        else if (d <= c->small_max) bins[0].push_back(u);
         else if (d <= c->mid_max) bins[1].push_back(u);
         else if (d <= c->wave_max) bins[2].push_back(u);
         else {
@@ -354,7 +359,6 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
 {
     if (!c) return fail(HB_EINVAL, "hb_run: ctx is NULL");
     if (p < 4 || p > 16) return fail(HB_EINVAL, "hb_run: precision p=%d outside 4..16", p);
-    if (p > 10) return fail(HB_ERANGE, "hb_run: precision p=%d not supported by this build (4..10)", p);
     if (max_iters < 0) return fail(HB_EINVAL, "hb_run: max_iters < 0");
     if (!c->off.ptr) return fail(HB_ESTATE, "hb_run: no graph staged");
     if (!set_device(c)) return fail(HB_EHIP, "hipSetDevice failed");
@@ -427,6 +431,7 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
     d.scratch = c->scratch.ptr;
     d.chunk_flag = c->chunk_flag.ptr;
     d.n = n;
+    d.n_linked = c->n_linked;
     d.p = (uint32_t)p;
     const float mm = (float)m * (float)m;
     d.am = alpha_f32(m) * mm;

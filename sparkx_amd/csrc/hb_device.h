@@ -24,6 +24,9 @@ struct HbIterState {
     uint32_t parity;     // registers read from regs[parity], written to regs[parity^1]
     uint32_t converged;
     double t, inv_t;     // t and 1/t as f64 (hyper_ball.rs:768-769,842-843)
+    uint32_t skip_scan;  // most nodes changed last iteration: recompute all without the
+                         // frontier scan (recomputing an unaffected node is a no-op)
+    uint32_t pad;
 };
 
 // kernel slots of one iteration: union bins 0..2, hub chunks, hub merge
@@ -35,6 +38,7 @@ struct HbIterRec {  // per-iteration counters, per kernel slot
     unsigned long long inc[HB_NSLOT];    // nodes whose estimate increased
     unsigned long long arcs[HB_NSLOT];   // neighbour sketches gathered
     unsigned long long nodes[HB_NSLOT];  // nodes recomputed
+    unsigned long long chg[HB_NSLOT];    // nodes whose registers changed
     unsigned long long nf_lo, nf_hi;     // N(t) * 2^40 as a 128-bit integer
 };
 
@@ -52,6 +56,7 @@ struct HbDev {
     uint8_t *scratch;      // partial sketches of hub chunks
     uint32_t *chunk_flag;  // 1 if the chunk was recomputed this iteration
     uint32_t n;
+    uint32_t n_linked;     // nodes with at least one neighbour (the scheduled ones)
     uint32_t p;
     float am;              // alpha_m * m * m, rounded to f32
     float five_m;          // 5m
@@ -237,6 +242,25 @@ __device__ __forceinline__ double hb_knn_bias(const double *__restrict__ raw,
     return s / 6.0;
 }
 
+// Lookup tables of the estimator for one p: lc[z] = (float)(m ln(m/z)), z = 1..m,
+// and the HLL++ bias points (raw, bias), 200 each.  Kernels stage them in LDS.
+struct HbTables {
+    const float *lc;
+    const double *raw;
+    const double *val;
+};
+
+#define HB_TABLES_LDS(P, d, name)                                                      \
+    __shared__ float name##_lc[(1 << (P)) + 1];                                        \
+    __shared__ double name##_raw[200], name##_val[200];                                \
+    for (uint32_t i_ = threadIdx.x; i_ <= (1u << (P)); i_ += blockDim.x) name##_lc[i_] = (d).lc[i_]; \
+    for (uint32_t i_ = threadIdx.x; i_ < 200u; i_ += blockDim.x) {                     \
+        name##_raw[i_] = (d).bias_raw[i_];                                             \
+        name##_val[i_] = (d).bias_val[i_];                                             \
+    }                                                                                  \
+    __syncthreads();                                                                   \
+    const HbTables name = {name##_lc, name##_raw, name##_val};
+
 // Estimate of the sketch held by a G-lane copy (each lane: 16 registers in acc).
 // All lanes of the copy return the same value.
 //   1. zeros > 0 and LC = m ln(m/zeros) <= threshold(p): return LC (HLL++).
@@ -249,13 +273,13 @@ __device__ __forceinline__ double hb_knn_bias(const double *__restrict__ raw,
 // f32 products and quotients are formed in f64 and rounded once, which gives the
 // correctly rounded f32 result (53 >= 2*24 + 2).
 template <int G>
-__device__ __forceinline__ float hb_estimate_copy(const HbDev &d, uint4 acc)
+__device__ __forceinline__ float hb_estimate_copy(const HbDev &d, const HbTables &tb, uint4 acc)
 {
     uint32_t z = hb_zero_bytes(acc.x) + hb_zero_bytes(acc.y) + hb_zero_bytes(acc.z) +
                  hb_zero_bytes(acc.w);
     z = hb_copy_sum<G>(z);
     if (z > 0) {
-        const float lc = d.lc[z];
+        const float lc = tb.lc[z];
         if ((double)lc <= d.lc_thr) return lc;
     }
     uint32_t mx = hb_max_byte(hb_bmax(hb_bmax(acc.x, acc.y), hb_bmax(acc.z, acc.w)));
@@ -279,6 +303,6 @@ __device__ __forceinline__ float hb_estimate_copy(const HbDev &d, uint4 acc)
         }
     }
     float est = (float)((double)d.am / (double)S);
-    if (est <= d.five_m) est = (float)((double)est - hb_knn_bias(d.bias_raw, d.bias_val, (double)est, 200));
+    if (est <= d.five_m) est = (float)((double)est - hb_knn_bias(tb.raw, tb.val, (double)est, 200));
     return est;
 }

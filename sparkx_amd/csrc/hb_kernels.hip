@@ -26,34 +26,39 @@ __device__ __forceinline__ bool bit_test(const uint32_t *bits, uint32_t v)
     return (bits[v >> 5] >> (v & 31u)) & 1u;
 }
 
-// block-wide sum of three per-lane counters, one atomic per counter per block
-__device__ __forceinline__ void block_add3(unsigned long long a, unsigned long long b,
-                                           unsigned long long c, HbIterRec *rec, int slot)
+// block-wide sums of per-lane counters, one atomic per counter per block
+__device__ __forceinline__ void block_add4(unsigned long long a, unsigned long long b,
+                                           unsigned long long c, unsigned long long e,
+                                           HbIterRec *rec, int slot)
 {
-    __shared__ unsigned long long red[3][kBlock / HB_WAVE];
+    __shared__ unsigned long long red[4][kBlock / HB_WAVE];
 #pragma unroll
     for (int s = 1; s < HB_WAVE; s <<= 1) {
         a += __shfl_xor(a, s);
         b += __shfl_xor(b, s);
         c += __shfl_xor(c, s);
+        e += __shfl_xor(e, s);
     }
     const int w = threadIdx.x / HB_WAVE;
     if ((threadIdx.x & (HB_WAVE - 1)) == 0) {
         red[0][w] = a;
         red[1][w] = b;
         red[2][w] = c;
+        red[3][w] = e;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned long long s0 = 0, s1 = 0, s2 = 0;
+        unsigned long long s0 = 0, s1 = 0, s2 = 0, s3 = 0;
         for (int i = 0; i < (int)(blockDim.x / HB_WAVE); ++i) {
             s0 += red[0][i];
             s1 += red[1][i];
             s2 += red[2][i];
+            s3 += red[3][i];
         }
         if (s0) atomicAdd(&rec->inc[slot], s0);
         if (s1) atomicAdd(&rec->arcs[slot], s1);
         if (s2) atomicAdd(&rec->nodes[slot], s2);
+        if (s3) atomicAdd(&rec->chg[slot], s3);
     }
 }
 
@@ -78,6 +83,7 @@ template <int P>
 __global__ __launch_bounds__(kBlock) void k_init(HbDev d)
 {
     constexpr uint32_t M = 1u << P, G = M / 16;
+    HB_TABLES_LDS(P, d, tb)
     const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t u = (uint32_t)(gtid / G), cl = (uint32_t)(gtid % G);
     if (u >= d.n) return;  // whole copies leave together (G divides the block)
@@ -87,7 +93,9 @@ __global__ __launch_bounds__(kBlock) void k_init(HbDev d)
     if (idx / 16 == cl) w[(idx % 16) / 4] = rank << (8 * (idx % 4));
     const uint4 r = make_uint4(w[0], w[1], w[2], w[3]);
     *(uint4 *)(d.regs0 + ((uint64_t)u << P) + cl * 16) = r;
-    const float e = hb_estimate_copy<G>(d, r);
+    // a node without neighbours keeps c_0 forever and is never scheduled: both buffers
+    if (d.off[u + 1] == d.off[u]) *(uint4 *)(d.regs1 + ((uint64_t)u << P) + cl * 16) = r;
+    const float e = hb_estimate_copy<G>(d, tb, r);
     if (cl == 0) {
         d.est[u] = e;
         d.dist[u] = 0.0;
@@ -123,17 +131,18 @@ __global__ __launch_bounds__(kBlock) void k_union(HbDev d, const uint32_t *__res
     static_assert(T >= (int)G && T <= HB_WAVE, "team width");
     const HbIterState *st = d.st;
     if (st->done) return;
+    HB_TABLES_LDS(P, d, tb)
     const uint32_t it = st->iter;
     const double t = st->t, inv_t = st->inv_t;
     const uint8_t *__restrict__ in = st->parity ? d.regs1 : d.regs0;
     uint8_t *__restrict__ out = st->parity ? d.regs0 : d.regs1;
     const uint32_t *__restrict__ bitsR = (it & 1u) ? d.bits0 : d.bits1;
     uint32_t *bitsW = (it & 1u) ? d.bits1 : d.bits0;
-    const bool use_frontier = d.frontier && it > 1;
+    const bool use_frontier = d.frontier && it > 1 && !st->skip_scan;
 
     const uint32_t tl = threadIdx.x % T, cl = tl % G, slot = tl / G;
     const uint32_t teams_per_grid = gridDim.x * (kBlock / T);
-    unsigned long long c_inc = 0, c_arcs = 0, c_nodes = 0;
+    unsigned long long c_inc = 0, c_arcs = 0, c_nodes = 0, c_chg = 0;
 
     for (uint32_t team = blockIdx.x * (kBlock / T) + threadIdx.x / T; team < count;
          team += teams_per_grid) {
@@ -146,12 +155,13 @@ __global__ __launch_bounds__(kBlock) void k_union(HbDev d, const uint32_t *__res
             for (uint32_t j = tl; j < deg; j += T) any |= bit_test(bitsR, nb[j]);
             active = hb_copy_or<T>(any) != 0;
         }
-        const uint4 own = *(const uint4 *)(in + ((uint64_t)u << P) + cl * 16);
         if (!active) {
             if (slot == 0 && bit_test(bitsR, u))
-                *(uint4 *)(out + ((uint64_t)u << P) + cl * 16) = own;
+                *(uint4 *)(out + ((uint64_t)u << P) + cl * 16) =
+                    *(const uint4 *)(in + ((uint64_t)u << P) + cl * 16);
             continue;
         }
+        const uint4 own = *(const uint4 *)(in + ((uint64_t)u << P) + cl * 16);
         HbAcc pa = hb_acc(own);
         uint32_t j = slot;
         for (; j + 3 * S < deg; j += 4 * S) {
@@ -172,18 +182,21 @@ __global__ __launch_bounds__(kBlock) void k_union(HbDev d, const uint32_t *__res
 
         const uint32_t diffw = (acc.x != own.x) | (acc.y != own.y) | (acc.z != own.z) | (acc.w != own.w);
         const bool changed = hb_copy_or<G>(diffw) != 0;
-        const float curr = hb_estimate_copy<G>(d, acc);
+        const float curr = hb_estimate_copy<G>(d, tb, acc);
         if (slot == 0) *(uint4 *)(out + ((uint64_t)u << P) + cl * 16) = acc;
         if (tl == 0) {
             const float prev = d.est[u];
             if (accumulate(d, u, prev, curr, t, inv_t)) c_inc += 1;
             d.est[u] = curr;
-            if (changed) atomicOr(bitsW + (u >> 5), 1u << (u & 31u));
+            if (changed) {
+                atomicOr(bitsW + (u >> 5), 1u << (u & 31u));
+                c_chg += 1;
+            }
             c_arcs += deg;
             c_nodes += 1;
         }
     }
-    block_add3(c_inc, c_arcs, c_nodes, d.rec + it, slot_id);
+    block_add4(c_inc, c_arcs, c_nodes, c_chg, d.rec + it, slot_id);
 }
 
 // ---------------------------------------------------------------------------
@@ -201,7 +214,7 @@ __global__ __launch_bounds__(kBlock) void k_chunk(HbDev d, const uint32_t *__res
     const uint32_t it = st->iter;
     const uint8_t *__restrict__ in = st->parity ? d.regs1 : d.regs0;
     const uint32_t *__restrict__ bitsR = (it & 1u) ? d.bits0 : d.bits1;
-    const bool use_frontier = d.frontier && it > 1;
+    const bool use_frontier = d.frontier && it > 1 && !st->skip_scan;
     const uint32_t tl = threadIdx.x % T, cl = tl % G, slot = tl / G;
     const uint32_t waves_per_grid = gridDim.x * (kBlock / T);
     unsigned long long c_arcs = 0;
@@ -240,7 +253,7 @@ __global__ __launch_bounds__(kBlock) void k_chunk(HbDev d, const uint32_t *__res
         if (slot == 0) *(uint4 *)(d.scratch + ((uint64_t)task << P) + cl * 16) = acc;
         if (tl == 0) c_arcs += len;
     }
-    block_add3(0, c_arcs, 0, d.rec + it, HB_SLOT_CHUNK);
+    block_add4(0, c_arcs, 0, 0, d.rec + it, HB_SLOT_CHUNK);
 }
 
 // hubs, phase 2: one wavefront per hub merges its own registers with the
@@ -254,6 +267,7 @@ __global__ __launch_bounds__(kBlock) void k_merge(HbDev d, const uint32_t *__res
     constexpr uint32_t M = 1u << P, G = M / 16, T = HB_WAVE, S = T / G;
     const HbIterState *st = d.st;
     if (st->done) return;
+    HB_TABLES_LDS(P, d, tb)
     const uint32_t it = st->iter;
     const double t = st->t, inv_t = st->inv_t;
     const uint8_t *__restrict__ in = st->parity ? d.regs1 : d.regs0;
@@ -262,19 +276,20 @@ __global__ __launch_bounds__(kBlock) void k_merge(HbDev d, const uint32_t *__res
     uint32_t *bitsW = (it & 1u) ? d.bits1 : d.bits0;
     const uint32_t tl = threadIdx.x % T, cl = tl % G, slot = tl / G;
     const uint32_t waves_per_grid = gridDim.x * (kBlock / T);
-    unsigned long long c_inc = 0, c_nodes = 0;
+    unsigned long long c_inc = 0, c_nodes = 0, c_chg = 0;
 
     for (uint32_t h = blockIdx.x * (kBlock / T) + threadIdx.x / T; h < nhubs; h += waves_per_grid) {
         const uint32_t u = hnode[h], t0 = htask0[h], nt = hntask[h];
         uint32_t any = 0;
         for (uint32_t j = tl; j < nt; j += T) any |= d.chunk_flag[t0 + j];
         const bool active = hb_copy_or<T>(any) != 0;
-        const uint4 own = *(const uint4 *)(in + ((uint64_t)u << P) + cl * 16);
         if (!active) {
             if (slot == 0 && bit_test(bitsR, u))
-                *(uint4 *)(out + ((uint64_t)u << P) + cl * 16) = own;
+                *(uint4 *)(out + ((uint64_t)u << P) + cl * 16) =
+                    *(const uint4 *)(in + ((uint64_t)u << P) + cl * 16);
             continue;
         }
+        const uint4 own = *(const uint4 *)(in + ((uint64_t)u << P) + cl * 16);
         uint4 acc = own;
         for (uint32_t j = slot; j < nt; j += S)
             if (d.chunk_flag[t0 + j])
@@ -283,17 +298,194 @@ __global__ __launch_bounds__(kBlock) void k_merge(HbDev d, const uint32_t *__res
         for (uint32_t s = G; s < T; s <<= 1) acc = hb_bmax4(acc, hb_shfl_xor4<T>(acc, s));
         const uint32_t diffw = (acc.x != own.x) | (acc.y != own.y) | (acc.z != own.z) | (acc.w != own.w);
         const bool changed = hb_copy_or<G>(diffw) != 0;
-        const float curr = hb_estimate_copy<G>(d, acc);
+        const float curr = hb_estimate_copy<G>(d, tb, acc);
         if (slot == 0) *(uint4 *)(out + ((uint64_t)u << P) + cl * 16) = acc;
         if (tl == 0) {
             const float prev = d.est[u];
             if (accumulate(d, u, prev, curr, t, inv_t)) c_inc += 1;
             d.est[u] = curr;
-            if (changed) atomicOr(bitsW + (u >> 5), 1u << (u & 31u));
+            if (changed) {
+                atomicOr(bitsW + (u >> 5), 1u << (u & 31u));
+                c_chg += 1;
+            }
             c_nodes += 1;
         }
     }
-    block_add3(c_inc, 0, c_nodes, d.rec + it, HB_SLOT_MERGE);
+    block_add4(c_inc, 0, c_nodes, c_chg, d.rec + it, HB_SLOT_MERGE);
+}
+
+// ---------------------------------------------------------------------------
+// precisions 11..16: one wavefront per node walks the register row in 1 KiB
+// chunks (64 lanes x 16 registers); a per-wave LDS histogram of register values
+// gives zeros, the maximum and the exact integer form of sum 2^-M; a row whose
+// sum is not exact in f32 replays the sequential loop from L2.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void hist_add_word(uint32_t *h, uint32_t x)
+{
+    atomicAdd(h + (x & 0xffu), 1u);
+    atomicAdd(h + ((x >> 8) & 0xffu), 1u);
+    atomicAdd(h + ((x >> 16) & 0xffu), 1u);
+    atomicAdd(h + (x >> 24), 1u);
+}
+
+// estimate from a 64-bin histogram of the row (all lanes of the wave return it)
+template <int P>
+__device__ float estimate_wide(const HbDev &d, const uint32_t *h, const uint8_t *row)
+{
+    constexpr uint32_t M = 1u << P;
+    const uint32_t lane = threadIdx.x & (HB_WAVE - 1);
+    const uint32_t z = h[0];
+    if (z > 0) {
+        const float lc = d.lc[z];
+        if ((double)lc <= d.lc_thr) return lc;
+    }
+    const uint32_t hv = h[lane];
+    uint32_t mx = hv ? lane : 0u;
+    for (int s = 1; s < HB_WAVE; s <<= 1) mx = max(mx, (uint32_t)__shfl_xor(mx, s));
+    // k = sum_v h[v] 2^(mx - v), saturated at 2^24
+    uint32_t k = 0;
+    if (hv) {
+        const uint32_t sh = mx - lane;
+        k = sh >= 24u ? (1u << 24) : min(hv << sh, 1u << 24);
+        if (sh < 24u && (hv >> (24u - sh)) != 0u) k = 1u << 24;
+    }
+    for (int s = 1; s < HB_WAVE; s <<= 1) k = min(k + (uint32_t)__shfl_xor(k, s), 1u << 24);
+    float S;
+    if (k < (1u << 24)) {
+        S = ldexpf((float)k, -(int)mx);
+    } else {
+        S = 0.0f;
+        if (lane == 0) {
+            __threadfence();
+            for (uint32_t i = 0; i < M; i += 4) {
+                const uint32_t w = __hip_atomic_load((const uint32_t *)(row + i), __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+                for (int b = 0; b < 4; ++b) S = __fadd_rn(S, hb_pow2neg((w >> (8 * b)) & 0xffu));
+            }
+        }
+        S = __shfl(S, 0);
+    }
+    float est = (float)((double)d.am / (double)S);
+    if (est <= d.five_m) est = (float)((double)est - hb_knn_bias(d.bias_raw, d.bias_val, (double)est, 200));
+    return est;
+}
+
+template <int P>
+__global__ __launch_bounds__(kBlock) void k_init_wide(HbDev d)
+{
+    constexpr uint32_t M = 1u << P;
+    const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t u = (uint32_t)(gtid / HB_WAVE), lane = (uint32_t)(gtid % HB_WAVE);
+    if (u >= d.n) return;
+    uint32_t idx, rank;
+    hb_index_rank(u, P, &idx, &rank);
+    const bool isolated = d.off[u + 1] == d.off[u];
+    for (uint32_t c = 0; c < M / 1024; ++c) {
+        const uint32_t base = c * 1024 + lane * 16;
+        uint32_t w[4] = {0, 0, 0, 0};
+        if (idx >= base && idx < base + 16) w[(idx - base) / 4] = rank << (8 * (idx % 4));
+        const uint4 r = make_uint4(w[0], w[1], w[2], w[3]);
+        *(uint4 *)(d.regs0 + ((uint64_t)u << P) + base) = r;
+        if (isolated) *(uint4 *)(d.regs1 + ((uint64_t)u << P) + base) = r;
+    }
+    if (lane == 0) {
+        d.est[u] = d.lc[M - 1];  // one non-zero register: linear counting with m-1 zeros
+        d.dist[u] = 0.0;
+        d.inv[u] = 0.0;
+    }
+}
+
+template <int P>
+__global__ __launch_bounds__(kBlock) void k_union_wide(HbDev d, const uint32_t *__restrict__ list,
+                                                       uint32_t count, int slot_id)
+{
+    constexpr uint32_t M = 1u << P;
+    const HbIterState *st = d.st;
+    if (st->done) return;
+    __shared__ uint32_t hist[kBlock / HB_WAVE][64];
+    const uint32_t it = st->iter;
+    const double t = st->t, inv_t = st->inv_t;
+    const uint8_t *__restrict__ in = st->parity ? d.regs1 : d.regs0;
+    uint8_t *out = st->parity ? d.regs0 : d.regs1;
+    const uint32_t *__restrict__ bitsR = (it & 1u) ? d.bits0 : d.bits1;
+    uint32_t *bitsW = (it & 1u) ? d.bits1 : d.bits0;
+    const bool use_frontier = d.frontier && it > 1 && !st->skip_scan;
+    const uint32_t lane = threadIdx.x & (HB_WAVE - 1), w = threadIdx.x / HB_WAVE;
+    uint32_t *h = hist[w];
+    unsigned long long c_inc = 0, c_arcs = 0, c_nodes = 0, c_chg = 0;
+
+    for (uint32_t i = blockIdx.x * (kBlock / HB_WAVE) + w; i < count; i += gridDim.x * (kBlock / HB_WAVE)) {
+        const uint32_t u = list[i];
+        const uint32_t e0 = d.off[u], deg = d.off[u + 1] - e0;
+        const uint32_t *__restrict__ nb = d.nbr + e0;
+        bool active = true;
+        if (use_frontier) {
+            uint32_t any = 0;
+            for (uint32_t j = lane; j < deg; j += HB_WAVE) any |= bit_test(bitsR, nb[j]);
+            active = hb_copy_or<HB_WAVE>(any) != 0;
+        }
+        const uint8_t *own_row = in + ((uint64_t)u << P);
+        uint8_t *out_row = out + ((uint64_t)u << P);
+        if (!active) {
+            if (bit_test(bitsR, u))
+                for (uint32_t c = 0; c < M / 1024; ++c)
+                    *(uint4 *)(out_row + c * 1024 + lane * 16) = *(const uint4 *)(own_row + c * 1024 + lane * 16);
+            continue;
+        }
+        h[lane] = 0u;
+        uint32_t diffw = 0;
+        for (uint32_t c = 0; c < M / 1024; ++c) {
+            const uint32_t off16 = c * 1024 + lane * 16;
+            const uint4 own = *(const uint4 *)(own_row + off16);
+            HbAcc pa = hb_acc(own);
+            for (uint32_t j = 0; j < deg; ++j) hb_acc_max(pa, *(const uint4 *)(in + ((uint64_t)nb[j] << P) + off16));
+            const uint4 acc = hb_acc_get(pa);
+            diffw |= (acc.x != own.x) | (acc.y != own.y) | (acc.z != own.z) | (acc.w != own.w);
+            *(uint4 *)(out_row + off16) = acc;
+            hist_add_word(h, acc.x);
+            hist_add_word(h, acc.y);
+            hist_add_word(h, acc.z);
+            hist_add_word(h, acc.w);
+        }
+        const bool changed = hb_copy_or<HB_WAVE>(diffw) != 0;
+        const float curr = estimate_wide<P>(d, h, out_row);
+        if (lane == 0) {
+            const float prev = d.est[u];
+            if (accumulate(d, u, prev, curr, t, inv_t)) c_inc += 1;
+            d.est[u] = curr;
+            if (changed) {
+                atomicOr(bitsW + (u >> 5), 1u << (u & 31u));
+                c_chg += 1;
+            }
+            c_arcs += deg;
+            c_nodes += 1;
+        }
+    }
+    block_add4(c_inc, c_arcs, c_nodes, c_chg, d.rec + it, slot_id);
+}
+
+template <int P>
+__global__ __launch_bounds__(kBlock) void k_estimates_wide(HbDev d, const uint8_t *__restrict__ regs,
+                                                           float *__restrict__ out)
+{
+    constexpr uint32_t M = 1u << P;
+    __shared__ uint32_t hist[kBlock / HB_WAVE][64];
+    const uint32_t lane = threadIdx.x & (HB_WAVE - 1), w = threadIdx.x / HB_WAVE;
+    uint32_t *h = hist[w];
+    for (uint32_t u = blockIdx.x * (kBlock / HB_WAVE) + w; u < d.n; u += gridDim.x * (kBlock / HB_WAVE)) {
+        const uint8_t *row = regs + ((uint64_t)u << P);
+        h[lane] = 0u;
+        for (uint32_t c = 0; c < M / 1024; ++c) {
+            const uint4 r = *(const uint4 *)(row + c * 1024 + lane * 16);
+            hist_add_word(h, r.x);
+            hist_add_word(h, r.y);
+            hist_add_word(h, r.z);
+            hist_add_word(h, r.w);
+        }
+        const float e = estimate_wide<P>(d, h, row);
+        if (lane == 0) out[u] = e;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -339,8 +531,11 @@ __global__ void k_finalize(HbDev d)
     HbIterState *st = d.st;
     if (st->done || threadIdx.x != 0 || blockIdx.x != 0) return;
     const uint32_t it = st->iter;
-    unsigned long long inc = 0;
-    for (int k = 0; k < HB_NSLOT; ++k) inc += d.rec[it].inc[k];
+    unsigned long long inc = 0, chg = 0;
+    for (int k = 0; k < HB_NSLOT; ++k) {
+        inc += d.rec[it].inc[k];
+        chg += d.rec[it].chg[k];
+    }
     if (inc == 0 || (st->max_iters && it >= st->max_iters)) {
         st->done = 1;
         st->executed = it;
@@ -349,6 +544,7 @@ __global__ void k_finalize(HbDev d)
     }
     st->iter = it + 1;
     st->parity ^= 1u;
+    st->skip_scan = 2 * chg > (unsigned long long)d.n_linked;
     st->t = st->t + 1.0;
     st->inv_t = __ddiv_rn(1.0, st->t);
 }
@@ -361,11 +557,12 @@ __global__ __launch_bounds__(kBlock) void k_estimates(HbDev d, const uint8_t *__
                                                       float *__restrict__ out)
 {
     constexpr uint32_t M = 1u << P, G = M / 16;
+    HB_TABLES_LDS(P, d, tb)
     const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t u = (uint32_t)(gtid / G), cl = (uint32_t)(gtid % G);
     if (u >= d.n) return;
     const uint4 r = *(const uint4 *)(regs + ((uint64_t)u << P) + cl * 16);
-    const float e = hb_estimate_copy<G>(d, r);
+    const float e = hb_estimate_copy<G>(d, tb, r);
     if (cl == 0) out[u] = e;
 }
 
@@ -431,8 +628,25 @@ static hipError_t launch_union_T(const HbDev &d, const uint32_t *list, uint32_t 
     default: return hipErrorInvalidValue;                  \
     }
 
+#define HB_DISPATCH_WIDE(p, ...)                            \
+    switch (p) {                                            \
+    case 11: { constexpr int P = 11; __VA_ARGS__; } break;  \
+    case 12: { constexpr int P = 12; __VA_ARGS__; } break;  \
+    case 13: { constexpr int P = 13; __VA_ARGS__; } break;  \
+    case 14: { constexpr int P = 14; __VA_ARGS__; } break;  \
+    case 15: { constexpr int P = 15; __VA_ARGS__; } break;  \
+    case 16: { constexpr int P = 16; __VA_ARGS__; } break;  \
+    default: return hipErrorInvalidValue;                   \
+    }
+
 hipError_t launch_init(const HbDev &d, hipStream_t s)
 {
+    if (d.p > 10) {
+        HB_DISPATCH_WIDE(d.p, {
+            k_init_wide<P><<<(unsigned)(((uint64_t)d.n * HB_WAVE + kBlock - 1) / kBlock), kBlock, 0, s>>>(d);
+        });
+        return hipGetLastError();
+    }
     HB_DISPATCH_P(d.p, {
         const uint64_t threads = (uint64_t)d.n * ((1 << P) / 16);
         k_init<P><<<(unsigned)((threads + kBlock - 1) / kBlock), kBlock, 0, s>>>(d);
@@ -457,8 +671,16 @@ hipError_t launch_iteration(const HbDev &d, const HbSchedule &sc, bool nf, hipSt
     for (int b = 0; b < sc.nbins; ++b) {
         if (sc.bin_count[b] == 0) continue;
         if (ev) (void)hipEventRecord(ev[2 * b], s);
-        HB_DISPATCH_P(d.p, e = launch_union_T<P>(d, sc.bin_list[b], sc.bin_count[b], sc.bin_team[b],
-                                                 sc.max_blocks, b, s));
+        if (d.p > 10) {
+            HB_DISPATCH_WIDE(d.p, {
+                k_union_wide<P><<<grid_for((uint64_t)sc.bin_count[b] * HB_WAVE, sc.max_blocks), kBlock, 0, s>>>(
+                    d, sc.bin_list[b], sc.bin_count[b], b);
+                e = hipGetLastError();
+            });
+        } else {
+            HB_DISPATCH_P(d.p, e = launch_union_T<P>(d, sc.bin_list[b], sc.bin_count[b], sc.bin_team[b],
+                                                     sc.max_blocks, b, s));
+        }
         if (ev) (void)hipEventRecord(ev[2 * b + 1], s);
         if (e != hipSuccess) return e;
     }
@@ -487,6 +709,12 @@ hipError_t launch_iteration(const HbDev &d, const HbSchedule &sc, bool nf, hipSt
 
 hipError_t launch_estimates(const HbDev &d, const uint8_t *regs, float *out, hipStream_t s)
 {
+    if (d.p > 10) {
+        HB_DISPATCH_WIDE(d.p, {
+            k_estimates_wide<P><<<grid_for((uint64_t)d.n * HB_WAVE, 8192), kBlock, 0, s>>>(d, regs, out);
+        });
+        return hipGetLastError();
+    }
     HB_DISPATCH_P(d.p, {
         const uint64_t threads = (uint64_t)d.n * ((1 << P) / 16);
         k_estimates<P><<<(unsigned)((threads + kBlock - 1) / kBlock), kBlock, 0, s>>>(d, regs, out);

@@ -50,9 +50,10 @@ def _assert_same(hbobj, ref, oracle, p, ctx=""):
     assert est.size == n
 
 
-@pytest.mark.parametrize("p", [4, 5, 6, 7, 8, 9, 10])
+@pytest.mark.parametrize("p", list(range(4, 17)))
 def test_er_small_all_precisions(hb, oracle, p):
-    g = hb.Graph.erdos_renyi(3000, 24000, seed=10 + p)
+    n = 3000 if p <= 12 else 600
+    g = hb.Graph.erdos_renyi(n, 8 * n, seed=10 + p)
     say("er_small", p, g.size(), g.width())
     ref = _oracle_run(oracle, g, p)
     with hb.HyperBall(g, p) as h:
@@ -82,7 +83,7 @@ def test_er_config_every_iteration(hb, oracle):
 
 @pytest.mark.parametrize("sigma,k,fixture", [(20, 2, "exact_closeness_dbg_sigma20_k2.f64"),
                                              (4, 7, "exact_closeness_dbg_sigma4_k7.f64")])
-@pytest.mark.parametrize("p", [8, 9, 10])
+@pytest.mark.parametrize("p", [8, 9, 10, 11, 12, 13])
 def test_fixture_graph_closeness_and_metrics(hb, oracle, sigma, k, fixture, p):
     """The reference's hyperball_profile (main.rs:855-944) on the de Bruijn
     fixture graphs, whose CSR keeps MTX duplicates and self-loops."""
@@ -98,9 +99,10 @@ def test_fixture_graph_closeness_and_metrics(hb, oracle, sigma, k, fixture, p):
         assert np.array_equal(c, c_ref)
         assert h.get_closeness_centrality(True) is c
         metrics = (oracle.mae(c, ex), oracle.mape(c, ex), oracle.spearman(c, ex))
-        if sigma == 20 and p == 10:
+        if sigma == 20 and p >= 10:
+            # linear-counting regime: the reference's published numbers, to the last bit
             from tests.golden.pin_hll_variant import PUBLISHED
-            assert metrics == (PUBLISHED[10]["mae"][0], PUBLISHED[10]["mape"][0], PUBLISHED[10]["rho"][0])
+            assert metrics == (PUBLISHED[p]["mae"][0], PUBLISHED[p]["mape"][0], PUBLISHED[p]["rho"][0])
 
 
 def test_schedules_frontier_dense_and_graph_capture_agree(hb, oracle):
