@@ -1,0 +1,203 @@
+// sparkx_hb.hpp -- header-only C++ mirror of the reference's HyperBallInner
+// (src/centralities/hyper_ball.rs:81-108, aliases HyperBall4..16 :46-58) over the
+// C ABI of libsparkx_hb.so (sparkx_hb.h).  Same constructor and method names,
+// argument meaning and error behaviour: fallible calls throw HyperBallError (the
+// reference returns Err), out-of-range node getters fail an assertion-like check
+// (the reference's assert!, :262,:268), and centralities are cached per
+// normalisation as in the reference (:290-424).
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <optional>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "sparkx_hb.h"
+
+namespace sparkx {
+
+struct HyperBallError : std::runtime_error {
+    int code;
+    HyperBallError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+inline void check(int rc)
+{
+    if (rc < 0) throw HyperBallError(rc, hb_last_error());
+}
+
+// Borrowed CSR view (GraphMemoryMap's index and edge arrays; 4- or 8-byte entries).
+struct CsrView {
+    const void *offsets = nullptr;
+    int offset_bytes = 8;
+    uint64_t nodes = 0;
+    const void *neighbours = nullptr;
+    int neighbour_bytes = 8;
+    uint64_t arcs = 0;
+    uint64_t size() const { return nodes; }   // GraphMemoryMap::size
+    uint64_t width() const { return arcs; }   // GraphMemoryMap::width
+};
+
+template <int P>
+class HyperBall {
+    static_assert(P >= 4 && P <= 16, "precision 4..16");
+
+public:
+    // HyperBallInner::new (hyper_ball.rs:124-133)
+    static HyperBall create(const CsrView &g, int device = 0) { return HyperBall(g, 0, 0u, device); }
+    // HyperBallInner::new_with_conf (:144-157): max_depth and threads do not change
+    // results in the reference (max_t is dead code, threads only split its CPU loop)
+    static HyperBall new_with_conf(const CsrView &g, std::optional<size_t> /*max_depth*/,
+                                   size_t /*threads*/, int device = 0)
+    {
+        return HyperBall(g, 0, 0u, device);
+    }
+    // extension: truncation and scheduling flags (HB_FLAG_*)
+    HyperBall(const CsrView &g, int max_iters, uint32_t flags, int device = 0) : g_(g)
+    {
+        check(hb_create(&ctx_, device));
+        try {
+            check(hb_stage_csr(ctx_, g.offsets, g.offset_bytes, g.nodes, g.neighbours,
+                               g.neighbour_bytes, g.arcs));
+            stats_ = std::make_unique<hb_stats>();
+            check(hb_run(ctx_, P, max_iters, flags, stats_.get()));
+        } catch (...) {
+            hb_destroy(ctx_);
+            throw;
+        }
+    }
+    HyperBall(const HyperBall &) = delete;
+    HyperBall &operator=(const HyperBall &) = delete;
+    HyperBall(HyperBall &&o) noexcept { *this = std::move(o); }
+    HyperBall &operator=(HyperBall &&o) noexcept
+    {
+        std::swap(ctx_, o.ctx_);
+        std::swap(g_, o.g_);
+        std::swap(stats_, o.stats_);
+        std::swap(closeness_, o.closeness_);
+        std::swap(harmonic_, o.harmonic_);
+        std::swap(lin_, o.lin_);
+        return *this;
+    }
+    ~HyperBall()
+    {
+        if (ctx_) hb_destroy(ctx_);
+    }
+
+    // iterations executed, including the final one without increase (:557-560)
+    size_t get_iters() const { return (size_t)stats_->iterations; }
+    const hb_stats &stats() const { return *stats_; }
+
+    // :272-280
+    std::vector<double> geodesic_distances() const
+    {
+        std::vector<double> d(g_.nodes);
+        check(hb_read_accumulators(ctx_, d.data(), nullptr));
+        return d;
+    }
+    std::vector<double> harmonic_geodesic_distances() const
+    {
+        std::vector<double> d(g_.nodes);
+        check(hb_read_accumulators(ctx_, nullptr, d.data()));
+        return d;
+    }
+    // :260-270 (assert!(node_id < self.g.size()))
+    double get_geodesic_distance(size_t node_id) const
+    {
+        if (node_id >= g_.nodes) throw std::out_of_range("node_id >= |V|");
+        return geodesic_distances()[node_id];
+    }
+    double get_harmonic_geodesic_distance(size_t node_id) const
+    {
+        if (node_id >= g_.nodes) throw std::out_of_range("node_id >= |V|");
+        return harmonic_geodesic_distances()[node_id];
+    }
+
+    // :434-477, :487-529, :532-555
+    const std::vector<double> &compute_closeness_centrality(bool normalize)
+    {
+        return closeness_[normalize] = centrality(HB_CLOSENESS, normalize);
+    }
+    const std::vector<double> &compute_harmonic_centrality(bool normalize)
+    {
+        return harmonic_[normalize] = centrality(HB_HARMONIC, normalize);
+    }
+    const std::vector<double> &compute_lins_centrality() { return lin_ = centrality(HB_LIN, false); }
+
+    // :290-299, :347-356, :397-402: Err("no such cached centrality")
+    const std::vector<double> &get_closeness_centrality(bool normalization) const
+    {
+        return cached(closeness_[normalization]);
+    }
+    const std::vector<double> &get_hamonic_centrality(bool normalization) const
+    {
+        return cached(harmonic_[normalization]);
+    }
+    const std::vector<double> &get_lins_centrality() const { return cached(lin_); }
+
+    // :327-337, :384-394, :418-424
+    const std::vector<double> &get_or_compute_closeness_centrality(bool normalization)
+    {
+        return closeness_[normalization].empty() && g_.nodes ? compute_closeness_centrality(normalization)
+                                                             : closeness_[normalization];
+    }
+    const std::vector<double> &get_or_compute_harmonic_centrality(bool normalization)
+    {
+        return harmonic_[normalization].empty() && g_.nodes ? compute_harmonic_centrality(normalization)
+                                                            : harmonic_[normalization];
+    }
+    const std::vector<double> &get_or_compute_lins_centrality()
+    {
+        return lin_.empty() && g_.nodes ? compute_lins_centrality() : lin_;
+    }
+
+    // the counters (u8 registers, node-major) the reference leaves in its file
+    std::vector<uint8_t> counters(int which = HB_REGS_REFERENCE) const
+    {
+        std::vector<uint8_t> r(g_.nodes << P);
+        check(hb_read_registers(ctx_, which, r.data()));
+        return r;
+    }
+    std::vector<float> estimates(int which = HB_REGS_REFERENCE) const
+    {
+        std::vector<float> e(g_.nodes);
+        check(hb_read_estimates(ctx_, which, e.data()));
+        return e;
+    }
+
+private:
+    std::vector<double> centrality(int kind, bool normalize) const
+    {
+        std::vector<double> c(g_.nodes);
+        check(hb_centrality(ctx_, kind, normalize ? 1 : 0, c.data()));
+        return c;
+    }
+    static const std::vector<double> &cached(const std::vector<double> &v)
+    {
+        if (v.empty()) throw HyperBallError(HB_ESTATE, "error no such cached centrality");
+        return v;
+    }
+
+    hb_ctx *ctx_ = nullptr;
+    CsrView g_{};
+    std::unique_ptr<hb_stats> stats_;
+    std::vector<double> closeness_[2], harmonic_[2], lin_;
+};
+
+using HyperBall4 = HyperBall<4>;
+using HyperBall5 = HyperBall<5>;
+using HyperBall6 = HyperBall<6>;
+using HyperBall7 = HyperBall<7>;
+using HyperBall8 = HyperBall<8>;
+using HyperBall9 = HyperBall<9>;
+using HyperBall10 = HyperBall<10>;
+using HyperBall11 = HyperBall<11>;
+using HyperBall12 = HyperBall<12>;
+using HyperBall13 = HyperBall<13>;
+using HyperBall14 = HyperBall<14>;
+using HyperBall15 = HyperBall<15>;
+using HyperBall16 = HyperBall<16>;
+
+}  // namespace sparkx

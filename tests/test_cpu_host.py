@@ -114,3 +114,19 @@ def test_hyperball_rejects_bad_precision():
         HyperBall(g, 3)
     with pytest.raises(ValueError):
         HyperBall(g, 17)
+
+
+def _build_cpp_mirror(tmp_path):
+    import subprocess
+    src = os.path.join(ROOT, "tests", "cpp", "hyperball_cpp_mirror.cpp")
+    exe = str(tmp_path / "hyperball_cpp_mirror")
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror", src,
+           "-L" + os.path.join(ROOT, "sparkx_amd"), "-lsparkx_hb", "-o", exe]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
+    return exe
+
+
+def test_cpp_mirror_compiles_and_links(tmp_path):
+    """The header-only C++ mirror of HyperBallInner builds against the C ABI."""
+    assert os.path.exists(_build_cpp_mirror(tmp_path))

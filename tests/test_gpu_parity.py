@@ -252,3 +252,25 @@ def test_abi_error_paths(hb):
         assert L.hb_read_registers(ctx, 5, (C.c_uint8 * 128)()) == _lib.HB_EINVAL
     finally:
         L.hb_destroy(ctx)
+
+
+def test_cpp_mirror_matches_python_mirror(hb, oracle, tmp_path):
+    """The C++ HyperBallInner mirror (include/sparkx_hb.hpp) gives the oracle's results."""
+    import subprocess
+    from tests.test_cpu_host import _build_cpp_mirror
+    exe = _build_cpp_mirror(tmp_path)
+    say("cpp mirror")
+    env = dict(os.environ, LD_LIBRARY_PATH=os.path.join(ROOT, "sparkx_amd"))
+    res = subprocess.run([exe], capture_output=True, text=True, env=env, timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    lines = res.stdout.split("\n")
+    off = np.array([0, 1, 3, 5, 7, 8, 10, 12, 14], np.uint64)
+    nb = np.array([1, 0, 2, 1, 3, 2, 4, 3, 6, 7, 5, 7, 5, 6], np.uint32)
+    ref = oracle.hyperball(off, nb, 6)
+    est = oracle.estimates(6, ref.regs_ref)
+    c = oracle.centrality(oracle.CLOSENESS, True, est, ref.dist, ref.inv)
+    lin = oracle.centrality(oracle.LIN, False, est, ref.dist, ref.inv)
+    assert lines[0] == f"iters {ref.iterations}"
+    for u in range(8):
+        _, uu, d, cc, ll = lines[1 + u].split()
+        assert (float(d), float(cc), float(ll)) == (ref.dist[u], c[u], lin[u])
