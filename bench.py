@@ -31,6 +31,8 @@ CONFIGS = {
     "rmat20": (dict(kind="rmat", scale=20, edge_factor=16, seed=1), 7, 0),
     "er": (dict(kind="er", n=65536, samples=1 << 20, seed=1), 6, 0),
     "kmer": (dict(kind="kmer", genome_len=4_000_000, copies=8, sub_rate=0.005, k=21, seed=1), 6, 64),
+    "batch64": (dict(kind="batch", count=64, seed=1), 0, 0),
+    "surrogate": (dict(kind="surrogate", train_graphs=32, train_scale=20, seed=1), 7, 0),
 }
 
 
@@ -106,6 +108,122 @@ def cpu_baseline(g, p, threads, iters):
     return r.iterations / r.loop_seconds, r.loop_seconds, r.iterations, r.init_seconds
 
 
+def bench_surrogate(args, spec, eval_graphs=None):
+    """Learned surrogate: MLP trained on GPU (Adam, lr 1e-3, MSE) on seeded R-MAT
+    scale-20 graphs labelled by full p=7 HyperBall; evaluated on R-MAT-24 and the
+    k-mer graph against full HyperBall: Spearman rho (reference tie semantics),
+    top-1% overlap, GPU time of 4 iterations (p=4) + features + MLP vs full run."""
+    import numpy as np
+
+    from sparkx_amd import Graph, HyperBall, metrics
+    from sparkx_amd.surrogate import Surrogate, labelled_graph
+    t0 = time.perf_counter()
+    xs, ys = [], []
+    for i in range(spec["train_graphs"]):
+        g = Graph.rmat(spec["train_scale"], 16, seed=spec["seed"] * 1000 + i)
+        x, y, _, _ = labelled_graph(g)
+        xs.append(x)
+        ys.append(y)
+    x = np.concatenate(xs)
+    y = np.concatenate(ys)
+    log(f"surrogate: {len(xs)} training graphs, {x.shape[0]} samples, labelled in {time.perf_counter() - t0:.1f}s")
+    s = Surrogate(seed=spec["seed"])
+    t0 = time.perf_counter()
+    losses = s.train(x, y, epochs=4, batch=8192, lr=1e-3, seed=spec["seed"])
+    train_s = time.perf_counter() - t0
+    log(f"trained in {train_s:.1f}s, epoch losses {[round(float(v), 5) for v in losses]}")
+    del xs, ys, x, y
+    results = {}
+    evals = eval_graphs or [("rmat24", CONFIGS["rmat24"][0], 7, 0), ("kmer", CONFIGS["kmer"][0], 6, 64)]
+    for name, gspec, p_full, mi in evals:
+        g = make_graph(gspec, args.graph_cache if name == "rmat24" else None)
+        with HyperBall(g, p_full, max_iters=mi) as full:
+            full.run(max_iters=mi)  # warm
+            st = full.run(max_iters=mi)
+            full_ms = st.ms_total + st.ms_init
+            truth = full.harmonic_geodesic_distances()
+        with HyperBall(g, 4, _run=False) as h:
+            s.rank(h)  # warm
+            pred, _, sur_ms = s.rank(h)
+            trunc = h.harmonic_geodesic_distances()
+        linked = g.degrees() > 0
+        results[name] = {
+            "nodes": g.size(), "arcs": g.width(), "full_p": p_full, "full_iterations": int(st.iterations),
+            "full_ms": round(full_ms, 3), "surrogate_ms": round(sur_ms, 3),
+            "time_ratio": round(sur_ms / full_ms, 4),
+            "spearman_all_nodes": round(metrics.spearman_rho(pred, truth), 5),
+            "spearman_linked_nodes": round(metrics.spearman_rho(pred[linked], truth[linked]), 5),
+            "top1pct_overlap": round(metrics.top_overlap(pred, truth, 0.01), 5),
+            "baseline_4iter_spearman_linked": round(metrics.spearman_rho(trunc[linked], truth[linked]), 5),
+            "baseline_4iter_top1pct_overlap": round(metrics.top_overlap(trunc, truth, 0.01), 5),
+        }
+        log(f"surrogate on {name}: {results[name]}")
+    s.close()
+    return {"train_graphs": spec["train_graphs"], "train_scale": spec["train_scale"],
+            "train_seconds": round(train_s, 2), "epoch_losses": [round(float(v), 6) for v in losses],
+            "eval": results}
+
+
+def bench_batch(args, spec):
+    """64 independent graphs (32 ER + 32 R-MAT, n log-uniform in [2^12, 2^18], average
+    degree U[4, 32], p uniform in {4..8}) in one work-queue launch per iteration,
+    hipGraph-replayed, schedule autotuned.  value = graph-iterations per second."""
+    from sparkx_amd.batch import HyperBallBatch, batched_workload
+    t0 = time.perf_counter()
+    items = batched_workload(spec["count"], seed=spec["seed"])
+    nodes = sum(g.size() for g, _ in items)
+    arcs = sum(g.width() for g, _ in items)
+    log(f"batch: {len(items)} graphs, {nodes} nodes, {arcs} arcs, built in {time.perf_counter() - t0:.1f}s")
+    bt = HyperBallBatch(items)
+    knobs, tms = bt.autotune()
+    log(f"autotuned schedule (small_max, mid_max, tile_arcs) = {knobs}, trial {tms:.2f} ms")
+    for _ in range(args.warmup):
+        bt.run()
+    walls, its, arcs_g, alg, dev_ms = [], 0, 0, 0, 0.0
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        st = bt.run()
+        walls.append(time.perf_counter() - t0)
+        its += st.iterations_total
+        arcs_g += st.arcs_gathered
+        alg += st.alg_bytes
+        dev_ms += st.ms_total
+        log(f"step {i}: {st.iterations_total} graph-iterations (longest {st.iterations_max}) in "
+            f"{walls[-1] * 1e3:.2f} ms wall, {st.ms_total:.2f} ms device, {st.launches} launches")
+    total = sum(walls)
+    achieved = alg / (dev_ms * 1e-3) / 1e9 if dev_ms else 0.0
+    cpu = None
+    if not args.no_cpu_baseline:
+        import numpy as np
+
+        from oracle import hb_oracle as O
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        threads = max(1, min(threads, os.cpu_count() or 1))
+        loop, n_it = 0.0, 0
+        for g, p in items[::8]:  # every 8th graph, run to convergence
+            r = O.hyperball(g.offsets.astype(np.uint64), g.neighbours, p, threads=threads, packed=True)
+            loop += r.loop_seconds
+            n_it += r.iterations
+        cpu = {"value": round(n_it / loop, 3), "unit": "graph-iter/s", "cores": threads, "kind": "port",
+               "sample": f"8 of the 64 graphs (every 8th) run to convergence, {n_it} iterations in "
+                         f"{loop:.1f}s, packed 6-bit registers, register init excluded"}
+    out = {
+        "metric": "Batched HyperBall graph-iterations/s (64 graphs, mixed p=4..8)",
+        "value": round(its / total, 2), "unit": "graph-iter/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(total / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+        "config": {"workload": "batch64", "graphs": len(items), "nodes": nodes, "arcs": arcs,
+                   "schedule": list(knobs), "tiles": int(st.tiles)},
+        "gteps": round(arcs_g / total / 1e9, 3),
+        "roofline": {"bound": "hbm", "kernel": "hb::k_batch_union (whole iteration)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    bt.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,6 +242,11 @@ def main():
     from sparkx_amd import HyperBall, _lib
 
     spec, p, max_iters = CONFIGS[args.config]
+    if spec["kind"] == "batch":
+        return bench_batch(args, spec)
+    if spec["kind"] == "surrogate":
+        print(json.dumps(bench_surrogate(args, spec)), flush=True)
+        return None
     t0 = time.perf_counter()
     g = make_graph(spec, args.graph_cache)
     log(f"graph {args.config}: n={g.size()} arcs={g.width()} max_deg={int(g.degrees().max())} "

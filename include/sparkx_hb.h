@@ -134,6 +134,93 @@ int hb_neighbourhood_function(hb_ctx *ctx, double *dst, int cap);
 int hb_set_schedule(hb_ctx *ctx, uint32_t small_max_deg, uint32_t wave_max_deg,
                     uint32_t chunk);
 
+/* ---- batches: many independent graphs advanced together ------------------
+ * No reference counterpart (the reference runs one graph per HyperBallInner).
+ * Graphs of mixed size and precision (p = 4..8) share one launch per iteration
+ * of a work-queue kernel over (graph, degree-bin) tiles; each graph keeps its own
+ * iteration count, termination rule and buffer parity, and its results are
+ * bit-identical to hb_run on that graph alone. */
+typedef struct {
+    const void *offsets;
+    int off_bytes;      /* 4 or 8 */
+    uint64_t n;         /* 1 <= n < 2^31 */
+    const void *nbrs;
+    int nbr_bytes;      /* 4 or 8 */
+    uint64_t arcs;
+    int p;              /* 4..8 */
+    int max_iters;      /* 0: the reference's convergence rule */
+} hb_graph_desc;
+
+typedef struct {
+    int32_t launches;          /* iteration launches until every graph stopped */
+    int32_t iterations_max;    /* longest graph run                            */
+    uint64_t iterations_total; /* sum of every graph's iterations              */
+    uint64_t arcs_gathered;
+    uint32_t tiles;            /* work-queue tiles per iteration               */
+    double ms_total;           /* device time of the iterations                */
+    double ms_init;
+    uint64_t nodes_recomputed;
+    uint64_t alg_bytes;        /* algorithmic bytes of all iterations: per recomputed
+                                  node 8 + 2*2^p + 40, per gathered arc 4 + 2^p     */
+} hb_batch_stats;
+
+typedef struct hb_batch hb_batch;
+
+/* Validates and stages `count` graphs into HBM. */
+int hb_batch_create(hb_batch **out, int device_ordinal, const hb_graph_desc *graphs, int count);
+void hb_batch_destroy(hb_batch *b);
+/* Tiling knobs (results never depend on them); 0 keeps the current value. */
+int hb_batch_set_schedule(hb_batch *b, uint32_t small_max_deg, uint32_t mid_max_deg, uint32_t tile_arcs);
+/* Runs every graph to its own stopping point; flags: HB_FLAG_DENSE, HB_FLAG_GRAPH. */
+int hb_batch_run(hb_batch *b, uint32_t flags, hb_batch_stats *stats);
+/* Iterations, convergence and per-iteration increase counts of one graph;
+ * returns the number of increase counts written (<= cap). */
+int hb_batch_graph_stats(hb_batch *b, int graph, int32_t *iterations, int32_t *converged,
+                         uint64_t *increases, int cap);
+int hb_batch_read_registers(hb_batch *b, int graph, int which, uint8_t *dst);
+int hb_batch_read_estimates(hb_batch *b, int graph, int which, float *dst);
+int hb_batch_read_accumulators(hb_batch *b, int graph, double *dist, double *inv);
+
+/* ---- autotuning -----------------------------------------------------------
+ * Times a grid of schedules on the first `trial_iters` iterations of this graph
+ * (or batch) and keeps the fastest; results never depend on the choice.  Writes
+ * the chosen knobs and their trial time.  No reference counterpart. */
+int hb_autotune(hb_ctx *ctx, int p, int trial_iters, uint32_t *small_max_deg,
+                uint32_t *wave_max_deg, uint32_t *chunk, double *trial_ms);
+int hb_batch_autotune(hb_batch *b, uint32_t *small_max_deg, uint32_t *mid_max_deg,
+                      uint32_t *tile_arcs, double *trial_ms);
+
+/* ---- estimate snapshots ------------------------------------------------------
+ * Record every node's estimate after iterations 1..k of the following runs
+ * (k <= 16; 0 turns it off).  hb_read_snapshots copies k*n floats, iteration-major;
+ * rows past the last executed iteration repeat its estimates.  No reference
+ * counterpart (they are the learned surrogate's inputs N_1..N_k). */
+int hb_set_snapshots(hb_ctx *ctx, int k);
+int hb_read_snapshots(hb_ctx *ctx, float *dst);
+
+/* ---- learned surrogate ---------------------------------------------------------
+ * An MLP 6 -> 64 -> 64 -> 1 with GELU (BF16 MFMA matmuls, f32 accumulation, f32
+ * master weights and Adam) mapping a node's (degree, log(1+degree), log N_1..N_4)
+ * to its log harmonic centrality.  Parameters are 4673 floats: W1[6][64], b1[64],
+ * W2[64][64], b2[64], W3[64], b3; normalisation is mean[6], std[6] of the inputs.
+ * No reference counterpart. */
+typedef struct hb_mlp hb_mlp;
+int hb_mlp_create(hb_mlp **out, int device_ordinal, uint64_t seed);
+void hb_mlp_destroy(hb_mlp *m);
+int hb_mlp_set_params(hb_mlp *m, const float *params, const float *norm);
+int hb_mlp_get_params(hb_mlp *m, float *params, float *norm);
+/* mean-squared-error gradient over `count` samples (x: count*6, y: count) */
+int hb_mlp_gradient(hb_mlp *m, const float *x, const float *y, uint64_t count, float *grad, float *loss);
+/* Adam training (minibatch, shuffled per epoch); sets the input normalisation from x */
+int hb_mlp_train(hb_mlp *m, const float *x, const float *y, uint64_t count, int epochs, int batch,
+                 float lr, uint64_t seed, float *epoch_loss);
+int hb_mlp_predict(hb_mlp *m, const float *x, uint64_t count, float *y);
+/* Anytime ranking on the staged graph: HyperBall at precision p_feat for 4
+ * iterations, device-side features, MLP inference; pred (n floats) and features
+ * (n*6, optional) are copied out; ms = device time of the whole pipeline.
+ * Replaces this context's previous run. */
+int hb_surrogate(hb_ctx *ctx, hb_mlp *m, int p_feat, float *pred, float *features, double *ms);
+
 /* Number of nodes / arcs staged (0 if none). */
 uint64_t hb_num_nodes(const hb_ctx *ctx);
 uint64_t hb_num_arcs(const hb_ctx *ctx);

@@ -24,7 +24,39 @@ EXPORTS = (
     "hb_neighbourhood_function", "hb_set_schedule", "hb_num_nodes", "hb_num_arcs",
     "hb_graph_er", "hb_graph_rmat", "hb_graph_kmer", "hb_graph_nodes", "hb_graph_arcs",
     "hb_graph_offsets", "hb_graph_neighbours", "hb_graph_free",
+    "hb_batch_create", "hb_batch_destroy", "hb_batch_set_schedule", "hb_batch_run",
+    "hb_batch_graph_stats", "hb_batch_read_registers", "hb_batch_read_estimates",
+    "hb_batch_read_accumulators", "hb_autotune", "hb_batch_autotune",
+    "hb_set_snapshots", "hb_read_snapshots", "hb_mlp_create", "hb_mlp_destroy", "hb_mlp_set_params",
+    "hb_mlp_get_params", "hb_mlp_gradient", "hb_mlp_train", "hb_mlp_predict", "hb_surrogate",
 )
+
+
+class HbGraphDesc(C.Structure):
+    _fields_ = [
+        ("offsets", C.c_void_p),
+        ("off_bytes", C.c_int),
+        ("n", C.c_uint64),
+        ("nbrs", C.c_void_p),
+        ("nbr_bytes", C.c_int),
+        ("arcs", C.c_uint64),
+        ("p", C.c_int),
+        ("max_iters", C.c_int),
+    ]
+
+
+class HbBatchStats(C.Structure):
+    _fields_ = [
+        ("launches", C.c_int32),
+        ("iterations_max", C.c_int32),
+        ("iterations_total", C.c_uint64),
+        ("arcs_gathered", C.c_uint64),
+        ("tiles", C.c_uint32),
+        ("ms_total", C.c_double),
+        ("ms_init", C.c_double),
+        ("nodes_recomputed", C.c_uint64),
+        ("alg_bytes", C.c_uint64),
+    ]
 
 
 class HbStats(C.Structure):
@@ -93,6 +125,27 @@ def load() -> C.CDLL:
         "hb_graph_offsets": (u32p, [vp]),
         "hb_graph_neighbours": (u32p, [vp]),
         "hb_graph_free": (None, [vp]),
+        "hb_batch_create": (C.c_int, [C.POINTER(vp), C.c_int, C.POINTER(HbGraphDesc), C.c_int]),
+        "hb_batch_destroy": (None, [vp]),
+        "hb_batch_set_schedule": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_uint32]),
+        "hb_batch_run": (C.c_int, [vp, C.c_uint32, C.POINTER(HbBatchStats)]),
+        "hb_batch_graph_stats": (C.c_int, [vp, C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                           C.POINTER(C.c_uint64), C.c_int]),
+        "hb_batch_read_registers": (C.c_int, [vp, C.c_int, C.c_int, u8p]),
+        "hb_batch_read_estimates": (C.c_int, [vp, C.c_int, C.c_int, f32p]),
+        "hb_batch_read_accumulators": (C.c_int, [vp, C.c_int, f64p, f64p]),
+        "hb_autotune": (C.c_int, [vp, C.c_int, C.c_int, u32p, u32p, u32p, f64p]),
+        "hb_batch_autotune": (C.c_int, [vp, u32p, u32p, u32p, f64p]),
+        "hb_set_snapshots": (C.c_int, [vp, C.c_int]),
+        "hb_read_snapshots": (C.c_int, [vp, f32p]),
+        "hb_mlp_create": (C.c_int, [C.POINTER(vp), C.c_int, C.c_uint64]),
+        "hb_mlp_destroy": (None, [vp]),
+        "hb_mlp_set_params": (C.c_int, [vp, f32p, f32p]),
+        "hb_mlp_get_params": (C.c_int, [vp, f32p, f32p]),
+        "hb_mlp_gradient": (C.c_int, [vp, f32p, f32p, C.c_uint64, f32p, C.POINTER(C.c_float)]),
+        "hb_mlp_train": (C.c_int, [vp, f32p, f32p, C.c_uint64, C.c_int, C.c_int, C.c_float, C.c_uint64, f32p]),
+        "hb_mlp_predict": (C.c_int, [vp, f32p, C.c_uint64, f32p]),
+        "hb_surrogate": (C.c_int, [vp, vp, C.c_int, f32p, f32p, C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

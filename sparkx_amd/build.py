@@ -13,8 +13,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsparkx_hb.so")
 PROBE = os.path.join(HERE, "hbm_copy_probe")  # PMC calibration / streaming-bandwidth probe
 
-SOURCES = ["hb_kernels.hip", "hb_api.hip", "hb_graphgen.cpp"]
-HEADERS = ["hb_device.h", "hb_kernels.h", "hll_bias_table.inc"]
+SOURCES = ["hb_kernels.hip", "hb_api.hip", "hb_batch.hip", "hb_mlp.hip", "hb_graphgen.cpp"]
+HEADERS = ["hb_device.h", "hb_kernels.h", "hb_union.h", "hll_bias_table.inc"]
 
 # -ffp-contract=off: the f64 accumulators and f32 estimates must not fuse a*b+c
 # (the reference never does); packed-fp32 instruction selection is disabled.
@@ -30,12 +30,28 @@ def hipcc() -> str:
 
 
 def build(verbose: bool = False) -> str:
-    cmd = [hipcc(), *FLAGS, *[os.path.join(CSRC, s) for s in SOURCES], "-o", LIB + ".tmp"]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed ({res.returncode}):\n{res.stdout}\n{res.stderr}")
+    import tempfile
+    compile_flags = [f for f in FLAGS if f != "-shared"]
+    with tempfile.TemporaryDirectory(prefix="sparkx_hb_build_") as tmp:
+        objs, procs = [], []
+        for src in SOURCES:  # one hipcc per source, in parallel
+            obj = os.path.join(tmp, src + ".o")
+            cmd = [hipcc(), *compile_flags, "-c", os.path.join(CSRC, src), "-o", obj]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)))
+            objs.append(obj)
+        errors = []
+        for src, pr in procs:
+            out, err = pr.communicate()
+            if pr.returncode != 0:
+                errors.append(f"{src}:\n{out}\n{err}")
+        if errors:
+            raise RuntimeError("hipcc failed:\n" + "\n".join(errors))
+        link = [hipcc(), *FLAGS, *objs, "-o", LIB + ".tmp"]
+        res = subprocess.run(link, capture_output=True, text=True)
+        if res.returncode != 0:
+            raise RuntimeError(f"link failed ({res.returncode}):\n{res.stdout}\n{res.stderr}")
     os.replace(LIB + ".tmp", LIB)
     probe = [hipcc(), "--offload-arch=gfx950", "-O3", "-Xclang", "-target-feature", "-Xclang",
              "-packed-fp32-ops", os.path.join(CSRC, "hbm_copy_probe.hip"), "-o", PROBE]

@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -73,6 +74,8 @@ struct DevBuf {
 
 }  // namespace
 
+void hb::set_error(const char *msg) { g_err = msg; }
+
 struct hb_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -99,6 +102,8 @@ struct hb_ctx {
     std::vector<HbIterRec> h_rec;
     uint32_t sched_p = 0, sched_key[4] = {};
     uint32_t n_linked = 0;
+    uint32_t snap_k = 0;
+    DevBuf<float> snaps, feat, pred;
 
     void release_run()
     {
@@ -118,6 +123,9 @@ struct hb_ctx {
         chunk_flag.release();
         st.release();
         rec.release();
+        snaps.release();
+        feat.release();
+        pred.release();
         ran = false;
         sched_p = 0;
     }
@@ -398,6 +406,7 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
     c->rec_cap = cap;
     HB_TRY(c->lc.ensure(m + 1));
     HB_TRY(c->bias.ensure(2 * HB_BIAS_K));
+    if (c->snap_k) HB_TRY(c->snaps.ensure((size_t)c->snap_k * c->n));
     {
         int rc = build_schedule(c, p);
         if (rc != HB_OK) return rc;
@@ -430,6 +439,8 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
     d.bias_val = c->bias.ptr + HB_BIAS_K;
     d.scratch = c->scratch.ptr;
     d.chunk_flag = c->chunk_flag.ptr;
+    d.snaps = c->snap_k ? c->snaps.ptr : nullptr;
+    d.snap_k = c->snap_k;
     d.n = n;
     d.n_linked = c->n_linked;
     d.p = (uint32_t)p;
@@ -565,6 +576,10 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
     if (rc != HB_OK) return rc;
 
     const uint32_t executed = c->h_st.executed;
+    // snapshot rows past the last iteration repeat its estimates (the sketches stopped changing)
+    for (uint32_t t = executed + 1; t <= c->snap_k; ++t)
+        HB_TRY(hipMemcpyAsync(c->snaps.ptr + (size_t)(t - 1) * c->n, c->est.ptr, c->n * 4, hipMemcpyDeviceToDevice,
+                              c->stream));
     c->h_rec.assign(executed + 1, HbIterRec{});
     HB_TRY(hipMemcpy(c->h_rec.data(), c->rec.ptr, (executed + 1) * sizeof(HbIterRec), hipMemcpyDeviceToHost));
     c->p = p;
@@ -692,4 +707,112 @@ extern "C" int hb_neighbourhood_function(hb_ctx *c, double *dst, int cap)
     }
     g_err.clear();
     return k;
+}
+
+// Coordinate-descent autotuner over the degree-bin and hub-chunk knobs: each
+// candidate runs the first `trial_iters` iterations (the dense, most expensive
+// ones) twice and keeps its best device time.  Leaves the best schedule set.
+extern "C" int hb_autotune(hb_ctx *c, int p, int trial_iters, uint32_t *small_max_deg,
+                           uint32_t *wave_max_deg, uint32_t *chunk, double *trial_ms)
+{
+    if (!c) return fail(HB_EINVAL, "hb_autotune: ctx is NULL");
+    if (trial_iters <= 0) trial_iters = 3;
+    std::unique_ptr<hb_stats> st(new hb_stats());
+    auto trial = [&](uint32_t s, uint32_t w, uint32_t k, double *best) -> int {
+        int rc = hb_set_schedule(c, s, w, k);
+        if (rc != HB_OK) return rc;
+        *best = 1e300;
+        for (int r = 0; r < 2; ++r) {
+            rc = hb_run(c, p, trial_iters, 0, st.get());
+            if (rc != HB_OK) return rc;
+            *best = std::min(*best, st->ms_total);
+        }
+        return HB_OK;
+    };
+    uint32_t knob[3] = {c->small_max, c->wave_max, c->chunk};
+    const uint32_t cand[3][4] = {{2, 4, 8, 16}, {512, 1024, 4096, 16384}, {512, 1024, 2048, 8192}};
+    double best = 0.0;
+    int rc = trial(knob[0], knob[1], knob[2], &best);
+    if (rc != HB_OK) return rc;
+    if (p <= 10) {
+        for (int k = 0; k < 3; ++k) {
+            for (uint32_t v : cand[k]) {
+                if (v == knob[k]) continue;
+                uint32_t t[3] = {knob[0], knob[1], knob[2]};
+                t[k] = v;
+                if (t[1] < t[0]) continue;
+                double ms = 0.0;
+                rc = trial(t[0], t[1], t[2], &ms);
+                if (rc != HB_OK) return rc;
+                if (ms < best) {
+                    best = ms;
+                    knob[k] = v;
+                }
+            }
+        }
+    }
+    rc = hb_set_schedule(c, knob[0], knob[1], knob[2]);
+    if (rc != HB_OK) return rc;
+    if (small_max_deg) *small_max_deg = knob[0];
+    if (wave_max_deg) *wave_max_deg = knob[1];
+    if (chunk) *chunk = knob[2];
+    if (trial_ms) *trial_ms = best;
+    c->ran = false;  // the trial results are not a completed run
+    g_err.clear();
+    return HB_OK;
+}
+
+extern "C" int hb_set_snapshots(hb_ctx *c, int k)
+{
+    if (!c) return fail(HB_EINVAL, "hb_set_snapshots: ctx is NULL");
+    if (k < 0 || k > 16) return fail(HB_EINVAL, "hb_set_snapshots: k=%d outside 0..16", k);
+    c->snap_k = (uint32_t)k;
+    g_err.clear();
+    return HB_OK;
+}
+
+extern "C" int hb_read_snapshots(hb_ctx *c, float *dst)
+{
+    if (!c || !dst) return fail(HB_EINVAL, "hb_read_snapshots: NULL argument");
+    if (!c->ran || !c->snap_k || !c->snaps.ptr) return fail(HB_ESTATE, "hb_read_snapshots: no run with snapshots");
+    if (!set_device(c)) return fail(HB_EHIP, "hipSetDevice failed");
+    HB_TRY(hipStreamSynchronize(c->stream));
+    HB_TRY(hipMemcpy(dst, c->snaps.ptr, (size_t)c->snap_k * c->n * 4, hipMemcpyDeviceToHost));
+    g_err.clear();
+    return HB_OK;
+}
+
+extern "C" int hb_surrogate(hb_ctx *c, hb_mlp *mlp, int p_feat, float *pred, float *features, double *ms)
+{
+    if (!c || !mlp) return fail(HB_EINVAL, "hb_surrogate: NULL argument");
+    if (!c->off.ptr || c->n == 0) return fail(HB_ESTATE, "hb_surrogate: no graph staged");
+    if (hb::mlp_device(mlp) != c->device) return fail(HB_EINVAL, "hb_surrogate: model and graph on different devices");
+    if (!set_device(c)) return fail(HB_EHIP, "hipSetDevice failed");
+    const uint32_t keep = c->snap_k;
+    c->snap_k = 4;
+    hipEvent_t e0, e1;
+    HB_TRY(hipEventCreate(&e0));
+    HB_TRY(hipEventCreate(&e1));
+    HB_TRY(hipEventRecord(e0, c->stream));
+    int rc = hb_run(c, p_feat, 4, 0, nullptr);
+    c->snap_k = keep;
+    if (rc != HB_OK) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return rc;
+    }
+    HB_TRY(c->feat.ensure(c->n * 6));
+    HB_TRY(c->pred.ensure(c->n));
+    HB_TRY(hb::launch_surrogate_tail(mlp, c->off.ptr, c->snaps.ptr, (uint32_t)c->n, c->feat.ptr, c->pred.ptr, c->stream));
+    HB_TRY(hipEventRecord(e1, c->stream));
+    HB_TRY(hipEventSynchronize(e1));
+    float t = 0.0f;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ms) *ms = t;
+    if (pred) HB_TRY(hipMemcpy(pred, c->pred.ptr, c->n * 4, hipMemcpyDeviceToHost));
+    if (features) HB_TRY(hipMemcpy(features, c->feat.ptr, c->n * 24, hipMemcpyDeviceToHost));
+    g_err.clear();
+    return HB_OK;
 }

@@ -53,6 +53,8 @@ struct HbDev {
     HbIterRec *rec;        // indexed by t
     const float *lc;       // lc[z] = (float)(m ln(m/z)), z = 1..m
     const double *bias_raw, *bias_val;  // HLL++ points for this p
+    float *snaps;          // estimates after iterations 1..snap_k, iteration-major
+    uint32_t snap_k;
     uint8_t *scratch;      // partial sketches of hub chunks
     uint32_t *chunk_flag;  // 1 if the chunk was recomputed this iteration
     uint32_t n;
@@ -248,6 +250,9 @@ struct HbTables {
     const float *lc;
     const double *raw;
     const double *val;
+    float am;       // alpha_m * m * m, rounded to f32
+    float five_m;   // 5m
+    double lc_thr;  // HLL++ linear-counting threshold for p
 };
 
 #define HB_TABLES_LDS(P, d, name)                                                      \
@@ -259,7 +264,7 @@ struct HbTables {
         name##_val[i_] = (d).bias_val[i_];                                             \
     }                                                                                  \
     __syncthreads();                                                                   \
-    const HbTables name = {name##_lc, name##_raw, name##_val};
+    const HbTables name = {name##_lc, name##_raw, name##_val, (d).am, (d).five_m, (d).lc_thr};
 
 // Estimate of the sketch held by a G-lane copy (each lane: 16 registers in acc).
 // All lanes of the copy return the same value.
@@ -273,14 +278,14 @@ struct HbTables {
 // f32 products and quotients are formed in f64 and rounded once, which gives the
 // correctly rounded f32 result (53 >= 2*24 + 2).
 template <int G>
-__device__ __forceinline__ float hb_estimate_copy(const HbDev &d, const HbTables &tb, uint4 acc)
+__device__ __forceinline__ float hb_estimate_copy(const HbTables &tb, uint4 acc)
 {
     uint32_t z = hb_zero_bytes(acc.x) + hb_zero_bytes(acc.y) + hb_zero_bytes(acc.z) +
                  hb_zero_bytes(acc.w);
     z = hb_copy_sum<G>(z);
     if (z > 0) {
         const float lc = tb.lc[z];
-        if ((double)lc <= d.lc_thr) return lc;
+        if ((double)lc <= tb.lc_thr) return lc;
     }
     uint32_t mx = hb_max_byte(hb_bmax(hb_bmax(acc.x, acc.y), hb_bmax(acc.z, acc.w)));
     mx = hb_copy_max<G>(mx);
@@ -302,7 +307,7 @@ __device__ __forceinline__ float hb_estimate_copy(const HbDev &d, const HbTables
                 for (int b = 0; b < 4; ++b) S = __fadd_rn(S, hb_pow2neg((w[q] >> (8 * b)) & 0xffu));
         }
     }
-    float est = (float)((double)d.am / (double)S);
-    if (est <= d.five_m) est = (float)((double)est - hb_knn_bias(tb.raw, tb.val, (double)est, 200));
+    float est = (float)((double)tb.am / (double)S);
+    if (est <= tb.five_m) est = (float)((double)est - hb_knn_bias(tb.raw, tb.val, (double)est, 200));
     return est;
 }

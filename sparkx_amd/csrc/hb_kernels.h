@@ -6,6 +6,8 @@
 
 #include "hb_device.h"
 
+struct hb_mlp;
+
 namespace hb {
 
 // Degree-binned schedule of one run (device pointers).
@@ -21,6 +23,9 @@ struct HbSchedule {
     unsigned max_blocks = 8192;        // grid cap of the persistent loops
 };
 
+// thread-local message returned by hb_last_error() (set by every C-ABI entry point)
+void set_error(const char *msg);
+
 hipError_t launch_init(const HbDev &d, hipStream_t s);
 hipError_t launch_fill(uint32_t *p, uint64_t count, uint32_t value, hipStream_t s);
 // ev (nullable): 2 * HB_NSLOT events recorded around each kernel slot that launches
@@ -29,5 +34,10 @@ hipError_t launch_iteration(const HbDev &d, const HbSchedule &sc, bool nf, hipSt
 hipError_t launch_estimates(const HbDev &d, const uint8_t *regs, float *out, hipStream_t s);
 hipError_t launch_centrality(const HbDev &d, const float *est, int kind, int normalize, double *out,
                              hipStream_t s);
+
+// surrogate tail on a staged graph: features from snapshots, MLP inference
+hipError_t launch_surrogate_tail(struct ::hb_mlp *n, const uint32_t *off, const float *snaps, uint32_t nodes,
+                                 float *x_dev, float *out_dev, hipStream_t s);
+int mlp_device(const struct ::hb_mlp *n);
 
 }  // namespace hb

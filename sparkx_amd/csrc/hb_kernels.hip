@@ -16,15 +16,11 @@
 // identical to recomputing every node.
 #include "hb_device.h"
 #include "hb_kernels.h"
+#include "hb_union.h"
 
 namespace hb {
 
 static constexpr int kBlock = 256;
-
-__device__ __forceinline__ bool bit_test(const uint32_t *bits, uint32_t v)
-{
-    return (bits[v >> 5] >> (v & 31u)) & 1u;
-}
 
 // block-wide sums of per-lane counters, one atomic per counter per block
 __device__ __forceinline__ void block_add4(unsigned long long a, unsigned long long b,
@@ -95,7 +91,7 @@ __global__ __launch_bounds__(kBlock) void k_init(HbDev d)
     *(uint4 *)(d.regs0 + ((uint64_t)u << P) + cl * 16) = r;
     // a node without neighbours keeps c_0 forever and is never scheduled: both buffers
     if (d.off[u + 1] == d.off[u]) *(uint4 *)(d.regs1 + ((uint64_t)u << P) + cl * 16) = r;
-    const float e = hb_estimate_copy<G>(d, tb, r);
+    const float e = hb_estimate_copy<G>(tb, r);
     if (cl == 0) {
         d.est[u] = e;
         d.dist[u] = 0.0;
@@ -127,76 +123,29 @@ template <int P, int T>
 __global__ __launch_bounds__(kBlock) void k_union(HbDev d, const uint32_t *__restrict__ list,
                                                   uint32_t count, int slot_id)
 {
-    constexpr uint32_t M = 1u << P, G = M / 16, S = T / G;
-    static_assert(T >= (int)G && T <= HB_WAVE, "team width");
     const HbIterState *st = d.st;
     if (st->done) return;
     HB_TABLES_LDS(P, d, tb)
     const uint32_t it = st->iter;
-    const double t = st->t, inv_t = st->inv_t;
-    const uint8_t *__restrict__ in = st->parity ? d.regs1 : d.regs0;
-    uint8_t *__restrict__ out = st->parity ? d.regs0 : d.regs1;
-    const uint32_t *__restrict__ bitsR = (it & 1u) ? d.bits0 : d.bits1;
-    uint32_t *bitsW = (it & 1u) ? d.bits1 : d.bits0;
-    const bool use_frontier = d.frontier && it > 1 && !st->skip_scan;
-
-    const uint32_t tl = threadIdx.x % T, cl = tl % G, slot = tl / G;
+    HbView v;
+    v.off = d.off;
+    v.nbr = d.nbr;
+    v.in = st->parity ? d.regs1 : d.regs0;
+    v.out = st->parity ? d.regs0 : d.regs1;
+    v.est = d.est;
+    v.dist = d.dist;
+    v.inv = d.inv;
+    v.bitsR = (it & 1u) ? d.bits0 : d.bits1;
+    v.bitsW = (it & 1u) ? d.bits1 : d.bits0;
+    v.t = st->t;
+    v.inv_t = st->inv_t;
+    v.use_frontier = d.frontier && it > 1 && !st->skip_scan;
+    const uint32_t tl = threadIdx.x % T;
     const uint32_t teams_per_grid = gridDim.x * (kBlock / T);
-    unsigned long long c_inc = 0, c_arcs = 0, c_nodes = 0, c_chg = 0;
-
-    for (uint32_t team = blockIdx.x * (kBlock / T) + threadIdx.x / T; team < count;
-         team += teams_per_grid) {
-        const uint32_t u = list[team];
-        const uint32_t e0 = d.off[u], deg = d.off[u + 1] - e0;
-        const uint32_t *__restrict__ nb = d.nbr + e0;
-        bool active = true;
-        if (use_frontier) {
-            uint32_t any = 0;
-            for (uint32_t j = tl; j < deg; j += T) any |= bit_test(bitsR, nb[j]);
-            active = hb_copy_or<T>(any) != 0;
-        }
-        if (!active) {
-            if (slot == 0 && bit_test(bitsR, u))
-                *(uint4 *)(out + ((uint64_t)u << P) + cl * 16) =
-                    *(const uint4 *)(in + ((uint64_t)u << P) + cl * 16);
-            continue;
-        }
-        const uint4 own = *(const uint4 *)(in + ((uint64_t)u << P) + cl * 16);
-        HbAcc pa = hb_acc(own);
-        uint32_t j = slot;
-        for (; j + 3 * S < deg; j += 4 * S) {
-            const uint32_t v0 = nb[j], v1 = nb[j + S], v2 = nb[j + 2 * S], v3 = nb[j + 3 * S];
-            const uint4 x0 = *(const uint4 *)(in + ((uint64_t)v0 << P) + cl * 16);
-            const uint4 x1 = *(const uint4 *)(in + ((uint64_t)v1 << P) + cl * 16);
-            const uint4 x2 = *(const uint4 *)(in + ((uint64_t)v2 << P) + cl * 16);
-            const uint4 x3 = *(const uint4 *)(in + ((uint64_t)v3 << P) + cl * 16);
-            hb_acc_max(pa, x0);
-            hb_acc_max(pa, x1);
-            hb_acc_max(pa, x2);
-            hb_acc_max(pa, x3);
-        }
-        for (; j < deg; j += S) hb_acc_max(pa, *(const uint4 *)(in + ((uint64_t)nb[j] << P) + cl * 16));
-        uint4 acc = hb_acc_get(pa);
-#pragma unroll
-        for (uint32_t s = G; s < T; s <<= 1) acc = hb_bmax4(acc, hb_shfl_xor4<T>(acc, s));
-
-        const uint32_t diffw = (acc.x != own.x) | (acc.y != own.y) | (acc.z != own.z) | (acc.w != own.w);
-        const bool changed = hb_copy_or<G>(diffw) != 0;
-        const float curr = hb_estimate_copy<G>(d, tb, acc);
-        if (slot == 0) *(uint4 *)(out + ((uint64_t)u << P) + cl * 16) = acc;
-        if (tl == 0) {
-            const float prev = d.est[u];
-            if (accumulate(d, u, prev, curr, t, inv_t)) c_inc += 1;
-            d.est[u] = curr;
-            if (changed) {
-                atomicOr(bitsW + (u >> 5), 1u << (u & 31u));
-                c_chg += 1;
-            }
-            c_arcs += deg;
-            c_nodes += 1;
-        }
-    }
-    block_add4(c_inc, c_arcs, c_nodes, c_chg, d.rec + it, slot_id);
+    HbCounts c;
+    for (uint32_t team = blockIdx.x * (kBlock / T) + threadIdx.x / T; team < count; team += teams_per_grid)
+        union_node<P, T>(v, tb, list[team], tl, c);
+    block_add4(c.inc, c.arcs, c.nodes, c.chg, d.rec + it, slot_id);
 }
 
 // ---------------------------------------------------------------------------
@@ -298,7 +247,7 @@ __global__ __launch_bounds__(kBlock) void k_merge(HbDev d, const uint32_t *__res
         for (uint32_t s = G; s < T; s <<= 1) acc = hb_bmax4(acc, hb_shfl_xor4<T>(acc, s));
         const uint32_t diffw = (acc.x != own.x) | (acc.y != own.y) | (acc.z != own.z) | (acc.w != own.w);
         const bool changed = hb_copy_or<G>(diffw) != 0;
-        const float curr = hb_estimate_copy<G>(d, tb, acc);
+        const float curr = hb_estimate_copy<G>(tb, acc);
         if (slot == 0) *(uint4 *)(out + ((uint64_t)u << P) + cl * 16) = acc;
         if (tl == 0) {
             const float prev = d.est[u];
@@ -525,6 +474,15 @@ __global__ __launch_bounds__(kBlock) void k_nf(HbDev d)
     }
 }
 
+// estimates after iteration t <= snap_k (surrogate inputs)
+__global__ __launch_bounds__(kBlock) void k_snapshot(HbDev d)
+{
+    const HbIterState *st = d.st;
+    if (st->done || st->iter > d.snap_k) return;
+    float *dst = d.snaps + (uint64_t)(st->iter - 1) * d.n;
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < d.n; u += gridDim.x * blockDim.x) dst[u] = d.est[u];
+}
+
 // end of iteration t: termination rule (hyper_ball.rs:833-849) and buffer swap
 __global__ void k_finalize(HbDev d)
 {
@@ -562,7 +520,7 @@ __global__ __launch_bounds__(kBlock) void k_estimates(HbDev d, const uint8_t *__
     const uint32_t u = (uint32_t)(gtid / G), cl = (uint32_t)(gtid % G);
     if (u >= d.n) return;
     const uint4 r = *(const uint4 *)(regs + ((uint64_t)u << P) + cl * 16);
-    const float e = hb_estimate_copy<G>(d, tb, r);
+    const float e = hb_estimate_copy<G>(tb, r);
     if (cl == 0) out[u] = e;
 }
 
@@ -700,6 +658,11 @@ hipError_t launch_iteration(const HbDev &d, const HbSchedule &sc, bool nf, hipSt
     }
     if (nf) {
         k_nf<<<grid_for(d.n, 1024), kBlock, 0, s>>>(d);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    if (d.snap_k) {
+        k_snapshot<<<grid_for(d.n, 4096), kBlock, 0, s>>>(d);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

@@ -200,6 +200,14 @@ class HyperBall:
             return self.compute_lins_centrality()
         return self._lin
 
+    def autotune(self, trial_iters: int = 3):
+        """Picks the fastest degree-bin / hub-chunk schedule for this graph (results are
+        unchanged by any choice); call before run()."""
+        s, w, k, ms = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_double()
+        _lib.check(self._L.hb_autotune(self._ctx, self.p, trial_iters, C.byref(s), C.byref(w), C.byref(k),
+                                       C.byref(ms)))
+        return (s.value, w.value, k.value), ms.value
+
     def neighbourhood_function(self) -> np.ndarray:
         """N(t) = sum_u |B(u, t)| estimates, t = 1..iterations (needs nf=True)."""
         it = self.get_iters()

@@ -130,3 +130,47 @@ def _build_cpp_mirror(tmp_path):
 def test_cpp_mirror_compiles_and_links(tmp_path):
     """The header-only C++ mirror of HyperBallInner builds against the C ABI."""
     assert os.path.exists(_build_cpp_mirror(tmp_path))
+
+
+def _write_dbg_mtx(path, sigma, k):
+    # MatrixMarket symmetric pattern file of the complete de Bruijn graph on
+    # sigma^k nodes: node u -> ((u mod sigma^(k-1)) * sigma + c), 1-based
+    n, base = sigma ** k, sigma ** (k - 1)
+    with open(path, "w") as fh:
+        fh.write("%%MatrixMarket matrix coordinate pattern symmetric\n% de Bruijn\n")
+        fh.write(f"{n} {n} {sigma ** (k + 1)}\n")
+        for u in range(n):
+            for c in range(sigma):
+                fh.write(f"{u + 1} {(u % base) * sigma + c + 1}\n")
+
+
+def test_mtx_reader_matches_reference_rules(tmp_path, oracle):
+    from sparkx_amd.cache import read_mtx
+    path = str(tmp_path / "dbg.mtx")
+    _write_dbg_mtx(path, 4, 5)
+    g = read_mtx(path)
+    off, nb = oracle.dbg_mtx_csr(4, 5)
+    assert np.array_equal(g.offsets, off) and np.array_equal(g.neighbours, nb.astype(np.uint64))
+
+
+def test_cache_ids_and_names():
+    from sparkx_amd import cache
+    # the id of the reference's committed exact-closeness fixture for this dataset
+    assert cache.graph_id("./datasets/graphs/synth_gen_dbg_protein20_k2_edges16k.mtx") == \
+        "46dfbb599cc933be7aca48553950fc09d8418d0cecb287a2598df9df16821730"
+    gid = "abc"
+    assert cache.cache_file("/c", "distances", gid).endswith("/hyperballdistances_abc.mmap")
+    assert cache.cache_file("/c", "closeness", gid, cache.centrality_seq("closeness", True)).endswith(
+        "/hyperballcloseness_2_abc.mmap")
+    assert cache.cache_file("/c", "harmonic", gid, cache.centrality_seq("harmonic", False)).endswith(
+        "/hyperballharmonic_1_abc.mmap")
+    assert cache.cache_file("/c", "lin", gid, cache.centrality_seq("lin", False)).endswith("/hyperballlin_0_abc.mmap")
+
+
+def test_graph_cache_roundtrip(tmp_path):
+    from sparkx_amd import Graph, cache
+    g = Graph.erdos_renyi(500, 2000, seed=3)
+    cache.save_graph_cache(g, str(tmp_path), "gid")
+    h = cache.load_cached_graph(str(tmp_path), "gid")
+    assert h.offsets.dtype == np.dtype("<u8")
+    assert np.array_equal(h.offsets, g.offsets) and np.array_equal(h.neighbours, g.neighbours)

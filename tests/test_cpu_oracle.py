@@ -307,3 +307,41 @@ def test_metrics_reference_semantics(oracle):
     assert oracle.spearman(np.array([1.0]), np.array([2.0])) == 1.0
     assert oracle.mae(np.array([1.0, 3.0]), np.array([2.0, 1.0])) == 1.5
     assert oracle.mape(np.array([1.0, 3.0]), np.array([2.0, 0.0])) == 100 * 0.5 / 2
+
+
+# ---------------------------------------------------------------------------
+# surrogate MLP oracle and evaluation metrics
+# ---------------------------------------------------------------------------
+def test_mlp_oracle_backward_matches_finite_differences():
+    from oracle import mlp_oracle as M
+    rng = np.random.default_rng(5)
+    params = rng.normal(0, 0.3, M.N_PARAMS)
+    x = rng.normal(0, 1, (40, 6)).astype(np.float32)
+    t = rng.normal(0, 1, 40)
+    mean, std = np.zeros(6, np.float32), np.ones(6, np.float32)
+    _, g = M.loss_and_grad(params, x, t, mean, std, emulate_bf16=False)
+    idx = rng.choice(M.N_PARAMS, 60, replace=False).tolist() + [M.B3, M.W1, M.B1 + 3, M.W3 + 7]
+    fd = M.finite_difference(params, x, t, mean, std, idx)
+    assert np.allclose(g[idx], fd, rtol=1e-5, atol=1e-7)
+
+
+def test_mlp_oracle_bf16_rounding():
+    from oracle import mlp_oracle as M
+    v = np.array([1.0, 1.00390625, 1.001953125, 1.005859375, -3.14159, 0.0], np.float32)
+    # bf16 spacing at 1 is 2^-7: 1 + 2^-8 is a tie -> even (1.0); 1 + 2^-9 rounds down;
+    # 1 + 3*2^-9 rounds up to 1 + 2^-7
+    assert list(M.bf16(v)[:4]) == [1.0, 1.0, 1.0, 1.0078125]
+    assert M.bf16(v)[4] == np.float32(-3.140625)
+
+
+def test_package_metrics_match_oracle(oracle):
+    from sparkx_amd import metrics
+    rng = np.random.default_rng(9)
+    for n in (1, 2, 17, 1000):
+        a = rng.integers(0, 5, n).astype(np.float64)  # many ties
+        b = rng.normal(size=n)
+        assert np.array_equal(metrics.ranks(a), oracle.ranks(a).astype(np.int64))
+        assert metrics.spearman_rho(a, b) == oracle.spearman(a, b)
+        assert abs(metrics.mae(a, b) - oracle.mae(a, b)) <= 1e-12 * max(1, abs(oracle.mae(a, b)))
+    assert metrics.top_overlap([3, 2, 1, 0], [3, 2, 0, 1], 0.5) == 1.0
+    assert metrics.top_overlap([0, 1, 2, 3], [3, 2, 1, 0], 0.25) == 0.0

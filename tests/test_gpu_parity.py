@@ -274,3 +274,203 @@ def test_cpp_mirror_matches_python_mirror(hb, oracle, tmp_path):
     for u in range(8):
         _, uu, d, cc, ll = lines[1 + u].split()
         assert (float(d), float(cc), float(ll)) == (ref.dist[u], c[u], lin[u])
+
+
+def _compare_batch_graph(bt, i, h, ctx):
+    from sparkx_amd import _lib
+    it, conv, inc = bt.graph_stats(i)
+    assert it == h.stats.iterations, ctx
+    assert list(inc) == list(h.stats.increases[:it]), ctx
+    for which in (_lib.HB_REGS_REFERENCE, _lib.HB_REGS_LATEST):
+        assert np.array_equal(bt.registers(i, which), h.registers(which)), ctx
+    d, v = bt.accumulators(i)
+    hd, hv = h._accumulators()
+    assert np.array_equal(d, hd) and np.array_equal(v, hv), ctx
+    assert np.array_equal(bt.estimates(i), h.estimates()), ctx
+
+
+def test_batch_64_graphs_match_single_graph_runs(hb):
+    """Batched configuration: 64 seeded graphs (32 ER + 32 R-MAT, n in [2^12, 2^18],
+    mixed p = 4..8) in one work-queue launch per iteration, hipGraph-replayed; every
+    graph bit-identical to its own single-graph run."""
+    from sparkx_amd.batch import HyperBallBatch, batched_workload
+    items = batched_workload(64, seed=1)
+    say("batch64", sum(g.size() for g, _ in items), sum(g.width() for g, _ in items))
+    with HyperBallBatch(items) as bt:
+        st = bt.run(graph_capture=True)
+        assert st.tiles > 0 and st.iterations_max > 0
+        for i, (g, p) in enumerate(items):
+            with hb.HyperBall(g, p) as h:
+                _compare_batch_graph(bt, i, h, f"graph {i} p={p}")
+
+
+def test_batch_small_graphs_against_oracle_and_schedules(hb, oracle):
+    from sparkx_amd.batch import HyperBallBatch
+    items = []
+    for i, p in enumerate((4, 5, 6, 7, 8, 6)):
+        g = hb.Graph.erdos_renyi(700 + 100 * i, 2000 + 500 * i, seed=50 + i) if i % 2 else hb.Graph.rmat(10, 8, seed=60 + i)
+        items.append((g, p))
+    refs = [_oracle_run(oracle, g, p) for g, p in items]
+    variants = [dict(graph_capture=True), dict(graph_capture=False), dict(dense=True),
+                dict(schedule=(2, 16, 512)), dict(schedule=(16, 256, 65536))]
+    from sparkx_amd import _lib
+    with HyperBallBatch(items) as bt:
+        for kw in variants:
+            say("batch variant", kw)
+            sched = kw.pop("schedule", None)
+            if sched:
+                bt.set_schedule(*sched)
+            bt.run(**kw)
+            for i, ((g, p), ref) in enumerate(zip(items, refs)):
+                it, conv, inc = bt.graph_stats(i)
+                assert it == ref.iterations and list(inc) == list(ref.increases), (kw, i)
+                assert np.array_equal(bt.registers(i, _lib.HB_REGS_REFERENCE).ravel(), ref.regs_ref), (kw, i)
+                assert np.array_equal(bt.registers(i, _lib.HB_REGS_LATEST).ravel(), ref.regs_last), (kw, i)
+                d, v = bt.accumulators(i)
+                assert np.array_equal(d, ref.dist) and np.array_equal(v, ref.inv), (kw, i)
+        knobs, ms = bt.autotune()
+        say("batch autotune", knobs, ms)
+        bt.run()
+        for i, ((g, p), ref) in enumerate(zip(items, refs)):
+            assert np.array_equal(bt.registers(i, _lib.HB_REGS_REFERENCE).ravel(), ref.regs_ref), i
+
+
+def test_batch_truncation_and_errors(hb, oracle):
+    from sparkx_amd import HyperBallError, _lib
+    from sparkx_amd.batch import HyperBallBatch
+    g = hb.Graph.erdos_renyi(2000, 6000, seed=77)
+    ref = _oracle_run(oracle, g, 6, max_iters=3)
+    with HyperBallBatch([(g, 6), (g, 7)], max_iters=3) as bt:
+        bt.run()
+        it, conv, inc = bt.graph_stats(0)
+        assert it == 3 and list(inc) == list(ref.increases)
+        assert np.array_equal(bt.registers(0, _lib.HB_REGS_REFERENCE).ravel(), ref.regs_ref)
+    with pytest.raises(HyperBallError) as ei:
+        HyperBallBatch([(g, 9)])
+    assert ei.value.code == _lib.HB_ERANGE
+
+
+def test_autotune_keeps_results(hb, oracle):
+    g = hb.Graph.rmat(13, 16, seed=12)
+    p = 7
+    ref = _oracle_run(oracle, g, p)
+    with hb.HyperBall(g, p, _run=False) as h:
+        knobs, ms = h.autotune(trial_iters=2)
+        say("autotune", knobs, ms)
+        assert ms > 0
+        h.run()
+        _assert_same(h, ref, oracle, p, f"autotuned {knobs}")
+
+
+# ---------------------------------------------------------------------------
+# learned surrogate: BF16 MFMA kernels against the CPU oracle
+# ---------------------------------------------------------------------------
+def _mlp_case(seed=3, n=1000):
+    from oracle import mlp_oracle as M
+    rng = np.random.default_rng(seed)
+    params = rng.normal(0, 0.25, M.N_PARAMS).astype(np.float32)
+    x = np.column_stack([rng.integers(0, 50, n), rng.normal(1, 1, n), rng.normal(2, 1, (n, 4))]).astype(np.float32)
+    t = rng.normal(1.0, 0.5, n).astype(np.float32)
+    norm = np.concatenate([x.mean(0), x.std(0)]).astype(np.float32)
+    return params, norm, x, t
+
+
+def test_mlp_gradient_and_forward_match_oracle(hb):
+    from oracle import mlp_oracle as M
+    from sparkx_amd.surrogate import Surrogate
+    params, norm, x, t = _mlp_case()
+    s = Surrogate()
+    s.set_params(params, norm)
+    say("mlp gradient")
+    g, loss = s.gradient(x, t)
+    ref_loss, ref_g = M.loss_and_grad(params.astype(np.float64), x, t, norm[:6], norm[6:])
+    assert abs(loss - ref_loss) <= 1e-5 * ref_loss
+    # f32 MFMA accumulation vs f64: tolerance relative to the gradient's scale
+    scale = np.abs(ref_g).max()
+    assert np.max(np.abs(g - ref_g)) <= 2e-4 * scale, np.max(np.abs(g - ref_g)) / scale
+    y = s.predict(x)
+    ref_y, _ = M.forward(params.astype(np.float64), x, norm[:6], norm[6:])
+    assert np.max(np.abs(y - ref_y)) <= 1e-5 * max(1.0, np.abs(ref_y).max())
+    s.close()
+
+
+def test_mlp_adam_step_matches_oracle(hb):
+    from oracle import mlp_oracle as M
+    from sparkx_amd.surrogate import Surrogate
+    params, norm, x, t = _mlp_case(seed=4, n=512)
+    s = Surrogate()
+    s.set_params(params, norm)
+    g, _ = s.gradient(x, t)
+    # one epoch with one minibatch = one Adam step; training recomputes the normalisation
+    s.train(x, t, epochs=1, batch=x.shape[0], lr=1e-3, seed=1)
+    p1, n1 = s.params()
+    mean = x.astype(np.float64).mean(0)
+    assert np.allclose(n1[:6], mean, rtol=1e-5, atol=1e-6)
+    s.set_params(params, n1)
+    g1, _ = s.gradient(x, t)
+    ref, _, _ = M.adam(params.astype(np.float64), g1.astype(np.float64), np.zeros(M.N_PARAMS),
+                       np.zeros(M.N_PARAMS), 1, lr=1e-3)
+    assert np.max(np.abs(p1 - ref)) <= 1e-6 + 1e-5 * np.abs(ref).max()
+    s.close()
+
+
+def test_snapshots_match_oracle_estimates(hb, oracle):
+    import ctypes as C
+    from sparkx_amd import _lib
+    g = hb.Graph.rmat(12, 8, seed=31)
+    trace = {}
+    _oracle_run(oracle, g, 4, on_iter=lambda t, r, d, i: trace.__setitem__(t, r))
+    with hb.HyperBall(g, 4, _run=False) as h:
+        _lib.check(h._L.hb_set_snapshots(h._ctx, 4))
+        h.run(max_iters=4)
+        snaps = np.empty((4, g.size()), np.float32)
+        _lib.check(h._L.hb_read_snapshots(h._ctx, _lib.ptr(snaps, C.c_float)))
+    for t in range(1, 5):
+        assert np.array_equal(snaps[t - 1], oracle.estimates(4, trace[t])), t
+
+
+def test_surrogate_beats_truncated_hyperball(hb):
+    """Trained on small R-MAT graphs, the MLP ranks a held-out graph's harmonic
+    centrality better than the 4-iteration HyperBall accumulator it starts from."""
+    from sparkx_amd import metrics
+    from sparkx_amd.surrogate import Surrogate, labelled_graph
+    xs, ys = [], []
+    for seed in range(4):
+        say("surrogate train graph", seed)
+        x, y, _, _ = labelled_graph(hb.Graph.rmat(13, 16, seed=100 + seed))
+        xs.append(x)
+        ys.append(y)
+    s = Surrogate(seed=7)
+    losses = s.train(np.concatenate(xs), np.concatenate(ys), epochs=6, batch=4096, lr=1e-3)
+    assert losses[-1] < losses[0]
+    g = hb.Graph.rmat(14, 16, seed=999)
+    x, y, trunc, _ = labelled_graph(g)
+    with hb.HyperBall(g, 4, _run=False) as h:
+        pred, feat, ms = s.rank(h, want_features=True)
+    assert ms > 0
+    assert np.allclose(feat, x, rtol=1e-6, atol=1e-6)
+    assert np.array_equal(pred, s.predict(feat))
+    truth = np.expm1(y.astype(np.float64))
+    linked = g.degrees() > 0
+    rho_mlp = metrics.spearman_rho(pred[linked], truth[linked])
+    rho_base = metrics.spearman_rho(trunc[linked], truth[linked])
+    say("surrogate rho", rho_mlp, "truncated", rho_base)
+    assert rho_mlp > rho_base
+    s.close()
+
+
+def test_results_written_under_reference_cache_names(hb, oracle, tmp_path):
+    from sparkx_amd import cache
+    g = hb.Graph.erdos_renyi(3000, 9000, seed=5)
+    cache.save_graph_cache(g, str(tmp_path), "g1")
+    g2 = cache.load_cached_graph(str(tmp_path), "g1")  # staged from the u64 memory maps
+    ref = _oracle_run(oracle, g, 8)
+    with hb.HyperBall(g2, 8) as h:
+        c = h.compute_closeness_centrality(True)
+        hh = h.compute_harmonic_centrality(False)
+        paths = cache.write_results(h, str(tmp_path), "g1")
+    assert len(paths) == 4
+    assert np.array_equal(cache.read_result(str(tmp_path), "distances", "g1"), ref.dist)
+    assert np.array_equal(cache.read_result(str(tmp_path), "inv_distances", "g1"), ref.inv)
+    assert np.array_equal(cache.read_result(str(tmp_path), "closeness", "g1", True), c)
+    assert np.array_equal(cache.read_result(str(tmp_path), "harmonic", "g1", False), hh)
