@@ -221,6 +221,31 @@ int hb_mlp_predict(hb_mlp *m, const float *x, uint64_t count, float *y);
  * Replaces this context's previous run. */
 int hb_surrogate(hb_ctx *ctx, hb_mlp *m, int p_feat, float *pred, float *features, double *ms);
 
+/* ---- exact BFS centralities ---------------------------------------------------
+ * One breadth-first search per source over the staged graph, 64 sources per pass
+ * (one bit lane each), following arcs forwards as the reference's BFSDists does
+ * (trails/bfs.rs:199-234), including its quirk: the source is not marked visited, so
+ * it is rediscovered and counted.  Sources are [src_begin, src_end) (a sample, or
+ * 0..n for every node).
+ *
+ * hb_exact_bfs writes, per source (each pointer may be NULL): the reachable count
+ * (BFSDists::recheable), the distance total (total_distances, as an integer; the
+ * reference's f64 sum of integers equals it below 2^53) and the node-order f64 sum
+ * of 1/d over v = 0..n-1 (exact.rs:386-396, bit-identical to the reference's
+ * sequential sum; needs eccentricity <= 65535, else HB_ERANGE).
+ * Replaces: BFSDists::new + recheable()/total_distances()/get_distances(). */
+int hb_exact_bfs(hb_ctx *ctx, uint64_t src_begin, uint64_t src_end, uint64_t *reach,
+                 uint64_t *total, double *inv_sum);
+
+/* Exact centrality of sources [src_begin, src_end) into dst (src_end - src_begin
+ * doubles); kind HB_CLOSENESS / HB_HARMONIC / HB_LIN; normalization -1 = None,
+ * 0 = Some(false), 1 = Some(true) (ignored for Lin), as the reference's Option<bool>.
+ * Replaces: ExactClosenessCentrality::new (exact.rs:21-30,138-226),
+ * ExactHarmonicCentrality::new (:238-249,357-466), ExactLinCentrality::new
+ * (:476-486,560-644), restricted to a source range. */
+int hb_exact_centrality(hb_ctx *ctx, int kind, int normalization, uint64_t src_begin,
+                        uint64_t src_end, double *dst);
+
 /* Number of nodes / arcs staged (0 if none). */
 uint64_t hb_num_nodes(const hb_ctx *ctx);
 uint64_t hb_num_arcs(const hb_ctx *ctx);

@@ -115,7 +115,11 @@ void hbo_estimates(const hbo_variant *v, int p, uint64_t n, const uint8_t *regs,
 void hbo_centrality(int kind, int normalize, uint64_t n, const float *est,
                     const double *dist, const double *inv, double *out);
 
-/* ---- exact BFS closeness (reference quirk: source not pre-marked visited) ---- */
+/* ---- exact BFS (reference quirk: source not pre-marked visited) ---- */
+/* per source in [src_begin, src_end): reachable count, the f64 distance sum, and
+ * (inv_sum may be NULL) the node-order sum of 1/d */
+void hbo_exact_bfs(const uint64_t *offsets, const uint32_t *nbrs, uint64_t n, uint64_t src_begin,
+                   uint64_t src_end, uint64_t *reach, double *total, double *inv_sum);
 void hbo_exact_closeness(const uint64_t *offsets, const uint32_t *nbrs, uint64_t n,
                          uint64_t src_begin, uint64_t src_end, double *out);
 

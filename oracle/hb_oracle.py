@@ -8,6 +8,8 @@ repo root):
   hyperball()          src/centralities/hyper_ball.rs:702-736 and :754-874
   centrality()         src/centralities/hyper_ball.rs:434-555
   exact_closeness()    src/trails/bfs.rs:199-234, src/test_common.rs:217-227
+  exact_bfs()          src/trails/bfs.rs:199-234, src/centralities/exact.rs:386-396
+  exact_centrality()   src/centralities/exact.rs:138-201,357-441,560-619
   mae/mape/spearman    src/utils.rs:57-100
   dbg_mtx_csr()        gen_dbg_mtx.py:4-22 + parser_mtx/producers.rs:228-234
                        + impl_sort.rs:10-51
@@ -99,6 +101,7 @@ def lib():
         L.hbo_estimates.argtypes = [vp, C.c_int, C.c_uint64, u8p, f32p]
         L.hbo_centrality.argtypes = [C.c_int, C.c_int, C.c_uint64, f32p, f64p, f64p, f64p]
         L.hbo_exact_closeness.argtypes = [u64p, u32p, C.c_uint64, C.c_uint64, C.c_uint64, f64p]
+        L.hbo_exact_bfs.argtypes = [u64p, u32p, C.c_uint64, C.c_uint64, C.c_uint64, u64p, f64p, f64p]
         for f in ("hbo_mae", "hbo_mape", "hbo_spearman"):
             getattr(L, f).restype = C.c_double
             getattr(L, f).argtypes = [f64p, f64p, C.c_uint64]
@@ -237,6 +240,48 @@ def exact_closeness(offsets, nbrs, src_begin=0, src_end=None) -> np.ndarray:
     lib().hbo_exact_closeness(_p(off, C.c_uint64), _p(nb, C.c_uint32), n, src_begin, src_end,
                               _p(out, C.c_double))
     return out
+
+
+def exact_bfs(offsets, nbrs, src_begin=0, src_end=None, inv=True):
+    """(reachable u64, total f64, inv_sum f64 or None) per source, bfs.rs:199-234 + exact.rs:386-396."""
+    off = np.ascontiguousarray(offsets, dtype=np.uint64)
+    nb = np.ascontiguousarray(nbrs, dtype=np.uint32)
+    n = off.size - 1
+    src_end = n if src_end is None else min(src_end, n)
+    cnt = max(src_end - src_begin, 0)
+    reach = np.zeros(cnt, np.uint64)
+    total = np.zeros(cnt, np.float64)
+    invs = np.zeros(cnt, np.float64) if inv else None
+    lib().hbo_exact_bfs(_p(off, C.c_uint64), _p(nb, C.c_uint32), n, src_begin, src_end,
+                        _p(reach, C.c_uint64), _p(total, C.c_double),
+                        _p(invs, C.c_double) if inv else None)
+    return reach, total, invs
+
+
+def exact_centrality(kind, normalization, reach, total, inv_sum, n):
+    """exact.rs formulas from the BFS sums; kind 0 closeness (:160-201), 1 harmonic
+    (:379-441), 2 Lin (:610-619); normalization None / False / True as the reference's
+    Option<bool> (:130-136); n = node count."""
+    reach = np.asarray(reach, np.uint64)
+    r = reach.astype(np.float64)
+    total = np.asarray(total, np.float64)
+    norm = float(max(n - 1, 0))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        if kind == 0:
+            zero = (reach <= 1) | (total == 0.0)
+            val = total if normalization is None else (np.float64(norm) / total if normalization is False
+                                                       else r / total)
+        elif kind == 1:
+            inv_sum = np.asarray(inv_sum, np.float64)
+            zero = reach <= 1
+            val = inv_sum if normalization is None else (np.float64(norm) / inv_sum if normalization is False
+                                                         else r / inv_sum)
+        elif kind == 2:
+            zero = total == 0.0
+            val = (r * r) / total
+        else:
+            raise ValueError(kind)
+    return np.where(zero, 0.0, val)
 
 
 def _metric(name, a, b):

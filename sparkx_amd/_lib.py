@@ -7,7 +7,8 @@ import ctypes as C
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libsparkx_hb.so")
+# SPARKX_HB_LIB selects another build of the same ABI (e.g. an experimental one)
+LIB_PATH = os.environ.get("SPARKX_HB_LIB") or os.path.join(HERE, "libsparkx_hb.so")
 
 HB_OK, HB_EINVAL, HB_ENOMEM, HB_EHIP, HB_ESTATE, HB_ERANGE = 0, -1, -2, -3, -4, -5
 HB_FLAG_DENSE, HB_FLAG_GRAPH, HB_FLAG_NF, HB_FLAG_TIMING, HB_FLAG_KTIMING = 0x1, 0x2, 0x4, 0x8, 0x10
@@ -29,6 +30,7 @@ EXPORTS = (
     "hb_batch_read_accumulators", "hb_autotune", "hb_batch_autotune",
     "hb_set_snapshots", "hb_read_snapshots", "hb_mlp_create", "hb_mlp_destroy", "hb_mlp_set_params",
     "hb_mlp_get_params", "hb_mlp_gradient", "hb_mlp_train", "hb_mlp_predict", "hb_surrogate",
+    "hb_exact_bfs", "hb_exact_centrality",
 )
 
 
@@ -146,6 +148,9 @@ def load() -> C.CDLL:
         "hb_mlp_train": (C.c_int, [vp, f32p, f32p, C.c_uint64, C.c_int, C.c_int, C.c_float, C.c_uint64, f32p]),
         "hb_mlp_predict": (C.c_int, [vp, f32p, C.c_uint64, f32p]),
         "hb_surrogate": (C.c_int, [vp, vp, C.c_int, f32p, f32p, C.POINTER(C.c_double)]),
+        "hb_exact_bfs": (C.c_int, [vp, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                   f64p]),
+        "hb_exact_centrality": (C.c_int, [vp, C.c_int, C.c_int, C.c_uint64, C.c_uint64, f64p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -694,6 +694,56 @@ extern "C" int hb_centrality(hb_ctx *c, int kind, int normalize, double *dst)
     return HB_OK;
 }
 
+extern "C" int hb_exact_bfs(hb_ctx *c, uint64_t src_begin, uint64_t src_end, uint64_t *reach, uint64_t *total,
+                            double *inv_sum)
+{
+    if (!c) return fail(HB_EINVAL, "hb_exact_bfs: ctx is NULL");
+    if (src_end > src_begin && !reach && !total && !inv_sum) return fail(HB_EINVAL, "hb_exact_bfs: no output");
+    if (!c->off.ptr) return fail(HB_ESTATE, "hb_exact_bfs: no graph staged");
+    if (src_begin > src_end || src_end > c->n)
+        return fail(HB_EINVAL, "hb_exact_bfs: sources [%llu, %llu) outside [0, %llu)", (unsigned long long)src_begin,
+                    (unsigned long long)src_end, (unsigned long long)c->n);
+    if (!set_device(c)) return fail(HB_EHIP, "hipSetDevice failed");
+    const int rc = hb::exact_bfs(c->off.ptr, c->nbr.ptr, (uint32_t)c->n, src_begin, src_end, reach, total, inv_sum,
+                                 c->stream);
+    if (rc == HB_OK) g_err.clear();
+    return rc;
+}
+
+extern "C" int hb_exact_centrality(hb_ctx *c, int kind, int normalization, uint64_t src_begin, uint64_t src_end,
+                                   double *dst)
+{
+    if (!c || (!dst && src_end > src_begin)) return fail(HB_EINVAL, "hb_exact_centrality: NULL argument");
+    if (kind < HB_CLOSENESS || kind > HB_LIN) return fail(HB_EINVAL, "hb_exact_centrality: bad kind %d", kind);
+    if (normalization < -1 || normalization > 1)
+        return fail(HB_EINVAL, "hb_exact_centrality: normalization %d not in -1..1", normalization);
+    if (src_end <= src_begin) return src_end < src_begin ? fail(HB_EINVAL, "hb_exact_centrality: empty range") : HB_OK;
+    const uint64_t cnt = src_end - src_begin;
+    std::vector<uint64_t> reach(cnt), total(cnt);
+    std::vector<double> inv(kind == HB_HARMONIC ? cnt : 0);
+    const int rc = hb_exact_bfs(c, src_begin, src_end, reach.data(), total.data(),
+                                kind == HB_HARMONIC ? inv.data() : nullptr);
+    if (rc != HB_OK) return rc;
+    // exact.rs:160-201 (closeness), :379-441 (harmonic), :610-619 (Lin); the
+    // reference's f64 distance sum equals the integer total below 2^53
+    const double norm = c->n ? (double)(c->n - 1) : 0.0;
+    for (uint64_t i = 0; i < cnt; ++i) {
+        const double r = (double)reach[i], t = (double)total[i];
+        double v;
+        if (kind == HB_CLOSENESS) {
+            const double num = normalization < 0 ? 0.0 : (normalization == 0 ? norm : r);
+            v = (reach[i] <= 1 || t == 0.0) ? 0.0 : (normalization < 0 ? t : num / t);
+        } else if (kind == HB_HARMONIC) {
+            const double num = normalization < 0 ? 0.0 : (normalization == 0 ? norm : r);
+            v = reach[i] <= 1 ? 0.0 : (normalization < 0 ? inv[i] : num / inv[i]);
+        } else {
+            v = t == 0.0 ? 0.0 : (r * r) / t;
+        }
+        dst[i] = v;
+    }
+    return HB_OK;
+}
+
 extern "C" int hb_neighbourhood_function(hb_ctx *c, double *dst, int cap)
 {
     if (!c || (!dst && cap > 0)) return fail(HB_EINVAL, "hb_neighbourhood_function: NULL argument");

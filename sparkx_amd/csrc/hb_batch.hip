@@ -479,9 +479,19 @@ int hb_batch_set_schedule(hb_batch *b, uint32_t small_max_deg, uint32_t mid_max_
     return HB_OK;
 }
 
+// The batched engine is switched off: its first GPU run (64 graphs of 4K-262K nodes,
+// p = 4..8) stalled without finishing and the cause has not been found by reading the
+// code, so it stays off until one is shown.  hb_batch_create and the readers still
+// validate their arguments; hb_batch_run (and the autotuner that calls it) return
+// HB_ESTATE.
+static constexpr bool kBatchEngineEnabled = false;
+
 int hb_batch_run(hb_batch *b, uint32_t flags, hb_batch_stats *stats)
 {
     if (!b) return bfail(HB_EINVAL, "hb_batch_run: NULL batch");
+    if (!kBatchEngineEnabled)
+        return bfail(HB_ESTATE, "hb_batch_run: the batched engine is switched off in this build "
+                                "(an unexplained stall of its first GPU run is under investigation)");
     HB_BTRY(hipSetDevice(b->device));
     {
         const int rc = build_tiles(b);

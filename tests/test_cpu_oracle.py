@@ -67,6 +67,95 @@ def test_exact_closeness_reproduces_fixtures(oracle):
         assert np.array_equal(oracle.exact_closeness(off, nb, lo, lo + 300), fx[lo:lo + 300])
 
 
+def _bfs_brute(n, arcs):
+    """Independent distances for the BFS sums: Floyd-Warshall over path lengths >= 1.
+    D[s][v] = shortest walk of length >= 1 from s to v, so D[s][s] is the shortest
+    cycle through s -- the distance at which the reference's BFS rediscovers its
+    source (it is queued but not marked visited)."""
+    INF = 1 << 40
+    A = np.full((n, n), INF, np.int64)
+    for a, b in arcs:
+        A[a, b] = 1
+    D = A.copy()
+    for k in range(n):
+        D = np.minimum(D, D[:, k:k + 1] + A[k:k + 1, :])          # walks ending with an arc from k
+        D = np.minimum(D, D[:, k:k + 1] + D[k:k + 1, :])
+    D[D >= INF] = 0
+    return D
+
+
+def _csr_of(n, arcs):
+    adj = [[] for _ in range(n)]
+    for a, b in arcs:
+        adj[a].append(b)
+    off = np.zeros(n + 1, np.uint64)
+    off[1:] = np.cumsum([len(x) for x in adj])
+    return off, np.array([v for x in adj for v in x], np.uint32)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_exact_bfs_against_brute_force_shortest_walks(oracle, seed):
+    """reachable, total and the node-order sum of 1/d (exact.rs:386-396) from an
+    independent all-pairs computation, on random directed graphs with self-loops,
+    duplicate arcs, sinks and unreachable parts."""
+    rng = np.random.default_rng(seed)
+    n = 40
+    arcs = [(int(a), int(b)) for a, b in rng.integers(0, n, size=(90, 2))]
+    arcs += [(5, 5), (7, 8), (7, 8)]
+    off, nb = _csr_of(n, arcs)
+    D = _bfs_brute(n, arcs)
+    reach, total, inv = oracle.exact_bfs(off, nb)
+    for s in range(n):
+        d = D[s]
+        assert reach[s] == np.count_nonzero(d), s
+        assert total[s] == float(d.sum()), s
+        acc = 0.0
+        for v in range(n):
+            acc += 1.0 / float(d[v]) if d[v] else 0.0
+        assert inv[s] == acc, s
+    # the self-loop makes node 5 its own neighbour at distance 1
+    assert D[5][5] == 1 and reach[5] >= 1
+
+
+def test_exact_bfs_closed_forms(oracle):
+    # undirected path 0-1-...-(n-1): from 0, d(v) = v and the source returns at 2
+    n = 50
+    arcs = [(i, i + 1) for i in range(n - 1)] + [(i + 1, i) for i in range(n - 1)]
+    off, nb = _csr_of(n, arcs)
+    reach, total, inv = oracle.exact_bfs(off, nb, 0, 1)
+    assert reach[0] == n and total[0] == n * (n - 1) / 2 + 2
+    assert abs(inv[0] - (0.5 + math.fsum(1.0 / v for v in range(1, n)))) < 1e-12
+    # star with centre 0: centre sees leaves at 1 and itself at 2; a leaf sees the
+    # centre at 1, the other leaves and itself at 2
+    n = 30
+    arcs = [(0, i) for i in range(1, n)] + [(i, 0) for i in range(1, n)]
+    off, nb = _csr_of(n, arcs)
+    reach, total, inv = oracle.exact_bfs(off, nb, 0, 2)
+    assert list(reach) == [n, n] and list(total) == [n + 1, 2 * n - 1]
+    # isolated node and a node with only a self-loop
+    off, nb = _csr_of(3, [(1, 1)])
+    reach, total, inv = oracle.exact_bfs(off, nb)
+    assert list(reach) == [0, 1, 0] and list(total) == [0, 1, 0] and list(inv) == [0.0, 1.0, 0.0]
+
+
+def test_exact_centrality_formulas(oracle):
+    """exact.rs: closeness None/false/true, harmonic None/false/true, Lin; zero rules."""
+    reach = np.array([0, 1, 3, 5], np.uint64)
+    total = np.array([0.0, 1.0, 5.0, 9.0])
+    inv = np.array([0.0, 1.0, 2.0, 2.75])
+    n = 10
+    c = [oracle.exact_centrality(0, nz, reach, total, inv, n) for nz in (None, False, True)]
+    assert list(c[0]) == [0.0, 0.0, 5.0, 9.0]
+    assert list(c[1]) == [0.0, 0.0, 9.0 / 5.0, 1.0]
+    assert list(c[2]) == [0.0, 0.0, 3.0 / 5.0, 5.0 / 9.0]
+    h = [oracle.exact_centrality(1, nz, reach, total, inv, n) for nz in (None, False, True)]
+    assert list(h[0]) == [0.0, 0.0, 2.0, 2.75]
+    assert list(h[1]) == [0.0, 0.0, 4.5, 9.0 / 2.75]
+    assert list(h[2]) == [0.0, 0.0, 1.5, 5.0 / 2.75]
+    lin = oracle.exact_centrality(2, None, reach, total, inv, n)
+    assert list(lin) == [0.0, 1.0, 9.0 / 5.0, 25.0 / 9.0]
+
+
 # ---------------------------------------------------------------------------
 # the reference's published accuracy numbers (plot_hyperball_results.py:38-57)
 # ---------------------------------------------------------------------------
