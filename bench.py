@@ -33,6 +33,7 @@ CONFIGS = {
     "kmer": (dict(kind="kmer", genome_len=4_000_000, copies=8, sub_rate=0.005, k=21, seed=1), 6, 64),
     "batch64": (dict(kind="batch", count=64, seed=1), 0, 0),
     "surrogate": (dict(kind="surrogate", train_graphs=32, train_scale=20, seed=1), 7, 0),
+    "exact": (dict(kind="exact", scale=20, edge_factor=16, seed=1, sources=2048), 0, 0),
 }
 
 
@@ -110,9 +111,10 @@ def cpu_baseline(g, p, threads, iters):
 
 def bench_surrogate(args, spec, eval_graphs=None):
     """Learned surrogate: MLP trained on GPU (Adam, lr 1e-3, MSE) on seeded R-MAT
-    scale-20 graphs labelled by full p=7 HyperBall; evaluated on R-MAT-24 and the
-    k-mer graph against full HyperBall: Spearman rho (reference tie semantics),
-    top-1% overlap, GPU time of 4 iterations (p=4) + features + MLP vs full run."""
+    scale-20 graphs labelled by full p=7 HyperBall (target: log(1+H) - log(1+H_4));
+    evaluated on R-MAT-24 and the k-mer graph against full HyperBall: Spearman rho
+    (reference tie semantics), top-1% overlap, GPU time of register init + 4
+    iterations (p=4) + features + MLP against register init + the full run."""
     import numpy as np
 
     from sparkx_amd import Graph, HyperBall, metrics
@@ -121,7 +123,7 @@ def bench_surrogate(args, spec, eval_graphs=None):
     xs, ys = [], []
     for i in range(spec["train_graphs"]):
         g = Graph.rmat(spec["train_scale"], 16, seed=spec["seed"] * 1000 + i)
-        x, y, _, _ = labelled_graph(g)
+        x, y, _, _, _ = labelled_graph(g)
         xs.append(x)
         ys.append(y)
     x = np.concatenate(xs)
@@ -224,6 +226,47 @@ def bench_batch(args, spec):
     print(json.dumps(out), flush=True)
 
 
+def bench_exact(args, spec):
+    """Exact BFS closeness (exact.rs semantics, bfs.rs source quirk) of the first
+    `sources` nodes of R-MAT scale 20: 64 bit-lane searches per pass on the GPU.
+    value = sources per second; TEPS counts every arc once per source."""
+    import numpy as np
+
+    from sparkx_amd.exact import exact_bfs
+    g = make_graph(dict(kind="rmat", scale=spec["scale"], edge_factor=spec["edge_factor"], seed=spec["seed"]),
+                   args.graph_cache)
+    k = spec["sources"]
+    log(f"exact: R-MAT {spec['scale']} n={g.size()} arcs={g.width()}, {k} sources")
+    for _ in range(args.warmup):
+        exact_bfs(g, (0, 64), inv_sum=True)
+    walls = []
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        reach, total, inv = exact_bfs(g, (0, k), inv_sum=True)
+        walls.append(time.perf_counter() - t0)
+        log(f"step {i}: {k} sources in {walls[-1]:.2f}s (host-timed, graph staging included)")
+    tot = sum(walls)
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import hb_oracle as O
+        ns = 4
+        t0 = time.perf_counter()
+        r, t, v = O.exact_bfs(g.offsets.astype(np.uint64), g.neighbours.astype(np.uint32), 0, ns)
+        dt = time.perf_counter() - t0
+        assert np.array_equal(r, reach[:ns]) and np.array_equal(v, inv[:ns])
+        cpu = {"value": round(ns / dt, 3), "unit": "sources/s", "cores": 1, "kind": "port",
+               "sample": f"first {ns} sources, one queue BFS each (bfs.rs:199-234) plus the node-order "
+                         f"harmonic sum, {dt:.1f}s"}
+    out = {"metric": f"Exact BFS centrality sources/s (R-MAT-{spec['scale']})", "value": round(args.steps * k / tot, 2),
+           "unit": "sources/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(tot / args.steps * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+           "config": {"workload": "exact", **spec, "nodes": g.size(), "arcs": g.width()},
+           "gteps": round(args.steps * k * g.width() / tot / 1e9, 3),
+           "roofline": None, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -244,6 +287,8 @@ def main():
     spec, p, max_iters = CONFIGS[args.config]
     if spec["kind"] == "batch":
         return bench_batch(args, spec)
+    if spec["kind"] == "exact":
+        return bench_exact(args, spec)
     if spec["kind"] == "surrogate":
         print(json.dumps(bench_surrogate(args, spec)), flush=True)
         return None

@@ -36,8 +36,8 @@ hipError_t launch_centrality(const HbDev &d, const float *est, int kind, int nor
                              hipStream_t s);
 
 // surrogate tail on a staged graph: features from snapshots, MLP inference
-hipError_t launch_surrogate_tail(struct ::hb_mlp *n, const uint32_t *off, const float *snaps, uint32_t nodes,
-                                 float *x_dev, float *out_dev, hipStream_t s);
+hipError_t launch_surrogate_tail(struct ::hb_mlp *n, const uint32_t *off, const float *snaps, const double *h4,
+                                 uint32_t nodes, float *x_dev, float *out_dev, hipStream_t s);
 int mlp_device(const struct ::hb_mlp *n);
 // exact BFS sums of sources [s_begin, s_end) over a staged CSR (hb_exact.hip)
 int exact_bfs(const uint32_t *off, const uint32_t *nbr, uint32_t n, uint64_t s_begin, uint64_t s_end,

@@ -305,16 +305,32 @@ __global__ void k_zero(float *p, int n)
     if (i < n) p[i] = 0.0f;
 }
 
-// surrogate features of every node: degree, log(1 + degree), log N_1..N_4
-__global__ void k_features(const uint32_t *off, const float *snaps, uint32_t n, float *x)
+// surrogate features of every node, all free of the graph's scale:
+//   log(1 + degree), log N_2/N_1, log N_3/N_2, log N_4/N_3, log n/N_4, log (1 + H_4)/n
+// with N_t the estimate after iteration t and H_4 the harmonic accumulator after 4
+// iterations; the network predicts log(1 + H) - log(1 + H_4)
+__global__ void k_features(const uint32_t *off, const float *snaps, const double *h4, uint32_t n, float *x)
 {
     const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= n) return;
     const float deg = (float)(off[u + 1] - off[u]);
+    const float ln_n = logf((float)n);
+    float l[4];
+    for (int t = 0; t < 4; ++t) l[t] = logf(snaps[(uint64_t)t * n + u]);
     float *r = x + (uint64_t)u * kDIn;
-    r[0] = deg;
-    r[1] = log1pf(deg);
-    for (int t = 0; t < 4; ++t) r[2 + t] = logf(snaps[(uint64_t)t * n + u]);
+    r[0] = log1pf(deg);
+    r[1] = l[1] - l[0];
+    r[2] = l[2] - l[1];
+    r[3] = l[3] - l[2];
+    r[4] = ln_n - l[3];
+    r[5] = (float)log1p(h4[u]) - ln_n;
+}
+
+// score = log(1 + H_4) + predicted residual
+__global__ void k_add_base(const double *h4, uint32_t n, float *out)
+{
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u < n) out[u] = (float)log1p(h4[u]) + out[u];
 }
 
 }  // namespace mlp
@@ -541,12 +557,13 @@ int hb_mlp_predict(hb_mlp *n, const float *x, uint64_t count, float *y)
 
 // device-resident surrogate on a staged graph (declared in sparkx_hb.h)
 namespace hb {
-hipError_t launch_surrogate_tail(hb_mlp *n, const uint32_t *off, const float *snaps, uint32_t nodes,
-                                 float *x_dev, float *out_dev, hipStream_t s)
+hipError_t launch_surrogate_tail(hb_mlp *n, const uint32_t *off, const float *snaps, const double *h4,
+                                 uint32_t nodes, float *x_dev, float *out_dev, hipStream_t s)
 {
-    mlp::k_features<<<(nodes + 255) / 256, 256, 0, s>>>(off, snaps, nodes, x_dev);
+    mlp::k_features<<<(nodes + 255) / 256, 256, 0, s>>>(off, snaps, h4, nodes, x_dev);
     mlp::Dev d{n->w, n->norm, x_dev, nullptr, nullptr, 0, nodes, 0.0f, nullptr, nullptr, out_dev};
     mlp::k_predict<<<mlp_grid(nodes, 8192), mlp::kBlock, 0, s>>>(d);
+    mlp::k_add_base<<<(nodes + 255) / 256, 256, 0, s>>>(h4, nodes, out_dev);
     return hipGetLastError();
 }
 int mlp_device(const hb_mlp *n) { return n->device; }

@@ -1,14 +1,61 @@
-// hbm_copy_probe.hip -- plain 1 GiB device-to-device copy kernel of known size.
-// Used to calibrate rocprofv3's FETCH_SIZE / WRITE_SIZE counters (KiB per
-// dispatch) against true bytes moved, and to measure streaming HBM bandwidth.
+// hbm_copy_probe.hip -- HBM reference kernels of known traffic:
+//   copy   : 1 GiB read + 1 GiB written, 16 B per lane (calibrates rocprofv3's
+//            FETCH_SIZE / WRITE_SIZE, KiB per dispatch, against true bytes)
+//   read   : 4 GiB streamed, 16 B per lane, nothing written but one word per wave
+//   gather : the access pattern of the HyperBall union at p = 7: a 4 B index read,
+//            then one random 128 B record of a 4 GiB table (8 lanes x 16 B), for
+//            2^27 indices -- the practical ceiling for the union kernels; also with
+//            64 B and 96 B records packed back to back (what p = 6 needs, and what
+//            p = 7 would need with 6-bit registers)
+// Each is timed with HIP events, best of 5, over a few grid sizes.
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
 #include <cstdio>
 
 __global__ void hbm_copy(const uint4 *__restrict__ a, uint4 *__restrict__ b, size_t n)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         b[i] = a[i];
+}
+
+__global__ void hbm_read(const uint4 *__restrict__ a, size_t n, uint32_t *out)
+{
+    uint32_t acc = 0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const uint4 v = a[i];
+        acc ^= v.x ^ v.y ^ v.z ^ v.w;
+    }
+    if (acc == 0x9e3779b9u) out[blockIdx.x] = acc;  // keeps the loads; never true for the fill pattern
+}
+
+__global__ void make_index(uint32_t *idx, size_t n, uint32_t records)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        uint64_t z = (uint64_t)i + 0x9e3779b97f4a7c15ull;
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        z ^= z >> 31;
+        idx[i] = (uint32_t)(z % records);
+    }
+}
+
+// records of `units` x 16 B (units <= 8) at a stride of `units` x 16 B; a group of 8
+// lanes serves one index, lanes >= units idle
+__global__ void hbm_gather(const uint4 *__restrict__ table, const uint32_t *__restrict__ idx, size_t n,
+                           uint32_t units, uint32_t *out)
+{
+    const uint32_t sub = threadIdx.x & 7u;
+    uint32_t acc = 0;
+    const size_t groups = (size_t)gridDim.x * blockDim.x / 8;
+    for (size_t g = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) / 8; g < n; g += groups) {
+        const uint32_t r = idx[g];
+        if (sub < units) {
+            const uint4 v = table[(size_t)r * units + sub];
+            acc ^= v.x ^ v.y ^ v.z ^ v.w;
+        }
+    }
+    if (acc == 0x9e3779b9u) out[blockIdx.x] = acc;
 }
 
 #define CHECK(x)                                                                  \
@@ -20,26 +67,71 @@ __global__ void hbm_copy(const uint4 *__restrict__ a, uint4 *__restrict__ b, siz
         }                                                                         \
     } while (0)
 
-int main()
+template <class F>
+static float best_ms(hipEvent_t e0, hipEvent_t e1, F launch)
 {
-    const size_t bytes = size_t(1) << 30, n = bytes / 16;
-    uint4 *a = nullptr, *b = nullptr;
-    CHECK(hipMalloc(&a, bytes));
-    CHECK(hipMalloc(&b, bytes));
-    CHECK(hipMemset(a, 1, bytes));
+    float best = 1e30f;
+    for (int r = 0; r < 5; ++r) {
+        (void)hipEventRecord(e0);
+        launch();
+        (void)hipEventRecord(e1);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best) best = ms;
+    }
+    return best;
+}
+
+int main(int argc, char **argv)
+{
+    const bool copy_only = argc > 1;  // the counter-calibration runs time the copy kernel alone
+    const size_t copy_bytes = size_t(1) << 30, copy_n = copy_bytes / 16;
+    const size_t table_bytes = size_t(4) << 30, table_n = table_bytes / 16;
+    const uint32_t records = (uint32_t)(table_bytes / 128);
+    const size_t n_idx = size_t(1) << 27;
+    uint4 *a = nullptr, *b = nullptr, *table = nullptr;
+    uint32_t *idx = nullptr, *sink = nullptr;
+    CHECK(hipMalloc(&a, copy_bytes));
+    CHECK(hipMalloc(&b, copy_bytes));
+    CHECK(hipMalloc(&sink, 131072 * 4));
+    CHECK(hipMemset(a, 1, copy_bytes));
     hipEvent_t e0, e1;
     CHECK(hipEventCreate(&e0));
     CHECK(hipEventCreate(&e1));
     for (int grid : {8192, 32768, 131072}) {
-        CHECK(hipEventRecord(e0));
-        hbm_copy<<<grid, 256>>>(a, b, n);
-        CHECK(hipEventRecord(e1));
-        CHECK(hipEventSynchronize(e1));
-        float ms = 0.0f;
-        CHECK(hipEventElapsedTime(&ms, e0, e1));
-        printf("copy 1 GiB grid %d: %.3f ms, %.1f GB/s read+write\n", grid, ms, 2.0 * bytes / (ms * 1e-3) / 1e9);
+        const float ms = best_ms(e0, e1, [&] { hbm_copy<<<grid, 256>>>(a, b, copy_n); });
+        CHECK(hipGetLastError());
+        printf("copy 1 GiB grid %d: %.3f ms, %.1f GB/s read+write\n", grid, ms, 2.0 * copy_bytes / (ms * 1e-3) / 1e9);
+        if (copy_only) break;
+    }
+    if (!copy_only) {
+        CHECK(hipMalloc(&table, table_bytes));
+        CHECK(hipMalloc(&idx, n_idx * 4));
+        CHECK(hipMemset(table, 3, table_bytes));
+        make_index<<<8192, 256>>>(idx, n_idx, records);
+        CHECK(hipGetLastError());
+        CHECK(hipDeviceSynchronize());
+        for (int grid : {4096, 16384, 65536}) {
+            const float ms = best_ms(e0, e1, [&] { hbm_read<<<grid, 256>>>(table, table_n, sink); });
+            CHECK(hipGetLastError());
+            printf("read 4 GiB grid %d: %.3f ms, %.1f GB/s\n", grid, ms, table_bytes / (ms * 1e-3) / 1e9);
+        }
+        for (uint32_t units : {8u, 6u, 4u}) {
+            // every record index < records and units <= 8 keeps r * units + sub < table_n
+            for (int grid : {4096, 16384, 65536}) {
+                const float ms = best_ms(e0, e1, [&] { hbm_gather<<<grid, 256>>>(table, idx, n_idx, units, sink); });
+                CHECK(hipGetLastError());
+                const double bytes = (double)n_idx * (4 + 16 * units);
+                printf("gather 2^27 x %u B (+4 B index) grid %d: %.3f ms, %.1f GB/s\n", 16 * units, grid, ms,
+                       bytes / (ms * 1e-3) / 1e9);
+            }
+        }
+        CHECK(hipFree(table));
+        CHECK(hipFree(idx));
     }
     CHECK(hipFree(a));
     CHECK(hipFree(b));
+    CHECK(hipFree(sink));
     return 0;
 }

@@ -1,10 +1,13 @@
 """Learned surrogate for anytime centrality rankings.
 
+After 4 HyperBall iterations each node has its estimates N_1..N_4 and its
+truncated harmonic accumulator H_4 = sum_{t<=4} (1/t)(N_t - N_{t-1}) (the
+reference's inverse-distance accumulator, hyper_ball.rs:824-825, stopped early).
 An MLP 6 -> 64 -> 64 -> 1 (GELU; BF16 MFMA matmuls with f32 accumulation; f32
-master weights and Adam) maps each node's (degree, log(1 + degree), log N_1..N_4)
--- N_t the HyperBall estimate after t iterations -- to its log harmonic
-centrality log(1 + H(u)), H(u) = sum_t (1/t) (N_t(u) - N_{t-1}(u)) from the full
-run (the reference's inverse-distance accumulator, hyper_ball.rs:824-825).
+master weights and Adam) predicts the remaining log-gap to the full run,
+log(1 + H) - log(1 + H_4), from features that do not grow with the graph:
+log(1 + degree), log N_2/N_1, log N_3/N_2, log N_4/N_3, log n/N_4 and
+log((1 + H_4)/n).  The ranking score is log(1 + H_4) plus that prediction.
 Training, inference and features run on the GPU (include/sparkx_hb.h, hb_mlp_*,
 hb_surrogate)."""
 from __future__ import annotations
@@ -20,14 +23,24 @@ from .hyper_ball import HyperBall
 N_PARAMS = 4673
 
 
-def features(g: Graph, snaps: np.ndarray) -> np.ndarray:
+def features(g: Graph, snaps: np.ndarray, h4: np.ndarray) -> np.ndarray:
     """Host form of the surrogate inputs (the device computes the same in f32)."""
     deg = g.degrees().astype(np.float32)
+    lg = np.log(np.asarray(snaps[:4], np.float32))
+    ln_n = np.log(np.float32(g.size()))
     x = np.empty((g.size(), 6), np.float32)
-    x[:, 0] = deg
-    x[:, 1] = np.log1p(deg)
-    x[:, 2:] = np.log(snaps[:4].T)
+    x[:, 0] = np.log1p(deg)
+    x[:, 1] = lg[1] - lg[0]
+    x[:, 2] = lg[2] - lg[1]
+    x[:, 3] = lg[3] - lg[2]
+    x[:, 4] = ln_n - lg[3]
+    x[:, 5] = np.log1p(np.asarray(h4, np.float64)).astype(np.float32) - ln_n
     return x
+
+
+def base_score(h4: np.ndarray) -> np.ndarray:
+    """log(1 + H_4) in f32, the part of the score the network does not predict."""
+    return np.log1p(np.asarray(h4, np.float64)).astype(np.float32)
 
 
 class Surrogate:
@@ -74,7 +87,7 @@ class Surrogate:
 
     def rank(self, h: HyperBall, p_feat: int = 4, want_features: bool = False):
         """Anytime ranking of the graph staged in `h` (its run is replaced): returns
-        (predicted log harmonic centrality, features or None, device ms)."""
+        (score = log(1 + H_4) + predicted residual, features or None, device ms)."""
         n = h.g.size()
         pred = np.empty(n, np.float32)
         feat = np.empty((n, 6), np.float32) if want_features else None
@@ -96,8 +109,9 @@ class Surrogate:
 
 
 def labelled_graph(g: Graph, p_label: int = 7, p_feat: int = 4, device: int = 0):
-    """(features, label, truncated-harmonic baseline, full-run ms) of one graph:
-    labels from the full HyperBall at p_label, features from 4 iterations at p_feat."""
+    """(features, target residual log(1+H) - log(1+H_4), H_4, full-run ms, H) of one
+    graph: H from the full HyperBall at p_label, features and H_4 from 4
+    iterations at p_feat."""
     with HyperBall(g, p_label, device=device) as full:
         inv = full.harmonic_geodesic_distances()
         ms_full = full.stats.ms_total + full.stats.ms_init
@@ -106,5 +120,6 @@ def labelled_graph(g: Graph, p_label: int = 7, p_feat: int = 4, device: int = 0)
         h.run(max_iters=4)
         snaps = np.empty((4, g.size()), np.float32)
         _lib.check(h._L.hb_read_snapshots(h._ctx, _lib.ptr(snaps, C.c_float)))
-        trunc = h.harmonic_geodesic_distances()
-    return features(g, snaps), np.log1p(inv).astype(np.float32), trunc, ms_full
+        h4 = h.harmonic_geodesic_distances()
+    y = (np.log1p(np.asarray(inv, np.float64)) - np.log1p(np.asarray(h4, np.float64))).astype(np.float32)
+    return features(g, snaps, h4), y, h4, ms_full, inv
