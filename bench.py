@@ -64,14 +64,16 @@ def _generate(spec):
     return Graph.kmer_dbg(spec["genome_len"], spec["copies"], spec["sub_rate"], spec["k"], seed=spec["seed"])
 
 
-def algorithmic_bytes(slot, arcs, nodes, items, p):
-    """Bytes a kernel slot must move (SURVEY-style accounting): per recomputed node
-    its offsets (8), own registers read + new registers written (2R), estimate
-    read + write (8), dist/inv read + write (32); per gathered arc its neighbour
-    id (4) and neighbour registers (R).  Hub chunks add one partial sketch write
-    per task, hub merges one partial read per task."""
+def algorithmic_bytes(slot, arcs, nodes, items, p, first_arcs=0, first_nodes=0):
+    """Bytes a kernel slot must move: per recomputed node its offsets (8), own
+    registers read + new registers written (2R), estimate read + write (8), dist/inv
+    read + write (32); per gathered arc its neighbour id (4) and neighbour registers
+    (R).  Hub chunks add one partial sketch write per task, hub merges one partial
+    read per task.  In iteration 1 (p <= 10) no register block is read: the
+    neighbours' and the node's own ids are hashed (first_arcs / first_nodes)."""
     R = 1 << p
     b = nodes * (8 + 2 * R + 8 + 32) + arcs * (4 + R)
+    b -= (first_arcs + first_nodes) * R
     if slot == 3:
         b += items * R
     return b
@@ -111,7 +113,8 @@ def cpu_baseline(g, p, threads, iters):
 
 def bench_surrogate(args, spec, eval_graphs=None):
     """Learned surrogate: MLP trained on GPU (Adam, lr 1e-3, MSE) on seeded R-MAT
-    scale-20 graphs labelled by full p=7 HyperBall (target: log(1+H) - log(1+H_4));
+    scale-20 graphs labelled by full p=7 HyperBall (inputs and target log(1+H)
+    standardised per graph over linked nodes);
     evaluated on R-MAT-24 and the k-mer graph against full HyperBall: Spearman rho
     (reference tie semantics), top-1% overlap, GPU time of register init + 4
     iterations (p=4) + features + MLP against register init + the full run."""
@@ -124,8 +127,9 @@ def bench_surrogate(args, spec, eval_graphs=None):
     for i in range(spec["train_graphs"]):
         g = Graph.rmat(spec["train_scale"], 16, seed=spec["seed"] * 1000 + i)
         x, y, _, _, _ = labelled_graph(g)
-        xs.append(x)
-        ys.append(y)
+        linked = g.degrees() > 0
+        xs.append(x[linked])
+        ys.append(y[linked])
     x = np.concatenate(xs)
     y = np.concatenate(ys)
     log(f"surrogate: {len(xs)} training graphs, {x.shape[0]} samples, labelled in {time.perf_counter() - t0:.1f}s")
@@ -320,7 +324,8 @@ def main():
             slot_ms[s] += st.kernel_ms[s]
             slot_launch[s] += st.kernel_launches[s]
             slot_bytes[s] += algorithmic_bytes(s, st.kernel_arcs[s], st.kernel_nodes[s],
-                                               st.kernel_items[s] * st.kernel_launches[s], p)
+                                               st.kernel_items[s] * st.kernel_launches[s], p,
+                                               st.kernel_first_arcs[s], st.kernel_first_nodes[s])
         per_iter_ms = [round(float(x), 3) for x in st.iter_ms[:st.iterations]]
         log(f"step {i}: {st.iterations} iterations in {walls[-1] * 1e3:.1f} ms wall "
             f"({st.ms_total:.1f} ms device), per-iteration ms {per_iter_ms}")

@@ -73,6 +73,15 @@ typedef struct {
     uint64_t kernel_nodes[HB_NKERNELS];   /* nodes recomputed                         */
     uint32_t kernel_items[HB_NKERNELS];   /* nodes (bins, merge) or tasks (chunks) scheduled */
     uint32_t kernel_team[HB_NKERNELS];    /* lanes per node / task                    */
+    /* the part of kernel_arcs / kernel_nodes done in iteration 1 with p <= 10, where
+       every sketch is still a single insertion: the neighbour's (and the node's own)
+       2^p-byte register block is not read, its id is hashed instead */
+    uint64_t kernel_first_arcs[HB_NKERNELS];
+    uint64_t kernel_first_nodes[HB_NKERNELS];
+    /* iterations whose recomputed nodes were listed from the changed nodes' neighbour
+       lists (symmetric graphs, few changes) instead of by scanning every node */
+    int32_t push_iterations;
+    int32_t symmetric;           /* the staged CSR is symmetric */
 } hb_stats;
 
 typedef struct hb_ctx hb_ctx;
@@ -202,10 +211,10 @@ int hb_read_snapshots(hb_ctx *ctx, float *dst);
  * An MLP 6 -> 64 -> 64 -> 1 with GELU (BF16 MFMA matmuls, f32 accumulation, f32
  * master weights and Adam).  Parameters are 4673 floats: W1[6][64], b1[64],
  * W2[64][64], b2[64], W3[64], b3; normalisation is mean[6], std[6] of the inputs.
- * In the surrogate the inputs are a node's scale-free features after 4 HyperBall
- * iterations -- log(1+deg), log N_2/N_1, log N_3/N_2, log N_4/N_3, log n/N_4,
- * log((1+H_4)/n) -- and the target is log(1+H) - log(1+H_4), H the harmonic
- * accumulator of the full run and H_4 the one after 4 iterations.
+ * In the surrogate the inputs are a node's degree, log(1+degree) and log N_1..N_4
+ * (N_t its estimate after t HyperBall iterations), each standardised by the graph's
+ * mean and standard deviation over nodes with a neighbour; the target is log(1+H),
+ * H the harmonic accumulator of the full run, standardised the same way.
  * No reference counterpart. */
 typedef struct hb_mlp hb_mlp;
 int hb_mlp_create(hb_mlp **out, int device_ordinal, uint64_t seed);
@@ -219,9 +228,10 @@ int hb_mlp_train(hb_mlp *m, const float *x, const float *y, uint64_t count, int 
                  float lr, uint64_t seed, float *epoch_loss);
 int hb_mlp_predict(hb_mlp *m, const float *x, uint64_t count, float *y);
 /* Anytime ranking on the staged graph: HyperBall at precision p_feat for 4
- * iterations, device-side features, MLP inference; pred (n floats) receives the
- * score log(1+H_4) + predicted residual, features (n*6, optional) the inputs;
- * ms = device time of register init, the 4 iterations, features and inference
+ * iterations, device-side standardised inputs, MLP inference; pred (n floats)
+ * receives the predicted standardised log harmonic centrality (nodes without
+ * neighbours: -3e38, last), features (n*6, optional) the standardised inputs;
+ * ms = device time of register init, the 4 iterations, inputs and inference
  * (host set-up excluded).  Replaces this context's previous run. */
 int hb_surrogate(hb_ctx *ctx, hb_mlp *m, int p_feat, float *pred, float *features, double *ms);
 

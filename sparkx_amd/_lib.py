@@ -82,6 +82,10 @@ class HbStats(C.Structure):
         ("kernel_nodes", C.c_uint64 * HB_NKERNELS),
         ("kernel_items", C.c_uint32 * HB_NKERNELS),
         ("kernel_team", C.c_uint32 * HB_NKERNELS),
+        ("kernel_first_arcs", C.c_uint64 * HB_NKERNELS),
+        ("kernel_first_nodes", C.c_uint64 * HB_NKERNELS),
+        ("push_iterations", C.c_int32),
+        ("symmetric", C.c_int32),
     ]
 
 

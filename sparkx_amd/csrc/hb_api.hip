@@ -93,6 +93,8 @@ struct hb_ctx {
     DevBuf<float> est, est_tmp, lc;
     DevBuf<double> dist, inv, bias, cent;
     DevBuf<uint32_t> bits0, bits1, sched_buf, chunk_flag;
+    DevBuf<uint32_t> active, dyn;  // push iterations: bitmap; per-bin lists + 3 counts
+    bool sym = false;              // staged CSR is symmetric (arc multiset fingerprints)
     DevBuf<HbIterState> st;
     DevBuf<HbIterRec> rec;
     uint32_t rec_cap = 0;
@@ -121,6 +123,8 @@ struct hb_ctx {
         bits1.release();
         sched_buf.release();
         chunk_flag.release();
+        active.release();
+        dyn.release();
         st.release();
         rec.release();
         snaps.release();
@@ -273,6 +277,19 @@ int hb_stage_csr(hb_ctx *c, const void *offsets, int off_bytes, uint64_t n, cons
         if (done[i]) (void)hipEventDestroy(done[i]);
     }
     if (rc != HB_OK) return rc;
+    // symmetric graphs may use push iterations (a changed node marks its neighbours)
+    c->sym = true;
+    if (n) {
+        unsigned long long *acc = nullptr, h_acc[4] = {};
+        HB_TRY(hipMalloc(&acc, 4 * sizeof(unsigned long long)));
+        hipError_t e = hipMemsetAsync(acc, 0, 4 * sizeof(unsigned long long), c->stream);
+        if (e == hipSuccess) e = hb::arc_fingerprints(c->off.ptr, c->nbr.ptr, (uint32_t)n, acc, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(h_acc, acc, sizeof h_acc, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        (void)hipFree(acc);
+        if (e != hipSuccess) return fail(HB_EHIP, "hb_stage_csr: symmetry check failed: %s", hipGetErrorString(e));
+        c->sym = h_acc[0] == h_acc[1] && h_acc[2] == h_acc[3];
+    }
     c->h_off.swap(h_off);
     c->n = n;
     c->arcs = arcs;
@@ -348,6 +365,7 @@ static int build_schedule(hb_ctx *c, int p)
                               c->stream));
     HB_TRY(c->scratch.ensure((size_t)s.ntasks << p));
     HB_TRY(c->chunk_flag.ensure(s.ntasks));
+    if (c->sym && p <= 10) HB_TRY(c->dyn.ensure((size_t)s.bin_count[0] + s.bin_count[1] + s.bin_count[2] + 3));
     HB_TRY(hipStreamSynchronize(c->stream));
     c->sched = s;
     c->sched_p = (uint32_t)p;
@@ -407,6 +425,8 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
     HB_TRY(c->lc.ensure(m + 1));
     HB_TRY(c->bias.ensure(2 * HB_BIAS_K));
     if (c->snap_k) HB_TRY(c->snaps.ensure((size_t)c->snap_k * c->n));
+    const bool push_ok = c->sym && p <= 10;
+    if (push_ok) HB_TRY(c->active.ensure(words));
     {
         int rc = build_schedule(c, p);
         if (rc != HB_OK) return rc;
@@ -439,6 +459,18 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
     d.bias_val = c->bias.ptr + HB_BIAS_K;
     d.scratch = c->scratch.ptr;
     d.chunk_flag = c->chunk_flag.ptr;
+    d.sym = push_ok ? 1 : 0;
+    d.small_max = c->small_max;
+    d.mid_max = c->mid_max;
+    d.wave_max = c->wave_max;
+    if (push_ok) {
+        d.active = c->active.ptr;
+        d.dyn_list = c->dyn.ptr + 3;
+        d.dyn_count = c->dyn.ptr;
+        d.dyn_base[0] = 0;
+        d.dyn_base[1] = c->sched.bin_count[0];
+        d.dyn_base[2] = c->sched.bin_count[0] + c->sched.bin_count[1];
+    }
     d.snaps = c->snap_k ? c->snaps.ptr : nullptr;
     d.snap_k = c->snap_k;
     d.n = n;
@@ -601,8 +633,17 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
             stats->kernel_items[sl] = items[sl];
             stats->kernel_team[sl] = sl < 3 ? (uint32_t)c->sched.bin_team[sl] : 64u;
         }
+        if (p <= 10 && executed >= 1)
+            for (int sl = 0; sl < HB_NSLOT; ++sl) {
+                // bins hash the ids of the neighbours and of the node itself, hub chunks
+                // those of the neighbours; the hub merge reads register blocks as always
+                if (sl <= HB_SLOT_CHUNK) stats->kernel_first_arcs[sl] = c->h_rec[1].arcs[sl];
+                if (sl < HB_SLOT_CHUNK) stats->kernel_first_nodes[sl] = c->h_rec[1].nodes[sl];
+            }
+        stats->symmetric = c->sym ? 1 : 0;
         for (uint32_t t = 1; t <= executed; ++t) {
             const HbIterRec &r = c->h_rec[t];
+            stats->push_iterations += r.push ? 1 : 0;
             uint64_t inc = 0, arcs = 0, nodes = 0;
             for (int sl = 0; sl < HB_NSLOT; ++sl) {
                 inc += r.inc[sl];
@@ -852,8 +893,8 @@ extern "C" int hb_surrogate(hb_ctx *c, hb_mlp *mlp, int p_feat, float *pred, flo
     HB_TRY(hipEventCreate(&e0));
     HB_TRY(hipEventCreate(&e1));
     HB_TRY(hipEventRecord(e0, c->stream));
-    HB_TRY(hb::launch_surrogate_tail(mlp, c->off.ptr, c->snaps.ptr, c->inv.ptr, (uint32_t)c->n, c->feat.ptr,
-                                     c->pred.ptr, c->stream));
+    HB_TRY(hb::launch_surrogate_tail(mlp, c->off.ptr, c->snaps.ptr, (uint32_t)c->n, c->feat.ptr, c->pred.ptr,
+                                     c->stream));
     HB_TRY(hipEventRecord(e1, c->stream));
     HB_TRY(hipEventSynchronize(e1));
     float t = 0.0f;

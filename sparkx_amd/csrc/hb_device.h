@@ -26,7 +26,9 @@ struct HbIterState {
     double t, inv_t;     // t and 1/t as f64 (hyper_ball.rs:768-769,842-843)
     uint32_t skip_scan;  // most nodes changed last iteration: recompute all without the
                          // frontier scan (recomputing an unaffected node is a no-op)
-    uint32_t pad;
+    uint32_t push;       // few nodes changed last iteration (symmetric graph): build the
+                         // list of nodes with a changed neighbour from the changed nodes
+                         // instead of scanning every neighbour list
 };
 
 // kernel slots of one iteration: union bins 0..2, hub chunks, hub merge
@@ -40,6 +42,7 @@ struct HbIterRec {  // per-iteration counters, per kernel slot
     unsigned long long nodes[HB_NSLOT];  // nodes recomputed
     unsigned long long chg[HB_NSLOT];    // nodes whose registers changed
     unsigned long long nf_lo, nf_hi;     // N(t) * 2^40 as a 128-bit integer
+    unsigned long long push;             // 1 if iteration t listed its nodes by push
 };
 
 struct HbDev {
@@ -57,6 +60,13 @@ struct HbDev {
     uint32_t snap_k;
     uint8_t *scratch;      // partial sketches of hub chunks
     uint32_t *chunk_flag;  // 1 if the chunk was recomputed this iteration
+    // push iterations (symmetric graphs, p <= 10): nodes with a changed neighbour
+    uint32_t *active;      // bitmap
+    uint32_t *dyn_list;    // their ids per degree bin, bin b at dyn_base[b]
+    uint32_t *dyn_count;   // [3]
+    uint32_t dyn_base[3];
+    uint32_t small_max, mid_max, wave_max;  // degree bounds of bins 0..2 (hubs above)
+    int sym;               // the CSR holds v -> u for every u -> v
     uint32_t n;
     uint32_t n_linked;     // nodes with at least one neighbour (the scheduled ones)
     uint32_t p;

@@ -32,12 +32,16 @@ hipError_t launch_fill(uint32_t *p, uint64_t count, uint32_t value, hipStream_t 
 hipError_t launch_iteration(const HbDev &d, const HbSchedule &sc, bool nf, hipStream_t s,
                             hipEvent_t *ev = nullptr);
 hipError_t launch_estimates(const HbDev &d, const uint8_t *regs, float *out, hipStream_t s);
+// acc[4] (zeroed by the caller) += arc fingerprints: symmetric iff acc[0] == acc[1]
+// and acc[2] == acc[3]
+hipError_t arc_fingerprints(const uint32_t *off, const uint32_t *nbr, uint32_t n, unsigned long long *acc,
+                            hipStream_t s);
 hipError_t launch_centrality(const HbDev &d, const float *est, int kind, int normalize, double *out,
                              hipStream_t s);
 
 // surrogate tail on a staged graph: features from snapshots, MLP inference
-hipError_t launch_surrogate_tail(struct ::hb_mlp *n, const uint32_t *off, const float *snaps, const double *h4,
-                                 uint32_t nodes, float *x_dev, float *out_dev, hipStream_t s);
+hipError_t launch_surrogate_tail(struct ::hb_mlp *n, const uint32_t *off, const float *snaps, uint32_t nodes,
+                                 float *x_dev, float *out_dev, hipStream_t s);
 int mlp_device(const struct ::hb_mlp *n);
 // exact BFS sums of sources [s_begin, s_end) over a staged CSR (hb_exact.hip)
 int exact_bfs(const uint32_t *off, const uint32_t *nbr, uint32_t n, uint64_t s_begin, uint64_t s_end,

@@ -106,13 +106,140 @@ __global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t *p, uint64_t count
         p[i] = value;
 }
 
-// start of iteration t: clear the bitmap iteration t writes (bits[t & 1])
+// start of iteration t: clear the bitmap iteration t writes (bits[t & 1]); in a push
+// iteration also the active bitmap and the per-bin list counts
 __global__ __launch_bounds__(kBlock) void k_begin(HbDev d, uint32_t words)
 {
     if (d.st->done) return;
     uint32_t *bw = (d.st->iter & 1u) ? d.bits1 : d.bits0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += gridDim.x * blockDim.x)
+    const bool push = d.st->push != 0;
+    if (push && blockIdx.x == 0 && threadIdx.x < 3) d.dyn_count[threadIdx.x] = 0u;
+    if (push && blockIdx.x == 0 && threadIdx.x == 0) d.rec[d.st->iter].push = 1ull;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += gridDim.x * blockDim.x) {
         bw[i] = 0u;
+        if (push) d.active[i] = 0u;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// push iterations (symmetric graph, few changes at t-1): u must be recomputed
+// iff one of its neighbours changed, i.e. iff u is a neighbour of a changed node.
+// k_push_mark sets those bits from the changed nodes' lists (a wavefront per
+// bitmap word); k_push_compact lists them per degree bin with wavefront-aggregated
+// appends and copies c_{t-1}(u) into the output buffer for every non-hub node that
+// changed at t-1 but has no changed neighbour (hubs: k_merge copies).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_push_mark(HbDev d, uint32_t words)
+{
+    const HbIterState *st = d.st;
+    if (st->done || !st->push) return;
+    const uint32_t *bitsR = (st->iter & 1u) ? d.bits0 : d.bits1;
+    const uint32_t lane = threadIdx.x & (HB_WAVE - 1);
+    const uint32_t waves = gridDim.x * (kBlock / HB_WAVE);
+    for (uint32_t wd = blockIdx.x * (kBlock / HB_WAVE) + threadIdx.x / HB_WAVE; wd < words; wd += waves) {
+        uint32_t m = bitsR[wd];  // one word per wavefront: the loop below is uniform
+        while (m) {
+            const uint32_t w = wd * 32u + (uint32_t)(__ffs(m) - 1);
+            m &= m - 1u;
+            const uint32_t e1 = d.off[w + 1];
+            for (uint32_t e = d.off[w] + lane; e < e1; e += HB_WAVE) {
+                const uint32_t u = d.nbr[e], bit = 1u << (u & 31u);
+                if (!(d.active[u >> 5] & bit)) atomicOr(d.active + (u >> 5), bit);
+            }
+        }
+    }
+}
+
+template <int P>
+__global__ __launch_bounds__(kBlock) void k_push_compact(HbDev d, uint32_t words)
+{
+    const HbIterState *st = d.st;
+    if (st->done || !st->push) return;
+    const uint32_t *bitsR = (st->iter & 1u) ? d.bits0 : d.bits1;
+    const uint8_t *in = st->parity ? d.regs1 : d.regs0;
+    uint8_t *out = st->parity ? d.regs0 : d.regs1;
+    const uint32_t lane = threadIdx.x & (HB_WAVE - 1);
+    // the trip count is uniform across a wavefront, so every lane takes part in the ballots
+    for (uint32_t base = blockIdx.x * blockDim.x; base < words; base += gridDim.x * blockDim.x) {
+        const uint32_t wd = base + threadIdx.x;
+        uint32_t a = 0, c = 0;
+        if (wd < words) {
+            a = d.active[wd];
+            c = bitsR[wd] & ~a;
+        }
+        while (c) {
+            const uint32_t u = wd * 32u + (uint32_t)(__ffs(c) - 1);
+            c &= c - 1u;
+            if (d.off[u + 1] - d.off[u] > d.wave_max) continue;
+            const uint4 *src = (const uint4 *)(in + ((uint64_t)u << P));
+            uint4 *dst = (uint4 *)(out + ((uint64_t)u << P));
+#pragma unroll
+            for (int k = 0; k < (1 << P) / 16; ++k) dst[k] = src[k];
+        }
+        while (__ballot(a != 0u)) {
+            uint32_t u = 0, bin = 3;
+            if (a) {
+                u = wd * 32u + (uint32_t)(__ffs(a) - 1);
+                a &= a - 1u;
+                const uint32_t deg = d.off[u + 1] - d.off[u];
+                // hubs (bin 3) are not listed: k_chunk / k_merge check their own chunks;
+                // deg 0 cannot occur on a symmetric graph and is never listed either
+                bin = deg == 0 ? 3u : deg <= d.small_max ? 0u : deg <= d.mid_max ? 1u : deg <= d.wave_max ? 2u : 3u;
+            }
+            for (uint32_t b = 0; b < 3; ++b) {
+                const unsigned long long m = __ballot(bin == b);
+                if (!m) continue;
+                const int leader = __ffsll(m) - 1;
+                uint32_t pos = 0;
+                if ((int)lane == leader) pos = atomicAdd(d.dyn_count + b, (uint32_t)__popcll(m));
+                pos = __shfl(pos, leader);
+                if (bin == b)
+                    d.dyn_list[d.dyn_base[b] + pos + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = u;
+            }
+        }
+    }
+}
+
+// Two order-free 64-bit fingerprints of the arc multiset and of its reverse:
+// acc[0] = sum h1(u, v), acc[1] = sum h1(v, u), acc[2], acc[3] the same with h2
+// (wrapping sums of splitmix64 mixes).  Equal pairs mean the CSR is symmetric
+// (as a multiset) except with probability about 2^-128.
+__device__ __forceinline__ uint64_t hb_mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+__global__ __launch_bounds__(kBlock) void k_arc_fingerprint(const uint32_t *off, const uint32_t *nbr, uint32_t n,
+                                                            unsigned long long *acc)
+{
+    const uint32_t lane = threadIdx.x & (HB_WAVE - 1);
+    const uint32_t waves = gridDim.x * (kBlock / HB_WAVE);
+    unsigned long long a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    for (uint32_t u = blockIdx.x * (kBlock / HB_WAVE) + threadIdx.x / HB_WAVE; u < n; u += waves) {
+        const uint32_t e1 = off[u + 1];
+        for (uint32_t e = off[u] + lane; e < e1; e += HB_WAVE) {
+            const uint64_t v = nbr[e];
+            const uint64_t fw = ((uint64_t)u << 32) | v, bw = (v << 32) | u;
+            a0 += hb_mix64(fw + 0x9e3779b97f4a7c15ull);
+            a1 += hb_mix64(bw + 0x9e3779b97f4a7c15ull);
+            a2 += hb_mix64(fw ^ 0xd6e8feb86659fd93ull);
+            a3 += hb_mix64(bw ^ 0xd6e8feb86659fd93ull);
+        }
+    }
+    for (int o = 1; o < HB_WAVE; o <<= 1) {
+        a0 += __shfl_xor(a0, o);
+        a1 += __shfl_xor(a1, o);
+        a2 += __shfl_xor(a2, o);
+        a3 += __shfl_xor(a3, o);
+    }
+    if (lane == 0) {
+        atomicAdd(acc + 0, a0);
+        atomicAdd(acc + 1, a1);
+        atomicAdd(acc + 2, a2);
+        atomicAdd(acc + 3, a3);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -127,6 +254,10 @@ __global__ __launch_bounds__(kBlock) void k_union(HbDev d, const uint32_t *__res
     if (st->done) return;
     HB_TABLES_LDS(P, d, tb)
     const uint32_t it = st->iter;
+    if (st->push) {  // only the bin's nodes with a changed neighbour (k_push_compact)
+        list = d.dyn_list + d.dyn_base[slot_id];
+        count = d.dyn_count[slot_id];
+    }
     HbView v;
     v.off = d.off;
     v.nbr = d.nbr;
@@ -139,7 +270,8 @@ __global__ __launch_bounds__(kBlock) void k_union(HbDev d, const uint32_t *__res
     v.bitsW = (it & 1u) ? d.bits1 : d.bits0;
     v.t = st->t;
     v.inv_t = st->inv_t;
-    v.use_frontier = d.frontier && it > 1 && !st->skip_scan;
+    v.use_frontier = d.frontier && it > 1 && !st->skip_scan && !st->push;
+    v.first = it == 1 && P <= 10;
     const uint32_t tl = threadIdx.x % T;
     const uint32_t teams_per_grid = gridDim.x * (kBlock / T);
     HbCounts c;
@@ -164,6 +296,7 @@ __global__ __launch_bounds__(kBlock) void k_chunk(HbDev d, const uint32_t *__res
     const uint8_t *__restrict__ in = st->parity ? d.regs1 : d.regs0;
     const uint32_t *__restrict__ bitsR = (it & 1u) ? d.bits0 : d.bits1;
     const bool use_frontier = d.frontier && it > 1 && !st->skip_scan;
+    const bool first = it == 1;
     const uint32_t tl = threadIdx.x % T, cl = tl % G, slot = tl / G;
     const uint32_t waves_per_grid = gridDim.x * (kBlock / T);
     unsigned long long c_arcs = 0;
@@ -182,21 +315,7 @@ __global__ __launch_bounds__(kBlock) void k_chunk(HbDev d, const uint32_t *__res
         }
         if (tl == 0) d.chunk_flag[task] = active ? 1u : 0u;
         if (!active) continue;
-        HbAcc pa = hb_acc(make_uint4(0, 0, 0, 0));
-        uint32_t j = slot;
-        for (; j + 3 * S < len; j += 4 * S) {
-            const uint32_t v0 = nb[j], v1 = nb[j + S], v2 = nb[j + 2 * S], v3 = nb[j + 3 * S];
-            const uint4 x0 = *(const uint4 *)(in + ((uint64_t)v0 << P) + cl * 16);
-            const uint4 x1 = *(const uint4 *)(in + ((uint64_t)v1 << P) + cl * 16);
-            const uint4 x2 = *(const uint4 *)(in + ((uint64_t)v2 << P) + cl * 16);
-            const uint4 x3 = *(const uint4 *)(in + ((uint64_t)v3 << P) + cl * 16);
-            hb_acc_max(pa, x0);
-            hb_acc_max(pa, x1);
-            hb_acc_max(pa, x2);
-            hb_acc_max(pa, x3);
-        }
-        for (; j < len; j += S) hb_acc_max(pa, *(const uint4 *)(in + ((uint64_t)nb[j] << P) + cl * 16));
-        uint4 acc = hb_acc_get(pa);
+        uint4 acc = hb_gather_union<P, S>(in, first, nb, len, slot, cl, make_uint4(0, 0, 0, 0));
 #pragma unroll
         for (uint32_t s = G; s < T; s <<= 1) acc = hb_bmax4(acc, hb_shfl_xor4<T>(acc, s));
         if (slot == 0) *(uint4 *)(d.scratch + ((uint64_t)task << P) + cl * 16) = acc;
@@ -503,6 +622,7 @@ __global__ void k_finalize(HbDev d)
     st->iter = it + 1;
     st->parity ^= 1u;
     st->skip_scan = 2 * chg > (unsigned long long)d.n_linked;
+    st->push = d.sym && d.p <= 10 && d.frontier && 16 * chg <= (unsigned long long)d.n_linked;
     st->t = st->t + 1.0;
     st->inv_t = __ddiv_rn(1.0, st->t);
 }
@@ -626,6 +746,12 @@ hipError_t launch_iteration(const HbDev &d, const HbSchedule &sc, bool nf, hipSt
     k_begin<<<grid_for(words, 2048), kBlock, 0, s>>>(d, words);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if (d.sym && d.p <= 10 && d.frontier) {  // return at once unless this is a push iteration
+        k_push_mark<<<grid_for((uint64_t)words * HB_WAVE, 8192), kBlock, 0, s>>>(d, words);
+        HB_DISPATCH_P(d.p, k_push_compact<P><<<grid_for(words, 4096), kBlock, 0, s>>>(d, words));
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     for (int b = 0; b < sc.nbins; ++b) {
         if (sc.bin_count[b] == 0) continue;
         if (ev) (void)hipEventRecord(ev[2 * b], s);
@@ -667,6 +793,14 @@ hipError_t launch_iteration(const HbDev &d, const HbSchedule &sc, bool nf, hipSt
         if (e != hipSuccess) return e;
     }
     k_finalize<<<1, 64, 0, s>>>(d);
+    return hipGetLastError();
+}
+
+hipError_t arc_fingerprints(const uint32_t *off, const uint32_t *nbr, uint32_t n, unsigned long long *acc,
+                            hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    k_arc_fingerprint<<<grid_for((uint64_t)n * HB_WAVE, 8192), kBlock, 0, s>>>(off, nbr, n, acc);
     return hipGetLastError();
 }
 

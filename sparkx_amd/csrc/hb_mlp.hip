@@ -305,32 +305,69 @@ __global__ void k_zero(float *p, int n)
     if (i < n) p[i] = 0.0f;
 }
 
-// surrogate features of every node, all free of the graph's scale:
-//   log(1 + degree), log N_2/N_1, log N_3/N_2, log N_4/N_3, log n/N_4, log (1 + H_4)/n
-// with N_t the estimate after iteration t and H_4 the harmonic accumulator after 4
-// iterations; the network predicts log(1 + H) - log(1 + H_4)
-__global__ void k_features(const uint32_t *off, const float *snaps, const double *h4, uint32_t n, float *x)
+// surrogate inputs of every node: degree, log(1 + degree), log N_1..N_4, with N_t
+// the estimate after iteration t
+__global__ void k_features(const uint32_t *off, const float *snaps, uint32_t n, float *x)
 {
     const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= n) return;
     const float deg = (float)(off[u + 1] - off[u]);
-    const float ln_n = logf((float)n);
-    float l[4];
-    for (int t = 0; t < 4; ++t) l[t] = logf(snaps[(uint64_t)t * n + u]);
     float *r = x + (uint64_t)u * kDIn;
-    r[0] = log1pf(deg);
-    r[1] = l[1] - l[0];
-    r[2] = l[2] - l[1];
-    r[3] = l[3] - l[2];
-    r[4] = ln_n - l[3];
-    r[5] = (float)log1p(h4[u]) - ln_n;
+    r[0] = deg;
+    r[1] = log1pf(deg);
+    for (int t = 0; t < 4; ++t) r[2 + t] = logf(snaps[(uint64_t)t * n + u]);
 }
 
-// score = log(1 + H_4) + predicted residual
-__global__ void k_add_base(const double *h4, uint32_t n, float *out)
+// per-graph sums over nodes with a neighbour: s[0..5] = sum x, s[6..11] = sum x^2,
+// s[12] = count (f64; one atomic per block and sum)
+__global__ __launch_bounds__(256) void k_feature_sums(const uint32_t *off, const float *x, uint32_t n, double *s)
+{
+    __shared__ double red[13][4];
+    double a[13];
+    for (int k = 0; k < 13; ++k) a[k] = 0.0;
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+        if (off[u + 1] == off[u]) continue;
+        const float *r = x + (uint64_t)u * kDIn;
+        for (int k = 0; k < kDIn; ++k) {
+            const double v = r[k];
+            a[k] += v;
+            a[6 + k] += v * v;
+        }
+        a[12] += 1.0;
+    }
+    for (int k = 0; k < 13; ++k)
+        for (int o = 1; o < 64; o <<= 1) a[k] += __shfl_xor(a[k], o);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 13; ++k) red[k][w] = a[k];
+    __syncthreads();
+    if (threadIdx.x < 13) {
+        const double t = red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] + red[threadIdx.x][3];
+        if (t != 0.0) atomicAdd(&s[threadIdx.x], t);
+    }
+}
+
+// standardise every input by the graph's mean and (population) standard deviation
+// over nodes with a neighbour; a constant input maps to 0
+__global__ void k_standardize(const double *s, uint32_t n, float *x)
 {
     const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (u < n) out[u] = (float)log1p(h4[u]) + out[u];
+    if (u >= n) return;
+    const double cnt = s[12] > 0.0 ? s[12] : 1.0;
+    float *r = x + (uint64_t)u * kDIn;
+    for (int k = 0; k < kDIn; ++k) {
+        const double mean = s[k] / cnt;
+        const double var = s[6 + k] / cnt - mean * mean;
+        const double sd = var > 0.0 ? sqrt(var) : 0.0;
+        r[k] = sd > 0.0 ? (float)(((double)r[k] - mean) / sd) : 0.0f;
+    }
+}
+
+// nodes without neighbours have harmonic centrality 0: rank them last
+__global__ void k_isolated_last(const uint32_t *off, uint32_t n, float *out)
+{
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u < n && off[u + 1] == off[u]) out[u] = -3.0e38f;
 }
 
 }  // namespace mlp
@@ -368,11 +405,12 @@ struct hb_mlp {
     float *w = nullptr, *m = nullptr, *v = nullptr, *g = nullptr, *norm = nullptr, *loss = nullptr;
     float *x = nullptr, *y = nullptr, *out = nullptr;
     uint32_t *idx = nullptr;
+    double *stats = nullptr;  // per-graph input sums of the surrogate (13 doubles)
     size_t cap = 0;
     uint64_t step = 0;
     ~hb_mlp()
     {
-        void *ptrs[] = {w, m, v, g, norm, loss, x, y, out, idx};
+        void *ptrs[] = {w, m, v, g, norm, loss, x, y, out, idx, stats};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
@@ -414,6 +452,7 @@ int hb_mlp_create(hb_mlp **out, int device_ordinal, uint64_t seed)
     for (float **p : {&n->w, &n->m, &n->v, &n->g}) HB_MTRY(hipMalloc(p, hb::mlp::kNParams * 4));
     HB_MTRY(hipMalloc(&n->norm, 12 * 4));
     HB_MTRY(hipMalloc(&n->loss, 4));
+    HB_MTRY(hipMalloc(&n->stats, 13 * sizeof(double)));
     // He-uniform weights for the GELU layers, zero biases; identity normalisation
     std::mt19937_64 rng(seed);
     std::vector<float> w(hb::mlp::kNParams, 0.0f);
@@ -557,13 +596,18 @@ int hb_mlp_predict(hb_mlp *n, const float *x, uint64_t count, float *y)
 
 // device-resident surrogate on a staged graph (declared in sparkx_hb.h)
 namespace hb {
-hipError_t launch_surrogate_tail(hb_mlp *n, const uint32_t *off, const float *snaps, const double *h4,
-                                 uint32_t nodes, float *x_dev, float *out_dev, hipStream_t s)
+hipError_t launch_surrogate_tail(hb_mlp *n, const uint32_t *off, const float *snaps, uint32_t nodes, float *x_dev,
+                                 float *out_dev, hipStream_t s)
 {
-    mlp::k_features<<<(nodes + 255) / 256, 256, 0, s>>>(off, snaps, h4, nodes, x_dev);
+    const unsigned g = (nodes + 255) / 256;
+    mlp::k_features<<<g, 256, 0, s>>>(off, snaps, nodes, x_dev);
+    hipError_t e = hipMemsetAsync(n->stats, 0, 13 * sizeof(double), s);
+    if (e != hipSuccess) return e;
+    mlp::k_feature_sums<<<std::min(g, 2048u), 256, 0, s>>>(off, x_dev, nodes, n->stats);
+    mlp::k_standardize<<<g, 256, 0, s>>>(n->stats, nodes, x_dev);
     mlp::Dev d{n->w, n->norm, x_dev, nullptr, nullptr, 0, nodes, 0.0f, nullptr, nullptr, out_dev};
     mlp::k_predict<<<mlp_grid(nodes, 8192), mlp::kBlock, 0, s>>>(d);
-    mlp::k_add_base<<<(nodes + 255) / 256, 256, 0, s>>>(h4, nodes, out_dev);
+    mlp::k_isolated_last<<<g, 256, 0, s>>>(off, nodes, out_dev);
     return hipGetLastError();
 }
 int mlp_device(const hb_mlp *n) { return n->device; }

@@ -16,6 +16,7 @@ namespace hb {
 struct HbView {
     const uint32_t *off, *nbr;
     const uint8_t *in;
+    bool first;             // iteration 1 with P <= 10: sketches hold only their own insertion
     uint8_t *out;
     float *est;
     double *dist, *inv;
@@ -47,6 +48,86 @@ __device__ __forceinline__ bool accumulate(const HbView &v, uint32_t u, float pr
     return false;
 }
 
+// (index << 6) | rank of the insertion of node id v at precision P (P <= 10)
+template <int P>
+__device__ __forceinline__ uint32_t hb_insertion(uint32_t v)
+{
+    uint32_t idx, rank;
+    hb_index_rank(v, P, &idx, &rank);
+    return (idx << 6) | rank;
+}
+
+// This lane's 16 registers of c_0(v) for copy lane cl, from w = (index << 6) | rank.
+__device__ __forceinline__ uint4 hb_own_from_word(uint32_t w, uint32_t cl)
+{
+    const uint32_t idx = w >> 6, rank = w & 63u;
+    uint4 r = make_uint4(0, 0, 0, 0);
+    if ((idx >> 4) == cl) {
+        const uint32_t v = rank << (8u * (idx & 3u)), q = (idx >> 2) & 3u;
+        r.x = q == 0 ? v : 0u;
+        r.y = q == 1 ? v : 0u;
+        r.z = q == 2 ? v : 0u;
+        r.w = q == 3 ? v : 0u;
+    }
+    return r;
+}
+
+// register idx of the copy := max(register, rank) when it falls in lane cl's 16
+__device__ __forceinline__ void hb_put_max(uint4 &a, uint32_t w, uint32_t cl)
+{
+    const uint32_t idx = w >> 6;
+    if ((idx >> 4) != cl) return;
+    const uint32_t sh = 8u * (idx & 3u), q = (idx >> 2) & 3u;
+    const uint32_t m = 0xffu << sh, r = (w & 63u) << sh;
+    uint32_t x = q == 0 ? a.x : q == 1 ? a.y : q == 2 ? a.z : a.w;
+    if ((x & m) < r) x = (x & ~m) | r;
+    if (q == 0) a.x = x;
+    else if (q == 1) a.y = x;
+    else if (q == 2) a.z = x;
+    else a.w = x;
+}
+
+// Union of the sketches of nb[slot], nb[slot + S], ... (j < len) into `start`,
+// for the 16 registers of copy lane cl (the copy is G = 2^P/16 consecutive lanes).
+// In iteration 1 (`first`) every neighbour's sketch is still its own insertion, so
+// instead of gathering its 2^P-byte register block the copy hashes the neighbour ids
+// -- G at a time, one per lane -- and broadcasts each (index, rank) to the copy: no
+// random memory access at all.
+template <int P, uint32_t S>
+__device__ __forceinline__ uint4 hb_gather_union(const uint8_t *__restrict__ in, bool first,
+                                                 const uint32_t *__restrict__ nb, uint32_t len, uint32_t slot,
+                                                 uint32_t cl, uint4 start)
+{
+    constexpr uint32_t G = (1u << P) / 16;
+    uint32_t j = slot;
+    if (first) {
+        uint4 a = start;
+        const uint32_t cnt = len > slot ? (len - slot + S - 1) / S : 0;  // this copy's neighbours
+        const uint32_t lane0 = (threadIdx.x & (HB_WAVE - 1)) - cl;         // first lane of the copy
+        for (uint32_t base = 0; base < cnt; base += G) {
+            const uint32_t q = base + cl;
+            const uint32_t w = q < cnt ? hb_insertion<P>(nb[slot + q * S]) : 0u;
+            const uint32_t lim = min(G, cnt - base);
+            for (uint32_t r = 0; r < lim; ++r) hb_put_max(a, (uint32_t)__shfl((int)w, (int)(lane0 + r)), cl);
+        }
+        return a;
+    }
+    HbAcc pa = hb_acc(start);
+    for (; j + 3 * S < len; j += 4 * S) {
+        const uint32_t v0 = nb[j], v1 = nb[j + S], v2 = nb[j + 2 * S], v3 = nb[j + 3 * S];
+        const uint4 x0 = *(const uint4 *)(in + ((uint64_t)v0 << P) + cl * 16);
+        const uint4 x1 = *(const uint4 *)(in + ((uint64_t)v1 << P) + cl * 16);
+        const uint4 x2 = *(const uint4 *)(in + ((uint64_t)v2 << P) + cl * 16);
+        const uint4 x3 = *(const uint4 *)(in + ((uint64_t)v3 << P) + cl * 16);
+        hb_acc_max(pa, x0);
+        hb_acc_max(pa, x1);
+        hb_acc_max(pa, x2);
+        hb_acc_max(pa, x3);
+    }
+    for (; j < len; j += S) hb_acc_max(pa, *(const uint4 *)(in + ((uint64_t)nb[j] << P) + cl * 16));
+    return hb_acc_get(pa);
+}
+
 template <int P, int T>
 __device__ __forceinline__ void union_node(const HbView &v, const HbTables &tb, uint32_t u, uint32_t tl,
                                            HbCounts &c)
@@ -68,23 +149,9 @@ __device__ __forceinline__ void union_node(const HbView &v, const HbTables &tb, 
             return;
         }
     }
-    const uint8_t *__restrict__ in = v.in;
-    const uint4 own = *(const uint4 *)(in + ((uint64_t)u << P) + cl * 16);
-    HbAcc pa = hb_acc(own);
-    uint32_t j = slot;
-    for (; j + 3 * S < deg; j += 4 * S) {
-        const uint32_t v0 = nb[j], v1 = nb[j + S], v2 = nb[j + 2 * S], v3 = nb[j + 3 * S];
-        const uint4 x0 = *(const uint4 *)(in + ((uint64_t)v0 << P) + cl * 16);
-        const uint4 x1 = *(const uint4 *)(in + ((uint64_t)v1 << P) + cl * 16);
-        const uint4 x2 = *(const uint4 *)(in + ((uint64_t)v2 << P) + cl * 16);
-        const uint4 x3 = *(const uint4 *)(in + ((uint64_t)v3 << P) + cl * 16);
-        hb_acc_max(pa, x0);
-        hb_acc_max(pa, x1);
-        hb_acc_max(pa, x2);
-        hb_acc_max(pa, x3);
-    }
-    for (; j < deg; j += S) hb_acc_max(pa, *(const uint4 *)(in + ((uint64_t)nb[j] << P) + cl * 16));
-    uint4 acc = hb_acc_get(pa);
+    const uint4 own = v.first ? hb_own_from_word(hb_insertion<P>(u), cl)
+                              : *(const uint4 *)(v.in + ((uint64_t)u << P) + cl * 16);
+    uint4 acc = hb_gather_union<P, S>(v.in, v.first, nb, deg, slot, cl, own);
 #pragma unroll
     for (uint32_t s = G; s < T; s <<= 1) acc = hb_bmax4(acc, hb_shfl_xor4<T>(acc, s));
 

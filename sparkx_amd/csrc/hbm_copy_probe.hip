@@ -7,6 +7,8 @@
 //            2^27 indices -- the practical ceiling for the union kernels; also with
 //            64 B and 96 B records packed back to back (what p = 6 needs, and what
 //            p = 7 would need with 6-bit registers)
+//   window : 16 B and 128 B records gathered from tables of 64 MiB .. 4 GiB, to show
+//            where the working set stops fitting the memory-side cache
 // Each is timed with HIP events, best of 5, over a few grid sizes.
 #include <hip/hip_runtime.h>
 
@@ -124,6 +126,18 @@ int main(int argc, char **argv)
                 CHECK(hipGetLastError());
                 const double bytes = (double)n_idx * (4 + 16 * units);
                 printf("gather 2^27 x %u B (+4 B index) grid %d: %.3f ms, %.1f GB/s\n", 16 * units, grid, ms,
+                       bytes / (ms * 1e-3) / 1e9);
+            }
+        }
+        for (uint32_t units : {1u, 8u}) {
+            for (size_t mib : {64, 128, 256, 512, 4096}) {
+                const uint32_t recs = (uint32_t)((mib << 20) / (16 * units));  // records * units * 16 B <= table_bytes
+                make_index<<<8192, 256>>>(idx, n_idx, recs);
+                CHECK(hipGetLastError());
+                const float ms = best_ms(e0, e1, [&] { hbm_gather<<<16384, 256>>>(table, idx, n_idx, units, sink); });
+                CHECK(hipGetLastError());
+                const double bytes = (double)n_idx * (4 + 16 * units);
+                printf("window %zu MiB, %u B records: %.3f ms, %.1f GB/s (record bytes + index)\n", mib, 16 * units, ms,
                        bytes / (ms * 1e-3) / 1e9);
             }
         }

@@ -1,14 +1,14 @@
 """Learned surrogate for anytime centrality rankings.
 
-After 4 HyperBall iterations each node has its estimates N_1..N_4 and its
-truncated harmonic accumulator H_4 = sum_{t<=4} (1/t)(N_t - N_{t-1}) (the
-reference's inverse-distance accumulator, hyper_ball.rs:824-825, stopped early).
 An MLP 6 -> 64 -> 64 -> 1 (GELU; BF16 MFMA matmuls with f32 accumulation; f32
-master weights and Adam) predicts the remaining log-gap to the full run,
-log(1 + H) - log(1 + H_4), from features that do not grow with the graph:
-log(1 + degree), log N_2/N_1, log N_3/N_2, log N_4/N_3, log n/N_4 and
-log((1 + H_4)/n).  The ranking score is log(1 + H_4) plus that prediction.
-Training, inference and features run on the GPU (include/sparkx_hb.h, hb_mlp_*,
+master weights and Adam) maps each node's degree, log(1 + degree) and log N_1..N_4
+-- N_t its HyperBall estimate after t iterations -- to its log harmonic
+centrality log(1 + H), H the full run's inverse-distance accumulator
+(hyper_ball.rs:824-825).  Inputs and target are standardised per graph (mean and
+standard deviation over the nodes that have a neighbour), so that a network trained
+on small graphs ranks the nodes of a larger one: the prediction is the node's
+standardised log harmonic centrality, which orders the graph's nodes.  Training,
+inference and the inputs run on the GPU (include/sparkx_hb.h, hb_mlp_*,
 hb_surrogate)."""
 from __future__ import annotations
 
@@ -23,24 +23,29 @@ from .hyper_ball import HyperBall
 N_PARAMS = 4673
 
 
-def features(g: Graph, snaps: np.ndarray, h4: np.ndarray) -> np.ndarray:
-    """Host form of the surrogate inputs (the device computes the same in f32)."""
+def raw_features(g: Graph, snaps: np.ndarray) -> np.ndarray:
+    """degree, log(1 + degree), log N_1..N_4 (f32, as the device computes them)."""
     deg = g.degrees().astype(np.float32)
-    lg = np.log(np.asarray(snaps[:4], np.float32))
-    ln_n = np.log(np.float32(g.size()))
     x = np.empty((g.size(), 6), np.float32)
-    x[:, 0] = np.log1p(deg)
-    x[:, 1] = lg[1] - lg[0]
-    x[:, 2] = lg[2] - lg[1]
-    x[:, 3] = lg[3] - lg[2]
-    x[:, 4] = ln_n - lg[3]
-    x[:, 5] = np.log1p(np.asarray(h4, np.float64)).astype(np.float32) - ln_n
+    x[:, 0] = deg
+    x[:, 1] = np.log1p(deg)
+    x[:, 2:] = np.log(np.asarray(snaps[:4], np.float32).T)
     return x
 
 
-def base_score(h4: np.ndarray) -> np.ndarray:
-    """log(1 + H_4) in f32, the part of the score the network does not predict."""
-    return np.log1p(np.asarray(h4, np.float64)).astype(np.float32)
+def standardize(v: np.ndarray, linked: np.ndarray) -> np.ndarray:
+    """(v - mean) / std over the linked rows (f64 statistics, population std;
+    a constant column maps to 0), returned as f32."""
+    v = np.asarray(v, np.float64)
+    mean = v[linked].mean(0)
+    sd = v[linked].std(0)
+    safe = np.where(sd > 0, sd, 1.0)
+    return np.where(sd > 0, (v - mean) / safe, 0.0).astype(np.float32)
+
+
+def features(g: Graph, snaps: np.ndarray) -> np.ndarray:
+    """Host form of the surrogate inputs: raw_features standardised per graph."""
+    return standardize(raw_features(g, snaps), g.degrees() > 0)
 
 
 class Surrogate:
@@ -87,7 +92,8 @@ class Surrogate:
 
     def rank(self, h: HyperBall, p_feat: int = 4, want_features: bool = False):
         """Anytime ranking of the graph staged in `h` (its run is replaced): returns
-        (score = log(1 + H_4) + predicted residual, features or None, device ms)."""
+        (predicted standardised log harmonic centrality, nodes without neighbours at
+        -3e38; standardised inputs or None; device ms)."""
         n = h.g.size()
         pred = np.empty(n, np.float32)
         feat = np.empty((n, 6), np.float32) if want_features else None
@@ -109,9 +115,10 @@ class Surrogate:
 
 
 def labelled_graph(g: Graph, p_label: int = 7, p_feat: int = 4, device: int = 0):
-    """(features, target residual log(1+H) - log(1+H_4), H_4, full-run ms, H) of one
-    graph: H from the full HyperBall at p_label, features and H_4 from 4
-    iterations at p_feat."""
+    """(inputs, target, H_4, full-run ms, H) of one graph: H from the full HyperBall
+    at p_label, the target its standardised log(1 + H); inputs from the first 4
+    iterations at p_feat, whose harmonic accumulator H_4 is the truncated baseline.
+    Rows of nodes without neighbours are included; training uses the linked ones."""
     with HyperBall(g, p_label, device=device) as full:
         inv = full.harmonic_geodesic_distances()
         ms_full = full.stats.ms_total + full.stats.ms_init
@@ -121,5 +128,6 @@ def labelled_graph(g: Graph, p_label: int = 7, p_feat: int = 4, device: int = 0)
         snaps = np.empty((4, g.size()), np.float32)
         _lib.check(h._L.hb_read_snapshots(h._ctx, _lib.ptr(snaps, C.c_float)))
         h4 = h.harmonic_geodesic_distances()
-    y = (np.log1p(np.asarray(inv, np.float64)) - np.log1p(np.asarray(h4, np.float64))).astype(np.float32)
-    return features(g, snaps, h4), y, h4, ms_full, inv
+    linked = g.degrees() > 0
+    y = standardize(np.log1p(np.asarray(inv, np.float64))[:, None], linked)[:, 0]
+    return features(g, snaps), y, h4, ms_full, inv
