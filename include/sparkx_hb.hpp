@@ -137,11 +137,12 @@ public:
     }
     const std::vector<double> &get_lins_centrality() const { return cached(lin_); }
 
-    // :327-337, :384-394, :418-424
+    // :327-337, :384-394, :418-424.  The closeness variant tests the *harmonic* slot
+    // (:332), reproduced: empty -> recompute, else the cached closeness (throws if none).
     const std::vector<double> &get_or_compute_closeness_centrality(bool normalization)
     {
-        return closeness_[normalization].empty() && g_.nodes ? compute_closeness_centrality(normalization)
-                                                             : closeness_[normalization];
+        return harmonic_[normalization].empty() ? compute_closeness_centrality(normalization)
+                                                : get_closeness_centrality(normalization);
     }
     const std::vector<double> &get_or_compute_harmonic_centrality(bool normalization)
     {

@@ -85,6 +85,26 @@ def write_results(hb, cache_dir: str, gid: str) -> list:
     return out
 
 
+def drop_cache(hb, cache_dir: str, gid: str) -> None:
+    """HyperBallInner::drop_cache (hyper_ball.rs:564-593) over the files write_results
+    wrote: the accumulators, then the closeness files -- the reference names seq 1
+    for both normalisation slots (:575-579), so with both cached the second removal
+    fails -- then the harmonic files, whose removal is conditioned on the *closeness*
+    slots (:582-586), then Lin's.  A missing file raises FileNotFoundError, as the
+    reference's `?` returns Err.  The counters file is not written here (its
+    hyperloglog-rs layout is unavailable), so it is not removed."""
+    for kind in ("distances", "inv_distances"):
+        os.remove(cache_file(cache_dir, kind, gid))
+    for i in (0, 1):
+        if hb._closeness[i] is not None:
+            os.remove(cache_file(cache_dir, "closeness", gid, 1))
+    for i, seq in ((0, 1), (1, 2)):
+        if hb._closeness[i] is not None:
+            os.remove(cache_file(cache_dir, "harmonic", gid, seq))
+    if hb._lin is not None:
+        os.remove(cache_file(cache_dir, "lin", gid, 0))
+
+
 def read_result(cache_dir: str, kind: str, gid: str, normalize: bool | None = None) -> np.ndarray:
     seq = None if kind in ("distances", "inv_distances") else centrality_seq(kind, bool(normalize))
     return np.fromfile(cache_file(cache_dir, kind, gid, seq), dtype="<f8")

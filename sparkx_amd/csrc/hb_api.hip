@@ -281,9 +281,8 @@ int hb_stage_csr(hb_ctx *c, const void *offsets, int off_bytes, uint64_t n, cons
     c->sym = true;
     if (n) {
         unsigned long long *acc = nullptr, h_acc[4] = {};
-        HB_TRY(hipMalloc(&acc, 4 * sizeof(unsigned long long)));
-        hipError_t e = hipMemsetAsync(acc, 0, 4 * sizeof(unsigned long long), c->stream);
-        if (e == hipSuccess) e = hb::arc_fingerprints(c->off.ptr, c->nbr.ptr, (uint32_t)n, acc, c->stream);
+        HB_TRY(hipMalloc(&acc, (4 + 4 * hb::kFingerprintBlocks) * sizeof(unsigned long long)));
+        hipError_t e = hb::arc_fingerprints(c->off.ptr, c->nbr.ptr, (uint32_t)n, acc, c->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(h_acc, acc, sizeof h_acc, hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         (void)hipFree(acc);

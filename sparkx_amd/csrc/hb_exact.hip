@@ -70,11 +70,12 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_push(const uint32_t *__restri
 }
 
 // newly = next & ~visited; visited |= newly; frontier = newly; per-source counts of
-// newly (one ballot per bit lane per wavefront, then one atomic per block and bit);
-// dist (optional, [node][64] u16) records the level of each newly set bit
+// newly (one ballot per bit lane per wavefront; the block's 64 counts go to
+// part[64 * block], summed by sum_columns); dist (optional, [node][64] u16)
+// records the level of each newly set bit
 __global__ __launch_bounds__(kBfsBlock) void k_bfs_apply(uint64_t *frontier, uint64_t *visited, uint64_t *next,
                                                          uint16_t *dist, uint32_t n, uint32_t level,
-                                                         unsigned long long *counts, unsigned long long *any)
+                                                         unsigned long long *part)
 {
     __shared__ unsigned long long wc[kBfsBlock / 64][64];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -115,10 +116,7 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_apply(uint64_t *frontier, uin
     if (threadIdx.x < 64) {
         unsigned long long s = 0;
         for (int w = 0; w < kBfsBlock / 64; ++w) s += wc[w][threadIdx.x];
-        if (s) {
-            atomicAdd(&counts[threadIdx.x], s);
-            atomicOr(any, 1ull);
-        }
+        part[64 * blockIdx.x + threadIdx.x] = s;
     }
 }
 
@@ -182,16 +180,16 @@ int exact_bfs(const uint32_t *off, const uint32_t *nbr, uint32_t n, uint64_t s_b
                                                    (unsigned long long)s_begin, (unsigned long long)s_end, n);
     if (s_begin == s_end) return HB_OK;
     Scoped bits, cnt, dist, inv, sums;
+    const unsigned grid = (unsigned)std::min<uint64_t>((n + kBfsBlock - 1) / kBfsBlock, 4096);
     HBX_TRY(hipMalloc(&bits.p, (size_t)3 * n * 8));
-    HBX_TRY(hipMalloc(&cnt.p, 65 * 8));
+    HBX_TRY(hipMalloc(&cnt.p, (size_t)(64 + 64 * grid) * 8));  // 64 totals, then 64 per block
     if (inv_sum) {
         HBX_TRY(hipMalloc(&dist.p, (size_t)n * 64 * 2));
         HBX_TRY(hipMalloc(&sums.p, 64 * 8));
     }
     uint64_t *fr = (uint64_t *)bits.p, *vis = fr + n, *nx = fr + 2 * (size_t)n;
     unsigned long long *d_cnt = (unsigned long long *)cnt.p;
-    const unsigned grid = (unsigned)std::min<uint64_t>((n + kBfsBlock - 1) / kBfsBlock, 4096);
-    std::vector<unsigned long long> hc(65);
+    std::vector<unsigned long long> hc(64);
     std::vector<double> h_inv;
     for (uint64_t s0 = s_begin; s0 < s_end; s0 += 64) {
         const uint32_t count = (uint32_t)std::min<uint64_t>(64, s_end - s0);
@@ -202,12 +200,14 @@ int exact_bfs(const uint32_t *off, const uint32_t *nbr, uint32_t n, uint64_t s_b
         for (uint32_t level = 1;; ++level) {
             if (dist.p && level > 65535u)
                 return efail(HB_ERANGE, "hb_exact_bfs: eccentricity above 65535 (harmonic sums need u16 distances)");
-            k_bfs_zero<unsigned long long><<<1, kBfsBlock, 0, s>>>(d_cnt, 65);
             k_bfs_push<<<grid, kBfsBlock, 0, s>>>(off, nbr, fr, vis, nx, n);
-            k_bfs_apply<<<grid, kBfsBlock, 0, s>>>(fr, vis, nx, (uint16_t *)dist.p, n, level, d_cnt, d_cnt + 64);
-            HBX_TRY(hipMemcpyAsync(hc.data(), d_cnt, 65 * 8, hipMemcpyDeviceToHost, s));
+            k_bfs_apply<<<grid, kBfsBlock, 0, s>>>(fr, vis, nx, (uint16_t *)dist.p, n, level, d_cnt + 64);
+            HBX_TRY(sum_columns(d_cnt + 64, grid, 64, d_cnt, s));
+            HBX_TRY(hipMemcpyAsync(hc.data(), d_cnt, 64 * 8, hipMemcpyDeviceToHost, s));
             HBX_TRY(hipStreamSynchronize(s));
-            if (!hc[64]) break;
+            bool any = false;
+            for (int k = 0; k < 64; ++k) any = any || hc[k] != 0;
+            if (!any) break;
             levels = level;
             for (int k = 0; k < 64; ++k) {
                 r[k] += hc[k];

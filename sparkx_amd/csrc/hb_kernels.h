@@ -32,10 +32,14 @@ hipError_t launch_fill(uint32_t *p, uint64_t count, uint32_t value, hipStream_t 
 hipError_t launch_iteration(const HbDev &d, const HbSchedule &sc, bool nf, hipStream_t s,
                             hipEvent_t *ev = nullptr);
 hipError_t launch_estimates(const HbDev &d, const uint8_t *regs, float *out, hipStream_t s);
-// acc[4] (zeroed by the caller) += arc fingerprints: symmetric iff acc[0] == acc[1]
-// and acc[2] == acc[3]
-hipError_t arc_fingerprints(const uint32_t *off, const uint32_t *nbr, uint32_t n, unsigned long long *acc,
+// arc fingerprints into work[0..3] (work holds 4 + 4 * kFingerprintBlocks values):
+// the CSR is symmetric iff work[0] == work[1] and work[2] == work[3]
+constexpr uint32_t kFingerprintBlocks = 2048;
+hipError_t arc_fingerprints(const uint32_t *off, const uint32_t *nbr, uint32_t n, unsigned long long *work,
                             hipStream_t s);
+// out[k] = sum_r part[r * cols + k] in a fixed order (T = unsigned long long or double)
+template <typename T>
+hipError_t sum_columns(const T *part, uint32_t rows, uint32_t cols, T *out, hipStream_t s);
 hipError_t launch_centrality(const HbDev &d, const float *est, int kind, int normalize, double *out,
                              hipStream_t s);
 

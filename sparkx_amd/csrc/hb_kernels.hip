@@ -200,45 +200,61 @@ __global__ __launch_bounds__(kBlock) void k_push_compact(HbDev d, uint32_t words
     }
 }
 
-// Two order-free 64-bit fingerprints of the arc multiset and of its reverse:
-// acc[0] = sum h1(u, v), acc[1] = sum h1(v, u), acc[2], acc[3] the same with h2
-// (wrapping sums of splitmix64 mixes).  Equal pairs mean the CSR is symmetric
-// (as a multiset) except with probability about 2^-128.
+// Order-free fingerprints of the arc multiset and of its reverse, with h a
+// splitmix64 mix of the pair: acc[0] = sum h(u, v), acc[1] = sum h(v, u),
+// acc[2] = sum h(u, v)^2, acc[3] = sum h(v, u)^2 (wrapping).  A symmetric multiset
+// gives acc[0] == acc[1] and acc[2] == acc[3]; any other CSR matches both sums only
+// by a coincidence of two 64-bit equations.
 __device__ __forceinline__ uint64_t hb_mix64(uint64_t z)
 {
+    z += 0x9e3779b97f4a7c15ull;
     z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
     z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
     return z ^ (z >> 31);
 }
 
+// One wavefront per window of 64 consecutive nodes walks the window's arcs
+// contiguously; each lane finds its arc's source node by a binary search over the
+// window's offsets held one per lane (shuffles).  Each block writes its four sums to
+// part[4 * block]; k_sum_columns adds the blocks (no same-address atomics).
 __global__ __launch_bounds__(kBlock) void k_arc_fingerprint(const uint32_t *off, const uint32_t *nbr, uint32_t n,
-                                                            unsigned long long *acc)
+                                                            unsigned long long *part)
 {
-    const uint32_t lane = threadIdx.x & (HB_WAVE - 1);
+    __shared__ unsigned long long red[kBlock / HB_WAVE][4];
+    const uint32_t lane = threadIdx.x & (HB_WAVE - 1), wv = threadIdx.x / HB_WAVE;
     const uint32_t waves = gridDim.x * (kBlock / HB_WAVE);
-    unsigned long long a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-    for (uint32_t u = blockIdx.x * (kBlock / HB_WAVE) + threadIdx.x / HB_WAVE; u < n; u += waves) {
-        const uint32_t e1 = off[u + 1];
-        for (uint32_t e = off[u] + lane; e < e1; e += HB_WAVE) {
-            const uint64_t v = nbr[e];
-            const uint64_t fw = ((uint64_t)u << 32) | v, bw = (v << 32) | u;
-            a0 += hb_mix64(fw + 0x9e3779b97f4a7c15ull);
-            a1 += hb_mix64(bw + 0x9e3779b97f4a7c15ull);
-            a2 += hb_mix64(fw ^ 0xd6e8feb86659fd93ull);
-            a3 += hb_mix64(bw ^ 0xd6e8feb86659fd93ull);
+    const uint32_t windows = (n + HB_WAVE - 1) / HB_WAVE;
+    unsigned long long a[4] = {0, 0, 0, 0};
+    for (uint32_t win = blockIdx.x * (kBlock / HB_WAVE) + wv; win < windows; win += waves) {
+        const uint32_t u0 = win * HB_WAVE, cnt = min((uint32_t)HB_WAVE, n - u0);
+        const uint32_t my_off = lane < cnt ? off[u0 + lane] : 0xffffffffu;  // start of node u0 + lane
+        const uint32_t e0 = off[u0], e1 = off[u0 + cnt];
+        for (uint32_t base = e0; base < e1; base += HB_WAVE) {  // uniform trip count
+            const uint32_t e = base + lane;
+            uint32_t lo = 0;  // last node of the window whose list starts at or before e
+            for (uint32_t step = HB_WAVE / 2; step; step >>= 1) {
+                const uint32_t start = (uint32_t)__shfl((int)my_off, (int)(lo + step));
+                if (start <= e) lo += step;
+            }
+            if (e < e1) {
+                const uint64_t u = u0 + lo, v = nbr[e];
+                const uint64_t hf = hb_mix64((u << 32) | v), hb = hb_mix64((v << 32) | u);
+                a[0] += hf;
+                a[1] += hb;
+                a[2] += hf * hf;
+                a[3] += hb * hb;
+            }
         }
     }
-    for (int o = 1; o < HB_WAVE; o <<= 1) {
-        a0 += __shfl_xor(a0, o);
-        a1 += __shfl_xor(a1, o);
-        a2 += __shfl_xor(a2, o);
-        a3 += __shfl_xor(a3, o);
-    }
-    if (lane == 0) {
-        atomicAdd(acc + 0, a0);
-        atomicAdd(acc + 1, a1);
-        atomicAdd(acc + 2, a2);
-        atomicAdd(acc + 3, a3);
+    for (int k = 0; k < 4; ++k)
+        for (int o = 1; o < HB_WAVE; o <<= 1) a[k] += __shfl_xor(a[k], o);
+    if (lane == 0)
+        for (int k = 0; k < 4; ++k) red[wv][k] = a[k];
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        unsigned long long t = 0;
+        for (int w = 0; w < kBlock / HB_WAVE; ++w) t += red[w][threadIdx.x];
+        part[4 * blockIdx.x + threadIdx.x] = t;
     }
 }
 
@@ -271,7 +287,9 @@ __global__ __launch_bounds__(kBlock) void k_union(HbDev d, const uint32_t *__res
     v.t = st->t;
     v.inv_t = st->inv_t;
     v.use_frontier = d.frontier && it > 1 && !st->skip_scan && !st->push;
+    __shared__ uint32_t s_first[kBlock * 16];
     v.first = it == 1 && P <= 10;
+    v.scr = s_first;
     const uint32_t tl = threadIdx.x % T;
     const uint32_t teams_per_grid = gridDim.x * (kBlock / T);
     HbCounts c;
@@ -297,6 +315,7 @@ __global__ __launch_bounds__(kBlock) void k_chunk(HbDev d, const uint32_t *__res
     const uint32_t *__restrict__ bitsR = (it & 1u) ? d.bits0 : d.bits1;
     const bool use_frontier = d.frontier && it > 1 && !st->skip_scan;
     const bool first = it == 1;
+    __shared__ uint32_t s_first[kBlock * 16];
     const uint32_t tl = threadIdx.x % T, cl = tl % G, slot = tl / G;
     const uint32_t waves_per_grid = gridDim.x * (kBlock / T);
     unsigned long long c_arcs = 0;
@@ -315,7 +334,7 @@ __global__ __launch_bounds__(kBlock) void k_chunk(HbDev d, const uint32_t *__res
         }
         if (tl == 0) d.chunk_flag[task] = active ? 1u : 0u;
         if (!active) continue;
-        uint4 acc = hb_gather_union<P, S>(in, first, nb, len, slot, cl, make_uint4(0, 0, 0, 0));
+        uint4 acc = hb_gather_union<P, S>(in, first, s_first, nb, len, slot, cl, make_uint4(0, 0, 0, 0));
 #pragma unroll
         for (uint32_t s = G; s < T; s <<= 1) acc = hb_bmax4(acc, hb_shfl_xor4<T>(acc, s));
         if (slot == 0) *(uint4 *)(d.scratch + ((uint64_t)task << P) + cl * 16) = acc;
@@ -796,13 +815,44 @@ hipError_t launch_iteration(const HbDev &d, const HbSchedule &sc, bool nf, hipSt
     return hipGetLastError();
 }
 
-hipError_t arc_fingerprints(const uint32_t *off, const uint32_t *nbr, uint32_t n, unsigned long long *acc,
+hipError_t arc_fingerprints(const uint32_t *off, const uint32_t *nbr, uint32_t n, unsigned long long *work,
                             hipStream_t s)
 {
     if (n == 0) return hipSuccess;
-    k_arc_fingerprint<<<grid_for((uint64_t)n * HB_WAVE, 8192), kBlock, 0, s>>>(off, nbr, n, acc);
+    k_arc_fingerprint<<<kFingerprintBlocks, kBlock, 0, s>>>(off, nbr, n, work + 4);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return sum_columns(work + 4, kFingerprintBlocks, 4, work, s);
+}
+
+// out[k] = sum over r < rows of part[r * cols + k], one block per column, a fixed
+// summation order (so f64 sums are reproducible)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sum_columns(const T *part, uint32_t rows, uint32_t cols, T *out)
+{
+    __shared__ T red[kBlock];
+    const uint32_t k = blockIdx.x;
+    T t = 0;
+    for (uint32_t r = threadIdx.x; r < rows; r += kBlock) t += part[(uint64_t)r * cols + k];
+    red[threadIdx.x] = t;
+    __syncthreads();
+    for (uint32_t w = kBlock / 2; w; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[k] = red[0];
+}
+
+template <typename T>
+hipError_t sum_columns(const T *part, uint32_t rows, uint32_t cols, T *out, hipStream_t s)
+{
+    if (cols == 0) return hipSuccess;
+    k_sum_columns<T><<<cols, kBlock, 0, s>>>(part, rows, cols, out);
     return hipGetLastError();
 }
+template hipError_t sum_columns<unsigned long long>(const unsigned long long *, uint32_t, uint32_t,
+                                                    unsigned long long *, hipStream_t);
+template hipError_t sum_columns<double>(const double *, uint32_t, uint32_t, double *, hipStream_t);
 
 hipError_t launch_estimates(const HbDev &d, const uint8_t *regs, float *out, hipStream_t s)
 {

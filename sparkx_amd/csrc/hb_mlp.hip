@@ -318,9 +318,11 @@ __global__ void k_features(const uint32_t *off, const float *snaps, uint32_t n, 
     for (int t = 0; t < 4; ++t) r[2 + t] = logf(snaps[(uint64_t)t * n + u]);
 }
 
-// per-graph sums over nodes with a neighbour: s[0..5] = sum x, s[6..11] = sum x^2,
-// s[12] = count (f64; one atomic per block and sum)
-__global__ __launch_bounds__(256) void k_feature_sums(const uint32_t *off, const float *x, uint32_t n, double *s)
+// per-graph sums over nodes with a neighbour, per block: part[13 * block + k] with
+// k < 6: sum x_k, 6 <= k < 12: sum x_k^2, 12: count (f64; summed over the blocks
+// in a fixed order by sum_columns)
+constexpr unsigned kSumBlocks = 2048;
+__global__ __launch_bounds__(256) void k_feature_sums(const uint32_t *off, const float *x, uint32_t n, double *part)
 {
     __shared__ double red[13][4];
     double a[13];
@@ -341,10 +343,9 @@ __global__ __launch_bounds__(256) void k_feature_sums(const uint32_t *off, const
     if ((threadIdx.x & 63) == 0)
         for (int k = 0; k < 13; ++k) red[k][w] = a[k];
     __syncthreads();
-    if (threadIdx.x < 13) {
-        const double t = red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] + red[threadIdx.x][3];
-        if (t != 0.0) atomicAdd(&s[threadIdx.x], t);
-    }
+    if (threadIdx.x < 13)
+        part[13 * blockIdx.x + threadIdx.x] =
+            red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] + red[threadIdx.x][3];
 }
 
 // standardise every input by the graph's mean and (population) standard deviation
@@ -405,7 +406,7 @@ struct hb_mlp {
     float *w = nullptr, *m = nullptr, *v = nullptr, *g = nullptr, *norm = nullptr, *loss = nullptr;
     float *x = nullptr, *y = nullptr, *out = nullptr;
     uint32_t *idx = nullptr;
-    double *stats = nullptr;  // per-graph input sums of the surrogate (13 doubles)
+    double *stats = nullptr;  // per-graph input sums of the surrogate: 13 totals, then 13 per block
     size_t cap = 0;
     uint64_t step = 0;
     ~hb_mlp()
@@ -452,7 +453,7 @@ int hb_mlp_create(hb_mlp **out, int device_ordinal, uint64_t seed)
     for (float **p : {&n->w, &n->m, &n->v, &n->g}) HB_MTRY(hipMalloc(p, hb::mlp::kNParams * 4));
     HB_MTRY(hipMalloc(&n->norm, 12 * 4));
     HB_MTRY(hipMalloc(&n->loss, 4));
-    HB_MTRY(hipMalloc(&n->stats, 13 * sizeof(double)));
+    HB_MTRY(hipMalloc(&n->stats, 13 * (1 + hb::mlp::kSumBlocks) * sizeof(double)));
     // He-uniform weights for the GELU layers, zero biases; identity normalisation
     std::mt19937_64 rng(seed);
     std::vector<float> w(hb::mlp::kNParams, 0.0f);
@@ -601,9 +602,11 @@ hipError_t launch_surrogate_tail(hb_mlp *n, const uint32_t *off, const float *sn
 {
     const unsigned g = (nodes + 255) / 256;
     mlp::k_features<<<g, 256, 0, s>>>(off, snaps, nodes, x_dev);
-    hipError_t e = hipMemsetAsync(n->stats, 0, 13 * sizeof(double), s);
+    mlp::k_feature_sums<<<mlp::kSumBlocks, 256, 0, s>>>(off, x_dev, nodes, n->stats + 13);
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    mlp::k_feature_sums<<<std::min(g, 2048u), 256, 0, s>>>(off, x_dev, nodes, n->stats);
+    e = sum_columns(n->stats + 13, mlp::kSumBlocks, 13, n->stats, s);
+    if (e != hipSuccess) return e;
     mlp::k_standardize<<<g, 256, 0, s>>>(n->stats, nodes, x_dev);
     mlp::Dev d{n->w, n->norm, x_dev, nullptr, nullptr, 0, nodes, 0.0f, nullptr, nullptr, out_dev};
     mlp::k_predict<<<mlp_grid(nodes, 8192), mlp::kBlock, 0, s>>>(d);

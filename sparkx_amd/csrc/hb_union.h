@@ -17,6 +17,7 @@ struct HbView {
     const uint32_t *off, *nbr;
     const uint8_t *in;
     bool first;             // iteration 1 with P <= 10: sketches hold only their own insertion
+    uint32_t *scr;          // LDS, 16 u32 per lane of the block (iteration 1 only)
     uint8_t *out;
     float *est;
     double *dist, *inv;
@@ -90,27 +91,35 @@ __device__ __forceinline__ void hb_put_max(uint4 &a, uint32_t w, uint32_t cl)
 // Union of the sketches of nb[slot], nb[slot + S], ... (j < len) into `start`,
 // for the 16 registers of copy lane cl (the copy is G = 2^P/16 consecutive lanes).
 // In iteration 1 (`first`) every neighbour's sketch is still its own insertion, so
-// instead of gathering its 2^P-byte register block the copy hashes the neighbour ids
-// -- G at a time, one per lane -- and broadcasts each (index, rank) to the copy: no
-// random memory access at all.
+// instead of gathering its 2^P-byte register block the copy's lanes hash the
+// neighbour ids, one each, and raise register idx to the rank with an LDS atomic max
+// on a u32 per register (`scr`: 16 u32 per lane, the copy's registers contiguous
+// from the copy's first lane); no random global memory access at all.
 template <int P, uint32_t S>
-__device__ __forceinline__ uint4 hb_gather_union(const uint8_t *__restrict__ in, bool first,
+__device__ __forceinline__ uint4 hb_gather_union(const uint8_t *__restrict__ in, bool first, uint32_t *scr,
                                                  const uint32_t *__restrict__ nb, uint32_t len, uint32_t slot,
                                                  uint32_t cl, uint4 start)
 {
     constexpr uint32_t G = (1u << P) / 16;
     uint32_t j = slot;
     if (first) {
-        uint4 a = start;
-        const uint32_t cnt = len > slot ? (len - slot + S - 1) / S : 0;  // this copy's neighbours
-        const uint32_t lane0 = (threadIdx.x & (HB_WAVE - 1)) - cl;         // first lane of the copy
-        for (uint32_t base = 0; base < cnt; base += G) {
-            const uint32_t q = base + cl;
-            const uint32_t w = q < cnt ? hb_insertion<P>(nb[slot + q * S]) : 0u;
-            const uint32_t lim = min(G, cnt - base);
-            for (uint32_t r = 0; r < lim; ++r) hb_put_max(a, (uint32_t)__shfl((int)w, (int)(lane0 + r)), cl);
+        uint32_t *mine = scr + threadIdx.x * 16u;              // this lane's 16 registers
+        uint32_t *copy = scr + (threadIdx.x - cl) * 16u;       // register idx of the copy at copy[idx]
+        const uint32_t sw[4] = {start.x, start.y, start.z, start.w};
+#pragma unroll
+        for (int k = 0; k < 16; ++k) mine[k] = (sw[k >> 2] >> (8 * (k & 3))) & 0xffu;
+        __builtin_amdgcn_wave_barrier();
+        for (uint32_t q = slot + cl * S; q < len; q += G * S) {
+            const uint32_t w = hb_insertion<P>(nb[q]);
+            atomicMax(copy + (w >> 6), w & 63u);
         }
-        return a;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        uint32_t r[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            r[k] = mine[4 * k] | (mine[4 * k + 1] << 8) | (mine[4 * k + 2] << 16) | (mine[4 * k + 3] << 24);
+        return make_uint4(r[0], r[1], r[2], r[3]);
     }
     HbAcc pa = hb_acc(start);
     for (; j + 3 * S < len; j += 4 * S) {
@@ -151,7 +160,7 @@ __device__ __forceinline__ void union_node(const HbView &v, const HbTables &tb, 
     }
     const uint4 own = v.first ? hb_own_from_word(hb_insertion<P>(u), cl)
                               : *(const uint4 *)(v.in + ((uint64_t)u << P) + cl * 16);
-    uint4 acc = hb_gather_union<P, S>(v.in, v.first, nb, deg, slot, cl, own);
+    uint4 acc = hb_gather_union<P, S>(v.in, v.first, v.scr, nb, deg, slot, cl, own);
 #pragma unroll
     for (uint32_t s = G; s < T; s <<= 1) acc = hb_bmax4(acc, hb_shfl_xor4<T>(acc, s));
 

@@ -9,6 +9,8 @@
 //            p = 7 would need with 6-bit registers)
 //   window : 16 B and 128 B records gathered from tables of 64 MiB .. 4 GiB, to show
 //            where the working set stops fitting the memory-side cache
+//   atomics: one 64-bit atomic add per block (or per wavefront) to the same address,
+//            to size the cost of per-block counters
 // Each is timed with HIP events, best of 5, over a few grid sizes.
 #include <hip/hip_runtime.h>
 
@@ -58,6 +60,11 @@ __global__ void hbm_gather(const uint4 *__restrict__ table, const uint32_t *__re
         }
     }
     if (acc == 0x9e3779b9u) out[blockIdx.x] = acc;
+}
+
+__global__ void atomic_per_block(unsigned long long *p, int per_wave)
+{
+    if (per_wave ? (threadIdx.x & 63) == 0 : threadIdx.x == 0) atomicAdd(p, 1ull);
 }
 
 #define CHECK(x)                                                                  \
@@ -139,6 +146,16 @@ int main(int argc, char **argv)
                 const double bytes = (double)n_idx * (4 + 16 * units);
                 printf("window %zu MiB, %u B records: %.3f ms, %.1f GB/s (record bytes + index)\n", mib, 16 * units, ms,
                        bytes / (ms * 1e-3) / 1e9);
+            }
+        }
+        unsigned long long *cnt = (unsigned long long *)sink;
+        for (int per_wave : {0, 1}) {
+            for (int grid : {1024, 8192, 32768}) {
+                const float ms = best_ms(e0, e1, [&] { atomic_per_block<<<grid, 256>>>(cnt, per_wave); });
+                CHECK(hipGetLastError());
+                const int n_atomics = grid * (per_wave ? 4 : 1);
+                printf("atomics: %d same-address u64 adds (%s), grid %d: %.3f ms, %.1f ns each\n", n_atomics,
+                       per_wave ? "one per wavefront" : "one per block", grid, ms, ms * 1e6 / n_atomics);
             }
         }
         CHECK(hipFree(table));

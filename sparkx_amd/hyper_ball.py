@@ -184,11 +184,14 @@ class HyperBall:
             raise LookupError("no such cached centrality")
         return self._lin
 
-    # hyper_ball.rs:327-337, :384-394, :418-424
+    # hyper_ball.rs:327-337, :384-394, :418-424.  The closeness variant tests the
+    # *harmonic* slot (:332), reproduced: with that slot empty it recomputes, otherwise
+    # it returns the cached closeness and fails like get_closeness_centrality if there
+    # is none.
     def get_or_compute_closeness_centrality(self, normalization: bool) -> np.ndarray:
-        if self._closeness[int(normalization)] is None:
+        if self._harmonic[int(normalization)] is None:
             return self.compute_closeness_centrality(normalization)
-        return self._closeness[int(normalization)]
+        return self.get_closeness_centrality(normalization)
 
     def get_or_compute_harmonic_centrality(self, normalization: bool) -> np.ndarray:
         if self._harmonic[int(normalization)] is None:

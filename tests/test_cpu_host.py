@@ -174,3 +174,32 @@ def test_graph_cache_roundtrip(tmp_path):
     h = cache.load_cached_graph(str(tmp_path), "gid")
     assert h.offsets.dtype == np.dtype("<u8")
     assert np.array_equal(h.offsets, g.offsets) and np.array_equal(h.neighbours, g.neighbours)
+
+
+def test_drop_cache_follows_the_reference_quirks(tmp_path):
+    """hyper_ball.rs:564-593: closeness slots both remove seq 1; harmonic files go
+    with the closeness slots; a missing file is an error."""
+    from types import SimpleNamespace
+
+    from sparkx_amd import cache
+    d, gid = str(tmp_path), "g"
+
+    def touch(kind, seq=None):
+        open(cache.cache_file(d, kind, gid, seq), "wb").close()
+
+    for k in ("distances", "inv_distances"):
+        touch(k)
+    touch("closeness", 1)
+    touch("harmonic", 1)
+    touch("lin", 0)
+    hb = SimpleNamespace(_closeness=[1, None], _harmonic=[None, None], _lin=1)
+    cache.drop_cache(hb, d, gid)
+    assert os.listdir(d) == []
+    for k in ("distances", "inv_distances"):
+        touch(k)
+    touch("closeness", 1)
+    touch("closeness", 2)
+    hb = SimpleNamespace(_closeness=[1, 1], _harmonic=[None, None], _lin=None)
+    with pytest.raises(FileNotFoundError):
+        cache.drop_cache(hb, d, gid)  # the second slot names seq 1 again
+    assert os.path.exists(cache.cache_file(d, "closeness", gid, 2))

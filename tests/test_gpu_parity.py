@@ -239,6 +239,17 @@ def test_centralities_match_oracle(hb, oracle):
             h.get_geodesic_distance(g.size())
         assert h.get_geodesic_distance(5) == ref.dist[5]
 
+    # get_or_compute_closeness tests the harmonic slot (hyper_ball.rs:332), reproduced
+    with hb.HyperBall(g, p) as h:
+        c1 = h.get_or_compute_closeness_centrality(True)     # harmonic slot empty: computes
+        c2 = h.get_or_compute_closeness_centrality(True)     # still empty: computes again
+        assert c1 is not c2 and np.array_equal(c1, c2)
+        h.compute_harmonic_centrality(False)
+        with pytest.raises(LookupError):
+            h.get_or_compute_closeness_centrality(False)     # harmonic set, closeness not
+        h.compute_harmonic_centrality(True)
+        assert h.get_or_compute_closeness_centrality(True) is h.get_closeness_centrality(True)
+
 
 def test_neighbourhood_function_exact(hb, oracle):
     g = hb.Graph.erdos_renyi(20000, 50000, seed=8)
