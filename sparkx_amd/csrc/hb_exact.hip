@@ -52,19 +52,43 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_zero(T *p, uint64_t count)
         p[i] = 0;
 }
 
-// next[v] |= frontier[u] for every arc u -> v whose bits are not all visited at v
+// next[v] |= frontier[u] for every arc u -> v whose bits are not all visited at v.
+// A wavefront takes 64 consecutive nodes at a time: a frontier node of degree <= 64
+// is walked by its own lane, a larger one by the whole wavefront, so hubs do not
+// serialise on one lane.
+__device__ __forceinline__ void hb_bfs_push_arc(const uint64_t *__restrict__ visited, uint64_t *next, uint32_t v,
+                                                uint64_t f)
+{
+    const uint64_t add = f & ~visited[v];
+    if (add) atomicOr((unsigned long long *)&next[v], (unsigned long long)add);
+}
+
 __global__ __launch_bounds__(kBfsBlock) void k_bfs_push(const uint32_t *__restrict__ off, const uint32_t *__restrict__ nbr,
                                                         const uint64_t *__restrict__ frontier,
                                                         const uint64_t *__restrict__ visited, uint64_t *next, uint32_t n)
 {
-    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
-        const uint64_t f = frontier[u];
-        if (!f) continue;
-        const uint32_t e1 = off[u + 1];
-        for (uint32_t e = off[u]; e < e1; ++e) {
-            const uint32_t v = nbr[e];
-            const uint64_t add = f & ~visited[v];
-            if (add) atomicOr((unsigned long long *)&next[v], (unsigned long long)add);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t waves = gridDim.x * (kBfsBlock / 64);
+    for (uint32_t base = (blockIdx.x * (kBfsBlock / 64) + threadIdx.x / 64) * 64; base < n; base += waves * 64) {
+        const uint32_t u = base + lane;  // uniform loop bound: every lane takes part in the ballot
+        uint64_t f = 0;
+        uint32_t e0 = 0, e1 = 0;
+        if (u < n) {
+            f = frontier[u];
+            if (f) {
+                e0 = off[u];
+                e1 = off[u + 1];
+            }
+        }
+        if (e1 - e0 <= 64)
+            for (uint32_t e = e0; e < e1; ++e) hb_bfs_push_arc(visited, next, nbr[e], f);
+        unsigned long long big = __ballot(e1 - e0 > 64);
+        while (big) {
+            const int i = __ffsll(big) - 1;
+            big &= big - 1;
+            const uint32_t b0 = (uint32_t)__shfl((int)e0, i), b1 = (uint32_t)__shfl((int)e1, i);
+            const uint64_t fi = (uint64_t)__shfl((long long)f, i);
+            for (uint32_t e = b0 + lane; e < b1; e += 64) hb_bfs_push_arc(visited, next, nbr[e], fi);
         }
     }
 }

@@ -213,37 +213,41 @@ __device__ __forceinline__ uint64_t hb_mix64(uint64_t z)
     return z ^ (z >> 31);
 }
 
-// One wavefront per window of 64 consecutive nodes walks the window's arcs
-// contiguously; each lane finds its arc's source node by a binary search over the
-// window's offsets held one per lane (shuffles).  Each block writes its four sums to
-// part[4 * block]; k_sum_columns adds the blocks (no same-address atomics).
+// A wavefront takes 64 consecutive nodes at a time: a node of degree <= 64 is walked
+// by its own lane, a larger one by the whole wavefront (ballot over the 64, then a
+// uniform loop), so hubs do not serialise on one lane.  Each block writes its four
+// sums to part[4 * block]; sum_columns adds the blocks in a fixed order.
+__device__ __forceinline__ void hb_fp_add(unsigned long long *a, uint64_t u, uint64_t v)
+{
+    const uint64_t hf = hb_mix64((u << 32) | v), hb = hb_mix64((v << 32) | u);
+    a[0] += hf;
+    a[1] += hb;
+    a[2] += hf * hf;
+    a[3] += hb * hb;
+}
+
 __global__ __launch_bounds__(kBlock) void k_arc_fingerprint(const uint32_t *off, const uint32_t *nbr, uint32_t n,
                                                             unsigned long long *part)
 {
     __shared__ unsigned long long red[kBlock / HB_WAVE][4];
     const uint32_t lane = threadIdx.x & (HB_WAVE - 1), wv = threadIdx.x / HB_WAVE;
     const uint32_t waves = gridDim.x * (kBlock / HB_WAVE);
-    const uint32_t windows = (n + HB_WAVE - 1) / HB_WAVE;
     unsigned long long a[4] = {0, 0, 0, 0};
-    for (uint32_t win = blockIdx.x * (kBlock / HB_WAVE) + wv; win < windows; win += waves) {
-        const uint32_t u0 = win * HB_WAVE, cnt = min((uint32_t)HB_WAVE, n - u0);
-        const uint32_t my_off = lane < cnt ? off[u0 + lane] : 0xffffffffu;  // start of node u0 + lane
-        const uint32_t e0 = off[u0], e1 = off[u0 + cnt];
-        for (uint32_t base = e0; base < e1; base += HB_WAVE) {  // uniform trip count
-            const uint32_t e = base + lane;
-            uint32_t lo = 0;  // last node of the window whose list starts at or before e
-            for (uint32_t step = HB_WAVE / 2; step; step >>= 1) {
-                const uint32_t start = (uint32_t)__shfl((int)my_off, (int)(lo + step));
-                if (start <= e) lo += step;
-            }
-            if (e < e1) {
-                const uint64_t u = u0 + lo, v = nbr[e];
-                const uint64_t hf = hb_mix64((u << 32) | v), hb = hb_mix64((v << 32) | u);
-                a[0] += hf;
-                a[1] += hb;
-                a[2] += hf * hf;
-                a[3] += hb * hb;
-            }
+    for (uint32_t base = (blockIdx.x * (kBlock / HB_WAVE) + wv) * HB_WAVE; base < n; base += waves * HB_WAVE) {
+        const uint32_t u = base + lane;  // uniform loop bound: every lane takes part in the ballot
+        uint32_t e0 = 0, e1 = 0;
+        if (u < n) {
+            e0 = off[u];
+            e1 = off[u + 1];
+        }
+        if (e1 - e0 <= HB_WAVE)
+            for (uint32_t e = e0; e < e1; ++e) hb_fp_add(a, u, nbr[e]);
+        unsigned long long big = __ballot(e1 - e0 > HB_WAVE);
+        while (big) {
+            const int i = __ffsll(big) - 1;
+            big &= big - 1;
+            const uint32_t b0 = (uint32_t)__shfl((int)e0, i), b1 = (uint32_t)__shfl((int)e1, i);
+            for (uint32_t e = b0 + lane; e < b1; e += HB_WAVE) hb_fp_add(a, base + (uint32_t)i, nbr[e]);
         }
     }
     for (int k = 0; k < 4; ++k)
