@@ -459,6 +459,7 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
     d.scratch = c->scratch.ptr;
     d.chunk_flag = c->chunk_flag.ptr;
     d.sym = push_ok ? 1 : 0;
+    d.arcs = c->arcs;
     d.small_max = c->small_max;
     d.mid_max = c->mid_max;
     d.wave_max = c->wave_max;

@@ -93,6 +93,7 @@ __device__ __forceinline__ void batch_tile(const HbBatchDev &b, const HbBatchGra
     v.inv_t = g.st.inv_t;
     v.use_frontier = b.frontier && it > 1 && !g.st.skip_scan;
     v.first = false;
+    v.gather_bits = nullptr;
     v.scr = nullptr;
     const uint32_t *list = b.lists + tile.first;
     for (uint32_t i = lane / T; i < tile.count; i += HB_WAVE / T) union_node<P, T>(v, tb, list[i], lane % T, c);

@@ -41,6 +41,7 @@ struct HbIterRec {  // per-iteration counters, per kernel slot
     unsigned long long arcs[HB_NSLOT];   // neighbour sketches gathered
     unsigned long long nodes[HB_NSLOT];  // nodes recomputed
     unsigned long long chg[HB_NSLOT];    // nodes whose registers changed
+    unsigned long long chg_arcs[HB_NSLOT];  // sum of their degrees
     unsigned long long nf_lo, nf_hi;     // N(t) * 2^40 as a 128-bit integer
     unsigned long long push;             // 1 if iteration t listed its nodes by push
 };
@@ -67,6 +68,7 @@ struct HbDev {
     uint32_t dyn_base[3];
     uint32_t small_max, mid_max, wave_max;  // degree bounds of bins 0..2 (hubs above)
     int sym;               // the CSR holds v -> u for every u -> v
+    uint64_t arcs;         // arcs of the staged CSR
     uint32_t n;
     uint32_t n_linked;     // nodes with at least one neighbour (the scheduled ones)
     uint32_t p;

@@ -22,6 +22,7 @@ struct HbView {
     float *est;
     double *dist, *inv;
     const uint32_t *bitsR;  // registers changed at t-1
+    const uint32_t *gather_bits;  // bitsR when only changed neighbours need gathering, else null
     uint32_t *bitsW;        // registers changed at t
     double t, inv_t;
     bool use_frontier;
@@ -29,6 +30,7 @@ struct HbView {
 
 struct HbCounts {
     unsigned long long inc = 0, arcs = 0, nodes = 0, chg = 0;
+    unsigned long long chg_arcs = 0;  // sum of the degrees of the nodes that changed
 };
 
 __device__ __forceinline__ bool bit_test(const uint32_t *bits, uint32_t v)
@@ -90,13 +92,17 @@ __device__ __forceinline__ void hb_put_max(uint4 &a, uint32_t w, uint32_t cl)
 
 // Union of the sketches of nb[slot], nb[slot + S], ... (j < len) into `start`,
 // for the 16 registers of copy lane cl (the copy is G = 2^P/16 consecutive lanes).
+// With `changed` (the nodes whose registers changed at t-1) only those neighbours are
+// gathered: an unchanged neighbour v has c_{t-1}(v) = c_{t-2}(v), which c_{t-1}(u)
+// already contains (v is u's neighbour), so leaving it out changes nothing.
 // In iteration 1 (`first`) every neighbour's sketch is still its own insertion, so
 // instead of gathering its 2^P-byte register block the copy's lanes hash the
 // neighbour ids, one each, and raise register idx to the rank with an LDS atomic max
 // on a u32 per register (`scr`: 16 u32 per lane, the copy's registers contiguous
 // from the copy's first lane); no random global memory access at all.
-template <int P, uint32_t S>
+template <int P, uint32_t S, bool SEL>
 __device__ __forceinline__ uint4 hb_gather_union(const uint8_t *__restrict__ in, bool first, uint32_t *scr,
+                                                 const uint32_t *__restrict__ changed,
                                                  const uint32_t *__restrict__ nb, uint32_t len, uint32_t slot,
                                                  uint32_t cl, uint4 start)
 {
@@ -122,6 +128,29 @@ __device__ __forceinline__ uint4 hb_gather_union(const uint8_t *__restrict__ in,
         return make_uint4(r[0], r[1], r[2], r[3]);
     }
     HbAcc pa = hb_acc(start);
+    if constexpr (SEL) {
+        const uint4 z = make_uint4(0, 0, 0, 0);  // registers are >= 0: max with 0 is a no-op
+        for (; j + 3 * S < len; j += 4 * S) {
+            const uint32_t v0 = nb[j], v1 = nb[j + S], v2 = nb[j + 2 * S], v3 = nb[j + 3 * S];
+            const bool c0 = bit_test(changed, v0), c1 = bit_test(changed, v1);
+            const bool c2 = bit_test(changed, v2), c3 = bit_test(changed, v3);
+            const uint4 x0 = c0 ? *(const uint4 *)(in + ((uint64_t)v0 << P) + cl * 16) : z;
+            const uint4 x1 = c1 ? *(const uint4 *)(in + ((uint64_t)v1 << P) + cl * 16) : z;
+            const uint4 x2 = c2 ? *(const uint4 *)(in + ((uint64_t)v2 << P) + cl * 16) : z;
+            const uint4 x3 = c3 ? *(const uint4 *)(in + ((uint64_t)v3 << P) + cl * 16) : z;
+            hb_acc_max(pa, x0);
+            hb_acc_max(pa, x1);
+            hb_acc_max(pa, x2);
+            hb_acc_max(pa, x3);
+        }
+        for (; j < len; j += S) {
+            const uint32_t v = nb[j];
+            if (bit_test(changed, v)) hb_acc_max(pa, *(const uint4 *)(in + ((uint64_t)v << P) + cl * 16));
+        }
+        return hb_acc_get(pa);
+    } else {
+        (void)changed;
+    }
     for (; j + 3 * S < len; j += 4 * S) {
         const uint32_t v0 = nb[j], v1 = nb[j + S], v2 = nb[j + 2 * S], v3 = nb[j + 3 * S];
         const uint4 x0 = *(const uint4 *)(in + ((uint64_t)v0 << P) + cl * 16);
@@ -137,7 +166,9 @@ __device__ __forceinline__ uint4 hb_gather_union(const uint8_t *__restrict__ in,
     return hb_acc_get(pa);
 }
 
-template <int P, int T>
+// SEL: gather only the neighbours in v.gather_bits (a separate instantiation, so the
+// dense path's code is unchanged by the selective one)
+template <int P, int T, bool SEL = false>
 __device__ __forceinline__ void union_node(const HbView &v, const HbTables &tb, uint32_t u, uint32_t tl,
                                            HbCounts &c)
 {
@@ -160,7 +191,7 @@ __device__ __forceinline__ void union_node(const HbView &v, const HbTables &tb, 
     }
     const uint4 own = v.first ? hb_own_from_word(hb_insertion<P>(u), cl)
                               : *(const uint4 *)(v.in + ((uint64_t)u << P) + cl * 16);
-    uint4 acc = hb_gather_union<P, S>(v.in, v.first, v.scr, nb, deg, slot, cl, own);
+    uint4 acc = hb_gather_union<P, S, SEL>(v.in, v.first, v.scr, v.gather_bits, nb, deg, slot, cl, own);
 #pragma unroll
     for (uint32_t s = G; s < T; s <<= 1) acc = hb_bmax4(acc, hb_shfl_xor4<T>(acc, s));
 
@@ -175,6 +206,7 @@ __device__ __forceinline__ void union_node(const HbView &v, const HbTables &tb, 
         if (changed) {
             atomicOr(v.bitsW + (u >> 5), 1u << (u & 31u));
             c.chg += 1;
+            c.chg_arcs += deg;
         }
         c.arcs += deg;
         c.nodes += 1;
