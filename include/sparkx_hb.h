@@ -82,6 +82,10 @@ typedef struct {
        lists (symmetric graphs, few changes) instead of by scanning every node */
     int32_t push_iterations;
     int32_t symmetric;           /* the staged CSR is symmetric */
+    /* neighbour register blocks actually read, per kernel slot: kernel_arcs counts the
+       neighbour ids scanned; iteration 1 hashes ids instead of reading blocks, and
+       outside dense iterations only neighbours that changed are read */
+    uint64_t kernel_gathered[HB_NKERNELS];
 } hb_stats;
 
 typedef struct hb_ctx hb_ctx;
@@ -259,6 +263,27 @@ int hb_exact_bfs(hb_ctx *ctx, uint64_t src_begin, uint64_t src_end, uint64_t *re
  * (:476-486,560-644), restricted to a source range. */
 int hb_exact_centrality(hb_ctx *ctx, int kind, int normalization, uint64_t src_begin,
                         uint64_t src_end, double *dst);
+
+/* ---- evaluation metrics, computed on `device` -----------------------------------
+ * v, a, b and rank may be host or device pointers (unified addressing); n doubles
+ * each (n uint32 for rank).
+ * hb_eval_ranks: rank[i] = position of v[i] in a stable descending order (ties keep
+ * index order; -0.0 equals +0.0), bit-identical to the reference's ranks().
+ * Replaces: utils::ranks (src/utils.rs:76-85).
+ *
+ * hb_eval_metrics: out[0] = MAE(a, b), out[1] = MAPE(a, b) in percent (terms with
+ * b[i] == 0 count 0), out[2] = Spearman rho(a, b) over those ranks, out[3] = the
+ * top-k overlap |top_k(a) & top_k(b)| / k with k = max(1, ceil(top_frac * n)).  The
+ * squared rank differences are summed exactly (128-bit integers, rounded once), the
+ * MAE / MAPE sums in a fixed two-stage order; both stay within a few ulps of the
+ * reference's sequential f64 sums.
+ * Replaces: utils::mae / mape / spearman_rho (src/utils.rs:57-74,87-100).
+ *
+ * 1 <= n < 2^32; a NaN anywhere gives HB_EINVAL (the reference's order for it is
+ * unspecified). */
+int hb_eval_ranks(int device, const double *v, uint64_t n, uint32_t *rank);
+int hb_eval_metrics(int device, const double *a, const double *b, uint64_t n, double top_frac,
+                    double *out);
 
 /* Number of nodes / arcs staged (0 if none). */
 uint64_t hb_num_nodes(const hb_ctx *ctx);

@@ -309,6 +309,15 @@ def ranks(v) -> np.ndarray:
     return out
 
 
+def top_overlap(pred, truth, frac: float = 0.01) -> float:
+    """|top-k(pred) & top-k(truth)| / k, k = max(1, ceil(frac * n)); the top k of each
+    array are the first k of its stable descending order (the ranks above)."""
+    rp, rt = ranks(pred), ranks(truth)
+    n = rt.size
+    k = max(1, int(np.ceil(frac * n)))
+    return float(np.count_nonzero((rp < k) & (rt < k))) / k
+
+
 def dbg_mtx_csr(sigma: int, k: int):
     """CSR of gen_dbg_mtx.py(sigma, k) as the reference's MTX reader builds it."""
     n = sigma ** k

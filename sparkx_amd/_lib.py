@@ -30,7 +30,7 @@ EXPORTS = (
     "hb_batch_read_accumulators", "hb_autotune", "hb_batch_autotune",
     "hb_set_snapshots", "hb_read_snapshots", "hb_mlp_create", "hb_mlp_destroy", "hb_mlp_set_params",
     "hb_mlp_get_params", "hb_mlp_gradient", "hb_mlp_train", "hb_mlp_predict", "hb_surrogate",
-    "hb_exact_bfs", "hb_exact_centrality",
+    "hb_exact_bfs", "hb_exact_centrality", "hb_eval_ranks", "hb_eval_metrics",
 )
 
 
@@ -86,6 +86,7 @@ class HbStats(C.Structure):
         ("kernel_first_nodes", C.c_uint64 * HB_NKERNELS),
         ("push_iterations", C.c_int32),
         ("symmetric", C.c_int32),
+        ("kernel_gathered", C.c_uint64 * HB_NKERNELS),
     ]
 
 
@@ -155,6 +156,8 @@ def load() -> C.CDLL:
         "hb_exact_bfs": (C.c_int, [vp, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                    f64p]),
         "hb_exact_centrality": (C.c_int, [vp, C.c_int, C.c_int, C.c_uint64, C.c_uint64, f64p]),
+        "hb_eval_ranks": (C.c_int, [C.c_int, f64p, C.c_uint64, u32p]),
+        "hb_eval_metrics": (C.c_int, [C.c_int, f64p, f64p, C.c_uint64, C.c_double, f64p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

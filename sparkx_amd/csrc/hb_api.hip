@@ -651,6 +651,7 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
                 nodes += r.nodes[sl];
                 stats->kernel_arcs[sl] += r.arcs[sl];
                 stats->kernel_nodes[sl] += r.nodes[sl];
+                stats->kernel_gathered[sl] += r.gathered[sl];
             }
             stats->arcs_gathered += arcs;
             stats->nodes_recomputed += nodes;

@@ -41,7 +41,7 @@ struct HbIterRec {  // per-iteration counters, per kernel slot
     unsigned long long arcs[HB_NSLOT];   // neighbour sketches gathered
     unsigned long long nodes[HB_NSLOT];  // nodes recomputed
     unsigned long long chg[HB_NSLOT];    // nodes whose registers changed
-    unsigned long long chg_arcs[HB_NSLOT];  // sum of their degrees
+    unsigned long long gathered[HB_NSLOT];  // neighbour register blocks read (arcs counts ids scanned)
     unsigned long long nf_lo, nf_hi;     // N(t) * 2^40 as a 128-bit integer
     unsigned long long push;             // 1 if iteration t listed its nodes by push
 };

@@ -57,7 +57,7 @@ __device__ __forceinline__ void block_add5(unsigned long long a, unsigned long l
         if (s1) atomicAdd(&rec->arcs[slot], s1);
         if (s2) atomicAdd(&rec->nodes[slot], s2);
         if (s3) atomicAdd(&rec->chg[slot], s3);
-        if (s4) atomicAdd(&rec->chg_arcs[slot], s4);
+        if (s4) atomicAdd(&rec->gathered[slot], s4);
     }
 }
 
@@ -309,7 +309,7 @@ __global__ __launch_bounds__(kBlock) void k_union(HbDev d, const uint32_t *__res
         for (uint32_t team = blockIdx.x * (kBlock / T) + threadIdx.x / T; team < count; team += teams_per_grid)
             union_node<P, T, false>(v, tb, list[team], tl, c);
     }
-    block_add5(c.inc, c.arcs, c.nodes, c.chg, c.chg_arcs, d.rec + it, slot_id);
+    block_add5(c.inc, c.arcs, c.nodes, c.chg, c.gathered, d.rec + it, slot_id);
 }
 
 // ---------------------------------------------------------------------------
@@ -332,7 +332,7 @@ __global__ __launch_bounds__(kBlock) void k_chunk(HbDev d, const uint32_t *__res
     __shared__ uint32_t s_first[kBlock * 16];
     const uint32_t tl = threadIdx.x % T, cl = tl % G, slot = tl / G;
     const uint32_t waves_per_grid = gridDim.x * (kBlock / T);
-    unsigned long long c_arcs = 0;
+    unsigned long long c_arcs = 0, c_gathered = 0;
 
     for (uint32_t task = blockIdx.x * (kBlock / T) + threadIdx.x / T; task < ntasks;
          task += waves_per_grid) {
@@ -348,17 +348,19 @@ __global__ __launch_bounds__(kBlock) void k_chunk(HbDev d, const uint32_t *__res
         }
         if (tl == 0) d.chunk_flag[task] = active ? 1u : 0u;
         if (!active) continue;
+        uint32_t gathered = 0;  // by this lane's copy
         uint4 acc = use_frontier
                         ? hb_gather_union<P, S, true>(in, first, s_first, bitsR, nb, len, slot, cl,
-                                                            make_uint4(0, 0, 0, 0))
+                                                      make_uint4(0, 0, 0, 0), gathered)
                         : hb_gather_union<P, S, false>(in, first, s_first, nullptr, nb, len, slot, cl,
-                                                       make_uint4(0, 0, 0, 0));
+                                                       make_uint4(0, 0, 0, 0), gathered);
+        if (cl == 0) c_gathered += gathered;
 #pragma unroll
         for (uint32_t s = G; s < T; s <<= 1) acc = hb_bmax4(acc, hb_shfl_xor4<T>(acc, s));
         if (slot == 0) *(uint4 *)(d.scratch + ((uint64_t)task << P) + cl * 16) = acc;
         if (tl == 0) c_arcs += len;
     }
-    block_add5(0, c_arcs, 0, 0, 0, d.rec + it, HB_SLOT_CHUNK);
+    block_add5(0, c_arcs, 0, 0, c_gathered, d.rec + it, HB_SLOT_CHUNK);
 }
 
 // hubs, phase 2: one wavefront per hub merges its own registers with the
@@ -381,7 +383,7 @@ __global__ __launch_bounds__(kBlock) void k_merge(HbDev d, const uint32_t *__res
     uint32_t *bitsW = (it & 1u) ? d.bits1 : d.bits0;
     const uint32_t tl = threadIdx.x % T, cl = tl % G, slot = tl / G;
     const uint32_t waves_per_grid = gridDim.x * (kBlock / T);
-    unsigned long long c_inc = 0, c_nodes = 0, c_chg = 0, c_chg_arcs = 0;
+    unsigned long long c_inc = 0, c_nodes = 0, c_chg = 0;
 
     for (uint32_t h = blockIdx.x * (kBlock / T) + threadIdx.x / T; h < nhubs; h += waves_per_grid) {
         const uint32_t u = hnode[h], t0 = htask0[h], nt = hntask[h];
@@ -412,12 +414,11 @@ __global__ __launch_bounds__(kBlock) void k_merge(HbDev d, const uint32_t *__res
             if (changed) {
                 atomicOr(bitsW + (u >> 5), 1u << (u & 31u));
                 c_chg += 1;
-                c_chg_arcs += d.off[u + 1] - d.off[u];
             }
             c_nodes += 1;
         }
     }
-    block_add5(c_inc, 0, c_nodes, c_chg, c_chg_arcs, d.rec + it, HB_SLOT_MERGE);
+    block_add5(c_inc, 0, c_nodes, c_chg, 0, d.rec + it, HB_SLOT_MERGE);
 }
 
 // ---------------------------------------------------------------------------
@@ -568,7 +569,7 @@ __global__ __launch_bounds__(kBlock) void k_union_wide(HbDev d, const uint32_t *
             c_nodes += 1;
         }
     }
-    block_add5(c_inc, c_arcs, c_nodes, c_chg, 0, d.rec + it, slot_id);
+    block_add5(c_inc, c_arcs, c_nodes, c_chg, c_arcs, d.rec + it, slot_id);  // gathers every row
 }
 
 template <int P>
@@ -646,11 +647,10 @@ __global__ void k_finalize(HbDev d)
     HbIterState *st = d.st;
     if (st->done || threadIdx.x != 0 || blockIdx.x != 0) return;
     const uint32_t it = st->iter;
-    unsigned long long inc = 0, chg = 0, chg_arcs = 0;
+    unsigned long long inc = 0, chg = 0;
     for (int k = 0; k < HB_NSLOT; ++k) {
         inc += d.rec[it].inc[k];
         chg += d.rec[it].chg[k];
-        chg_arcs += d.rec[it].chg_arcs[k];
     }
     if (inc == 0 || (st->max_iters && it >= st->max_iters)) {
         st->done = 1;
@@ -664,7 +664,6 @@ __global__ void k_finalize(HbDev d)
     // push when few nodes changed (on R-MAT-24 a push pass at 24% changed nodes whose lists
     // hold 2% of the arcs measured slower than the scan with selective gathers)
     st->push = d.sym && d.p <= 10 && d.frontier && 16 * chg <= (unsigned long long)d.n_linked;
-    (void)chg_arcs;
     st->t = st->t + 1.0;
     st->inv_t = __ddiv_rn(1.0, st->t);
 }

@@ -30,7 +30,7 @@ struct HbView {
 
 struct HbCounts {
     unsigned long long inc = 0, arcs = 0, nodes = 0, chg = 0;
-    unsigned long long chg_arcs = 0;  // sum of the degrees of the nodes that changed
+    unsigned long long gathered = 0;  // neighbour register blocks read
 };
 
 __device__ __forceinline__ bool bit_test(const uint32_t *bits, uint32_t v)
@@ -104,7 +104,7 @@ template <int P, uint32_t S, bool SEL>
 __device__ __forceinline__ uint4 hb_gather_union(const uint8_t *__restrict__ in, bool first, uint32_t *scr,
                                                  const uint32_t *__restrict__ changed,
                                                  const uint32_t *__restrict__ nb, uint32_t len, uint32_t slot,
-                                                 uint32_t cl, uint4 start)
+                                                 uint32_t cl, uint4 start, uint32_t &gathered)
 {
     constexpr uint32_t G = (1u << P) / 16;
     uint32_t j = slot;
@@ -134,6 +134,7 @@ __device__ __forceinline__ uint4 hb_gather_union(const uint8_t *__restrict__ in,
             const uint32_t v0 = nb[j], v1 = nb[j + S], v2 = nb[j + 2 * S], v3 = nb[j + 3 * S];
             const bool c0 = bit_test(changed, v0), c1 = bit_test(changed, v1);
             const bool c2 = bit_test(changed, v2), c3 = bit_test(changed, v3);
+            gathered += (uint32_t)c0 + (uint32_t)c1 + (uint32_t)c2 + (uint32_t)c3;
             const uint4 x0 = c0 ? *(const uint4 *)(in + ((uint64_t)v0 << P) + cl * 16) : z;
             const uint4 x1 = c1 ? *(const uint4 *)(in + ((uint64_t)v1 << P) + cl * 16) : z;
             const uint4 x2 = c2 ? *(const uint4 *)(in + ((uint64_t)v2 << P) + cl * 16) : z;
@@ -145,12 +146,16 @@ __device__ __forceinline__ uint4 hb_gather_union(const uint8_t *__restrict__ in,
         }
         for (; j < len; j += S) {
             const uint32_t v = nb[j];
-            if (bit_test(changed, v)) hb_acc_max(pa, *(const uint4 *)(in + ((uint64_t)v << P) + cl * 16));
+            if (bit_test(changed, v)) {
+                hb_acc_max(pa, *(const uint4 *)(in + ((uint64_t)v << P) + cl * 16));
+                ++gathered;
+            }
         }
         return hb_acc_get(pa);
     } else {
         (void)changed;
     }
+    gathered += len > slot ? (len - slot + S - 1) / S : 0;
     for (; j + 3 * S < len; j += 4 * S) {
         const uint32_t v0 = nb[j], v1 = nb[j + S], v2 = nb[j + 2 * S], v3 = nb[j + 3 * S];
         const uint4 x0 = *(const uint4 *)(in + ((uint64_t)v0 << P) + cl * 16);
@@ -191,7 +196,9 @@ __device__ __forceinline__ void union_node(const HbView &v, const HbTables &tb, 
     }
     const uint4 own = v.first ? hb_own_from_word(hb_insertion<P>(u), cl)
                               : *(const uint4 *)(v.in + ((uint64_t)u << P) + cl * 16);
-    uint4 acc = hb_gather_union<P, S, SEL>(v.in, v.first, v.scr, v.gather_bits, nb, deg, slot, cl, own);
+    uint32_t gathered = 0;  // by this lane's copy
+    uint4 acc = hb_gather_union<P, S, SEL>(v.in, v.first, v.scr, v.gather_bits, nb, deg, slot, cl, own, gathered);
+    if (cl == 0) c.gathered += gathered;
 #pragma unroll
     for (uint32_t s = G; s < T; s <<= 1) acc = hb_bmax4(acc, hb_shfl_xor4<T>(acc, s));
 
@@ -206,7 +213,6 @@ __device__ __forceinline__ void union_node(const HbView &v, const HbTables &tb, 
         if (changed) {
             atomicOr(v.bitsW + (u >> 5), 1u << (u & 31u));
             c.chg += 1;
-            c.chg_arcs += deg;
         }
         c.arcs += deg;
         c.nodes += 1;

@@ -423,14 +423,32 @@ def test_mlp_oracle_bf16_rounding():
     assert M.bf16(v)[4] == np.float32(-3.140625)
 
 
-def test_package_metrics_match_oracle(oracle):
-    from sparkx_amd import metrics
+def test_oracle_ranks_match_stable_sort(oracle):
+    """The oracle's merge sort against numpy's stable sort (a library routine) on
+    inputs full of ties, signed zeros and infinities."""
     rng = np.random.default_rng(9)
-    for n in (1, 2, 17, 1000):
-        a = rng.integers(0, 5, n).astype(np.float64)  # many ties
+    for n in (1, 2, 17, 1000, 4097):
+        a = rng.integers(-3, 4, n).astype(np.float64)  # many ties
+        a[rng.random(n) < 0.1] = -0.0
+        a[rng.random(n) < 0.02] = np.inf
+        a[rng.random(n) < 0.02] = -np.inf
+        order = np.argsort(-(a + 0.0), kind="stable")  # a + 0.0 maps -0.0 to +0.0
+        want = np.empty(n, np.int64)
+        want[order] = np.arange(n)
+        assert np.array_equal(oracle.ranks(a).astype(np.int64), want)
         b = rng.normal(size=n)
-        assert np.array_equal(metrics.ranks(a), oracle.ranks(a).astype(np.int64))
-        assert metrics.spearman_rho(a, b) == oracle.spearman(a, b)
-        assert abs(metrics.mae(a, b) - oracle.mae(a, b)) <= 1e-12 * max(1, abs(oracle.mae(a, b)))
-    assert metrics.top_overlap([3, 2, 1, 0], [3, 2, 0, 1], 0.5) == 1.0
-    assert metrics.top_overlap([0, 1, 2, 3], [3, 2, 1, 0], 0.25) == 0.0
+        rb = oracle.ranks(b).astype(np.int64)
+        d = oracle.ranks(a).astype(np.int64) - rb
+        ssd = float(sum(int(x) * int(x) for x in d))  # exact below 2^53
+        assert oracle.spearman(a, b) == 1.0 - 6.0 * ssd / (n * max(n * n - 1.0, 1.0))
+
+
+def test_oracle_top_overlap(oracle):
+    assert oracle.top_overlap([3, 2, 1, 0], [3, 2, 0, 1], 0.5) == 1.0
+    assert oracle.top_overlap([0, 1, 2, 3], [3, 2, 1, 0], 0.25) == 0.0
+    assert oracle.top_overlap([1, 1, 1, 1], [0, 0, 0, 1], 0.25) == 0.0  # ties: lowest index first
+    assert oracle.top_overlap([1, 1, 1, 1], [1, 0, 0, 0], 0.25) == 1.0
+    assert oracle.top_overlap([5.0], [7.0], 0.01) == 1.0  # k = max(1, ceil(0.01))
+    v = np.random.default_rng(2).normal(size=1000)
+    assert oracle.top_overlap(v, v, 0.013) == 1.0
+    assert oracle.top_overlap(v, -v, 0.5) == 0.0

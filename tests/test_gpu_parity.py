@@ -103,6 +103,13 @@ def test_fixture_graph_closeness_and_metrics(hb, oracle, sigma, k, fixture, p):
             # linear-counting regime: the reference's published numbers, to the last bit
             from tests.golden.pin_hll_variant import PUBLISHED
             assert metrics == (PUBLISHED[p]["mae"][0], PUBLISHED[p]["mape"][0], PUBLISHED[p]["rho"][0])
+        # the same metrics on the GPU: rho bit-exact (ssd < 2^53 here, so the reference's
+        # sequential sum is exact too), MAE / MAPE within 1e-12 relative
+        from sparkx_amd import metrics as M
+        m = M.evaluate(c, ex)
+        assert m["spearman"] == metrics[2]
+        assert abs(m["mae"] - metrics[0]) <= 1e-12 * metrics[0]
+        assert abs(m["mape"] - metrics[1]) <= 1e-12 * metrics[1]
 
 
 def test_schedules_frontier_dense_and_graph_capture_agree(hb, oracle):
@@ -136,6 +143,10 @@ def test_push_iterations_on_symmetric_graphs(hb, oracle):
             _assert_same(h, ref, oracle, p, str(kw))
             assert h.stats.symmetric == 1
             assert h.stats.push_iterations >= 1, h.stats.push_iterations
+            # iteration 1 hashes ids and later sparse iterations read only changed rows
+            gathered = sum(h.stats.kernel_gathered[s] for s in range(5))
+            scanned = sum(h.stats.kernel_arcs[s] for s in range(5))
+            assert 0 < gathered < scanned - g.width() // 2, (gathered, scanned)
     rng = np.random.default_rng(3)
     n = 4000
     src = rng.integers(0, n, 12000)
@@ -480,10 +491,9 @@ def test_snapshots_match_oracle_estimates(hb, oracle):
         assert np.array_equal(snaps[t - 1], oracle.estimates(4, trace[t])), t
 
 
-def test_surrogate_beats_truncated_hyperball(hb):
+def test_surrogate_beats_truncated_hyperball(hb, oracle):
     """Trained on small R-MAT graphs, the MLP ranks a held-out graph's harmonic
     centrality better than the 4-iteration HyperBall accumulator it starts from."""
-    from sparkx_amd import metrics
     from sparkx_amd.surrogate import Surrogate, labelled_graph
     xs, ys = [], []
     for seed in range(8):
@@ -506,8 +516,8 @@ def test_surrogate_beats_truncated_hyperball(hb):
     assert np.allclose(feat, x, rtol=1e-5, atol=1e-5)
     assert np.array_equal(pred[linked], s.predict(feat[linked]))
     assert np.all(pred[~linked] == np.float32(-3e38))
-    rho_mlp = metrics.spearman_rho(pred[linked], truth[linked])
-    rho_base = metrics.spearman_rho(h4[linked], truth[linked])
+    rho_mlp = oracle.spearman(pred[linked].astype(np.float64), truth[linked])
+    rho_base = oracle.spearman(h4[linked], truth[linked])
     say("surrogate rho", rho_mlp, "truncated", rho_base)
     assert rho_mlp > rho_base
     s.close()
