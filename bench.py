@@ -287,6 +287,8 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=1, help="iterations in the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graph-cache", default=None, help="path prefix to save/load the generated CSR (.npy)")
+    ap.add_argument("--layout", default="degree", choices=["degree", "input"],
+                    help="device node numbering (results are identical)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -308,8 +310,9 @@ def main():
     log(f"graph {args.config}: n={g.size()} arcs={g.width()} max_deg={int(g.degrees().max())} "
         f"built in {time.perf_counter() - t0:.1f}s")
     t0 = time.perf_counter()
-    h = HyperBall(g, p, _run=False)
-    log(f"staged CSR into HBM in {time.perf_counter() - t0:.2f}s (pinned staging, includes validation)")
+    h = HyperBall(g, p, _run=False, layout=args.layout)
+    log(f"staged CSR into HBM in {time.perf_counter() - t0:.2f}s (pinned staging, validation, symmetry "
+        f"check and the {args.layout} node layout)")
 
     for i in range(args.warmup):
         st = h.run(max_iters=max_iters)
@@ -385,6 +388,8 @@ def main():
         "config": {"workload": args.config, **{k: v for k, v in spec.items()}, "p": p,
                    "nodes": g.size(), "arcs": g.width(), "iterations_per_step": st.iterations,
                    "max_iters": max_iters},
+        "layout": args.layout,
+        "layout_build_ms": round(float(st.ms_layout), 2),
         "gteps": round(arcs_g / total / 1e9, 3),
         "gteps_dense_equivalent": round(iters * g.width() / total / 1e9, 3),
         "iteration_GBps_alg": round(iter_gbs, 1),

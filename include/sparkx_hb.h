@@ -43,6 +43,13 @@ typedef enum {
                                  even, its input if odd (hyper_ball.rs:840,865)      */
 #define HB_REGS_LATEST 1      /* output of the last executed iteration              */
 
+/* hb_set_layout: node numbering on the device */
+#define HB_LAYOUT_INPUT 0     /* the caller's numbering */
+#define HB_LAYOUT_DEGREE 1    /* default: nodes by descending degree (ties by id), each
+                                 neighbour list ascending; built on the device when the
+                                 graph is staged.  Results are identical and every read
+                                 returns the caller's order. */
+
 /* hb_centrality kinds */
 #define HB_CLOSENESS 0
 #define HB_HARMONIC 1
@@ -86,6 +93,10 @@ typedef struct {
        neighbour ids scanned; iteration 1 hashes ids instead of reading blocks, and
        outside dense iterations only neighbours that changed are read */
     uint64_t kernel_gathered[HB_NKERNELS];
+    int32_t layout;              /* HB_LAYOUT_* the run used */
+    float ms_layout;             /* device time it took to build the degree layout of the
+                                    staged graph (at hb_stage_csr / hb_set_layout; not
+                                    part of ms_init or ms_total); 0 for HB_LAYOUT_INPUT */
 } hb_stats;
 
 typedef struct hb_ctx hb_ctx;
@@ -117,6 +128,11 @@ int hb_stage_csr(hb_ctx *ctx, const void *offsets, int off_bytes, uint64_t n,
  * Replaces: HyperBallInner::new / new_with_conf and compute_with_proc_mem_impl
  * (src/centralities/hyper_ball.rs:124-157, :702-736, :754-874). `stats` may be NULL. */
 int hb_run(hb_ctx *ctx, int p, int max_iters, uint32_t flags, hb_stats *stats);
+
+/* Chooses the device numbering of the nodes (HB_LAYOUT_*) for the following runs
+ * (builds it now if a graph is staged); discards the results of an earlier run if it
+ * changes.  No reference counterpart: the reference walks the caller's CSR as given. */
+int hb_set_layout(hb_ctx *ctx, int layout);
 
 /* Copies n * 2^p u8 registers (node-major) of the selected buffer to dst.
  * Replaces: the counters file HyperBallInner::counters (hyper_ball.rs:92). */

@@ -13,6 +13,7 @@ LIB_PATH = os.environ.get("SPARKX_HB_LIB") or os.path.join(HERE, "libsparkx_hb.s
 HB_OK, HB_EINVAL, HB_ENOMEM, HB_EHIP, HB_ESTATE, HB_ERANGE = 0, -1, -2, -3, -4, -5
 HB_FLAG_DENSE, HB_FLAG_GRAPH, HB_FLAG_NF, HB_FLAG_TIMING, HB_FLAG_KTIMING = 0x1, 0x2, 0x4, 0x8, 0x10
 HB_REGS_REFERENCE, HB_REGS_LATEST = 0, 1
+HB_LAYOUT_INPUT, HB_LAYOUT_DEGREE = 0, 1
 HB_CLOSENESS, HB_HARMONIC, HB_LIN = 0, 1, 2
 HB_STATS_MAX_ITERS = 4096
 HB_NKERNELS = 5
@@ -30,7 +31,7 @@ EXPORTS = (
     "hb_batch_read_accumulators", "hb_autotune", "hb_batch_autotune",
     "hb_set_snapshots", "hb_read_snapshots", "hb_mlp_create", "hb_mlp_destroy", "hb_mlp_set_params",
     "hb_mlp_get_params", "hb_mlp_gradient", "hb_mlp_train", "hb_mlp_predict", "hb_surrogate",
-    "hb_exact_bfs", "hb_exact_centrality", "hb_eval_ranks", "hb_eval_metrics",
+    "hb_exact_bfs", "hb_exact_centrality", "hb_eval_ranks", "hb_eval_metrics", "hb_set_layout",
 )
 
 
@@ -87,6 +88,8 @@ class HbStats(C.Structure):
         ("push_iterations", C.c_int32),
         ("symmetric", C.c_int32),
         ("kernel_gathered", C.c_uint64 * HB_NKERNELS),
+        ("layout", C.c_int32),
+        ("ms_layout", C.c_float),
     ]
 
 
@@ -158,6 +161,7 @@ def load() -> C.CDLL:
         "hb_exact_centrality": (C.c_int, [vp, C.c_int, C.c_int, C.c_uint64, C.c_uint64, f64p]),
         "hb_eval_ranks": (C.c_int, [C.c_int, f64p, C.c_uint64, u32p]),
         "hb_eval_metrics": (C.c_int, [C.c_int, f64p, f64p, C.c_uint64, C.c_double, f64p]),
+        "hb_set_layout": (C.c_int, [vp, C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

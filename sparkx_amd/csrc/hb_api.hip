@@ -107,6 +107,34 @@ struct hb_ctx {
     uint32_t snap_k = 0;
     DevBuf<float> snaps, feat, pred;
 
+    // node order on the device: HB_LAYOUT_DEGREE renumbers the nodes by descending
+    // degree at the first run after staging; reads return the caller's order
+    int layout = HB_LAYOUT_DEGREE;
+    bool laid_out = false;  // perm .. rorig hold that numbering of the staged graph
+    DevBuf<uint32_t> perm, sigma, roff, rnbr, rorig;
+    std::vector<uint32_t> h_roff;
+    float ms_layout = 0.0f;  // device time of building it
+    DevBuf<uint8_t> readback;  // per-node results moved back to the caller's order
+
+    bool relabelled() const { return layout == HB_LAYOUT_DEGREE && laid_out; }
+    const uint32_t *run_off() const { return relabelled() ? roff.ptr : off.ptr; }
+    const uint32_t *run_nbr() const { return relabelled() ? rnbr.ptr : nbr.ptr; }
+    const uint32_t *run_orig() const { return relabelled() ? rorig.ptr : nbr.ptr; }
+    const uint32_t *run_node_id() const { return relabelled() ? perm.ptr : nullptr; }
+    const std::vector<uint32_t> &run_hoff() const { return relabelled() ? h_roff : h_off; }
+
+    void release_layout()
+    {
+        perm.release();
+        sigma.release();
+        roff.release();
+        rnbr.release();
+        rorig.release();
+        h_roff.clear();
+        laid_out = false;
+        ms_layout = 0.0f;
+    }
+
     void release_run()
     {
         regs0.release();
@@ -130,12 +158,15 @@ struct hb_ctx {
         snaps.release();
         feat.release();
         pred.release();
+        readback.release();
         ran = false;
         sched_p = 0;
     }
 };
 
 static bool set_device(hb_ctx *c) { return hipSetDevice(c->device) == hipSuccess; }
+
+static int ensure_layout(hb_ctx *c);
 
 extern "C" {
 
@@ -168,6 +199,7 @@ void hb_destroy(hb_ctx *c)
     set_device(c);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->release_run();
+    c->release_layout();
     c->off.release();
     c->nbr.release();
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -186,6 +218,22 @@ int hb_set_schedule(hb_ctx *c, uint32_t small_max_deg, uint32_t wave_max_deg, ui
     if (c->wave_max < c->small_max) c->wave_max = c->small_max;
     c->mid_max = std::max(c->small_max, std::min<uint32_t>(c->wave_max, 8 * c->small_max));
     c->sched_p = 0;  // rebuild on the next run
+    return HB_OK;
+}
+
+int hb_set_layout(hb_ctx *c, int layout)
+{
+    if (!c) return fail(HB_EINVAL, "hb_set_layout: ctx is NULL");
+    if (layout != HB_LAYOUT_INPUT && layout != HB_LAYOUT_DEGREE)
+        return fail(HB_EINVAL, "hb_set_layout: bad layout %d", layout);
+    if (layout != c->layout) {
+        if (!set_device(c)) return fail(HB_EHIP, "hipSetDevice failed");
+        (void)hipStreamSynchronize(c->stream);
+        c->release_run();  // results and schedule belong to the old numbering
+        c->layout = layout;
+        if (int rc = ensure_layout(c)) return rc;  // a staged graph gets it now
+    }
+    g_err.clear();
     return HB_OK;
 }
 
@@ -217,6 +265,7 @@ int hb_stage_csr(hb_ctx *c, const void *offsets, int off_bytes, uint64_t n, cons
         h_off[i] = (uint32_t)o;
     }
     c->release_run();
+    c->release_layout();
     c->n = 0;
     c->arcs = 0;
     HB_TRY(c->off.ensure(n + 1));
@@ -292,6 +341,7 @@ int hb_stage_csr(hb_ctx *c, const void *offsets, int off_bytes, uint64_t n, cons
     c->h_off.swap(h_off);
     c->n = n;
     c->arcs = arcs;
+    if (int rc2 = ensure_layout(c)) return rc2;  // degree layout: built with the staging
     g_err.clear();
     return HB_OK;
 }
@@ -307,10 +357,11 @@ static int build_schedule(hb_ctx *c, int p)
     if (c->sched_p == (uint32_t)p && !memcmp(key, c->sched_key, sizeof key)) return HB_OK;
     const uint32_t G = std::min(64u, (1u << p) / 16u);
     const uint32_t n = (uint32_t)c->n;
+    const std::vector<uint32_t> &h_off = c->run_hoff();
     c->n_linked = 0;
     std::vector<uint32_t> bins[3], tnode, tbeg, tend, hnode, htask0, hntask;
     for (uint32_t u = 0; u < n; ++u) {
-        const uint32_t d = c->h_off[u + 1] - c->h_off[u];
+        const uint32_t d = h_off[u + 1] - h_off[u];
         if (d == 0) continue;  // keeps c_0; written to both buffers at init
         ++c->n_linked;
         if (p > 10) bins[2].push_back(u);  // one wavefront per node walks the register row
@@ -321,10 +372,10 @@ static int build_schedule(hb_ctx *c, int p)
             hnode.push_back(u);
             htask0.push_back((uint32_t)tnode.size());
             uint32_t cnt = 0;
-            for (uint32_t e = c->h_off[u]; e < c->h_off[u + 1]; e += c->chunk, ++cnt) {
+            for (uint32_t e = h_off[u]; e < h_off[u + 1]; e += c->chunk, ++cnt) {
                 tnode.push_back(u);
                 tbeg.push_back(e);
-                tend.push_back(std::min(e + c->chunk, c->h_off[u + 1]));
+                tend.push_back(std::min(e + c->chunk, h_off[u + 1]));
             }
             hntask.push_back(cnt);
         }
@@ -380,6 +431,60 @@ static float alpha_f32(uint32_t m)
     return 0.7213f / (1.0f + 1.079f / (float)m);
 }
 
+// Builds the degree-ordered numbering of the staged graph once (device work, timed
+// with events into ms_layout); a no-op for HB_LAYOUT_INPUT or when already built.
+static int ensure_layout(hb_ctx *c)
+{
+    if (c->layout != HB_LAYOUT_DEGREE || c->laid_out || c->n == 0) return HB_OK;
+    const uint32_t n = (uint32_t)c->n, arcs = (uint32_t)c->arcs;
+    HB_TRY(c->perm.ensure(n));
+    HB_TRY(c->sigma.ensure(n));
+    HB_TRY(c->roff.ensure((size_t)n + 1));
+    HB_TRY(c->rnbr.ensure(arcs));
+    HB_TRY(c->rorig.ensure(arcs));
+    hipEvent_t e0, e1;
+    HB_TRY(hipEventCreate(&e0));
+    HB_TRY(hipEventCreate(&e1));
+    (void)hipEventRecord(e0, c->stream);
+    hipError_t e = hb::relabel_by_degree(c->off.ptr, c->nbr.ptr, n, arcs, c->perm.ptr, c->sigma.ptr, c->roff.ptr,
+                                         c->rnbr.ptr, c->rorig.ptr, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+    std::vector<uint32_t> h((size_t)n + 1);
+    if (e == hipSuccess) e = hipMemcpyAsync(h.data(), c->roff.ptr, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost,
+                                            c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    float ms = 0.0f;
+    if (e == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (e != hipSuccess) {
+        c->release_layout();
+        return fail(e == hipErrorOutOfMemory ? HB_ENOMEM : HB_EHIP, "degree layout failed: %s",
+                    hipGetErrorString(e));
+    }
+    c->h_roff.swap(h);
+    c->ms_layout = ms;
+    c->laid_out = true;
+    return HB_OK;
+}
+
+// Copies n per-node rows of row_bytes from device src to host dst in the caller's
+// node order (through c->readback when the nodes are renumbered).  Synchronous.
+static int copy_out(hb_ctx *c, const void *src, void *dst, size_t row_bytes)
+{
+    const size_t bytes = (size_t)c->n * row_bytes;
+    if (!c->relabelled()) {
+        HB_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+        HB_TRY(hipStreamSynchronize(c->stream));
+        return HB_OK;
+    }
+    HB_TRY(c->readback.ensure(bytes));
+    HB_TRY(hb::unpermute_rows(src, c->readback.ptr, c->sigma.ptr, (uint32_t)c->n, row_bytes, c->stream));
+    HB_TRY(hipMemcpyAsync(dst, c->readback.ptr, bytes, hipMemcpyDeviceToHost, c->stream));
+    HB_TRY(hipStreamSynchronize(c->stream));
+    return HB_OK;
+}
+
 extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats *stats)
 {
     if (!c) return fail(HB_EINVAL, "hb_run: ctx is NULL");
@@ -427,7 +532,8 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
     const bool push_ok = c->sym && p <= 10;
     if (push_ok) HB_TRY(c->active.ensure(words));
     {
-        int rc = build_schedule(c, p);
+        int rc = ensure_layout(c);  // normally built at staging already
+        if (rc == HB_OK) rc = build_schedule(c, p);
         if (rc != HB_OK) return rc;
     }
 
@@ -442,8 +548,10 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
 
     HbDev &d = c->dev;
     d = HbDev{};
-    d.off = c->off.ptr;
-    d.nbr = c->nbr.ptr;
+    d.off = c->run_off();
+    d.nbr = c->run_nbr();
+    d.orig = c->run_orig();
+    d.node_id = c->run_node_id();
     d.regs0 = c->regs0.ptr;
     d.regs1 = c->regs1.ptr;
     d.est = c->est.ptr;
@@ -641,6 +749,8 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
                 if (sl < HB_SLOT_CHUNK) stats->kernel_first_nodes[sl] = c->h_rec[1].nodes[sl];
             }
         stats->symmetric = c->sym ? 1 : 0;
+        stats->layout = c->relabelled() ? HB_LAYOUT_DEGREE : HB_LAYOUT_INPUT;
+        stats->ms_layout = c->relabelled() ? c->ms_layout : 0.0f;
         for (uint32_t t = 1; t <= executed; ++t) {
             const HbIterRec &r = c->h_rec[t];
             stats->push_iterations += r.push ? 1 : 0;
@@ -685,7 +795,7 @@ extern "C" int hb_read_registers(hb_ctx *c, int which, uint8_t *dst)
         return fail(HB_EINVAL, "hb_read_registers: bad selector %d", which);
     if (c->n == 0) return HB_OK;
     if (!set_device(c)) return fail(HB_EHIP, "hipSetDevice failed");
-    HB_TRY(hipMemcpy(dst, select_regs(c, which), (size_t)c->n << c->p, hipMemcpyDeviceToHost));
+    if (int rc = copy_out(c, select_regs(c, which), dst, (size_t)1 << c->p)) return rc;
     g_err.clear();
     return HB_OK;
 }
@@ -700,8 +810,7 @@ extern "C" int hb_read_estimates(hb_ctx *c, int which, float *dst)
     if (!set_device(c)) return fail(HB_EHIP, "hipSetDevice failed");
     HB_TRY(c->est_tmp.ensure(c->n));
     HB_TRY(hb::launch_estimates(c->dev, select_regs(c, which), c->est_tmp.ptr, c->stream));
-    HB_TRY(hipMemcpyAsync(dst, c->est_tmp.ptr, c->n * 4, hipMemcpyDeviceToHost, c->stream));
-    HB_TRY(hipStreamSynchronize(c->stream));
+    if (int rc = copy_out(c, c->est_tmp.ptr, dst, 4)) return rc;
     g_err.clear();
     return HB_OK;
 }
@@ -712,8 +821,10 @@ extern "C" int hb_read_accumulators(hb_ctx *c, double *dist, double *inv)
     if (!c->ran) return fail(HB_ESTATE, "hb_read_accumulators: no completed run");
     if (c->n == 0) return HB_OK;
     if (!set_device(c)) return fail(HB_EHIP, "hipSetDevice failed");
-    if (dist) HB_TRY(hipMemcpy(dist, c->dist.ptr, c->n * 8, hipMemcpyDeviceToHost));
-    if (inv) HB_TRY(hipMemcpy(inv, c->inv.ptr, c->n * 8, hipMemcpyDeviceToHost));
+    if (dist)
+        if (int rc = copy_out(c, c->dist.ptr, dist, 8)) return rc;
+    if (inv)
+        if (int rc = copy_out(c, c->inv.ptr, inv, 8)) return rc;
     g_err.clear();
     return HB_OK;
 }
@@ -730,8 +841,7 @@ extern "C" int hb_centrality(hb_ctx *c, int kind, int normalize, double *dst)
     // the epilogues re-estimate the reference's counters buffer (hyper_ball.rs:452,506,544)
     HB_TRY(hb::launch_estimates(c->dev, c->regs0.ptr, c->est_tmp.ptr, c->stream));
     HB_TRY(hb::launch_centrality(c->dev, c->est_tmp.ptr, kind, normalize ? 1 : 0, c->cent.ptr, c->stream));
-    HB_TRY(hipMemcpyAsync(dst, c->cent.ptr, c->n * 8, hipMemcpyDeviceToHost, c->stream));
-    HB_TRY(hipStreamSynchronize(c->stream));
+    if (int rc = copy_out(c, c->cent.ptr, dst, 8)) return rc;
     g_err.clear();
     return HB_OK;
 }
@@ -868,8 +978,8 @@ extern "C" int hb_read_snapshots(hb_ctx *c, float *dst)
     if (!c || !dst) return fail(HB_EINVAL, "hb_read_snapshots: NULL argument");
     if (!c->ran || !c->snap_k || !c->snaps.ptr) return fail(HB_ESTATE, "hb_read_snapshots: no run with snapshots");
     if (!set_device(c)) return fail(HB_EHIP, "hipSetDevice failed");
-    HB_TRY(hipStreamSynchronize(c->stream));
-    HB_TRY(hipMemcpy(dst, c->snaps.ptr, (size_t)c->snap_k * c->n * 4, hipMemcpyDeviceToHost));
+    for (uint32_t t = 0; t < c->snap_k; ++t)
+        if (int rc = copy_out(c, c->snaps.ptr + (size_t)t * c->n, dst + (size_t)t * c->n, 4)) return rc;
     g_err.clear();
     return HB_OK;
 }
@@ -894,7 +1004,7 @@ extern "C" int hb_surrogate(hb_ctx *c, hb_mlp *mlp, int p_feat, float *pred, flo
     HB_TRY(hipEventCreate(&e0));
     HB_TRY(hipEventCreate(&e1));
     HB_TRY(hipEventRecord(e0, c->stream));
-    HB_TRY(hb::launch_surrogate_tail(mlp, c->off.ptr, c->snaps.ptr, (uint32_t)c->n, c->feat.ptr, c->pred.ptr,
+    HB_TRY(hb::launch_surrogate_tail(mlp, c->run_off(), c->snaps.ptr, (uint32_t)c->n, c->feat.ptr, c->pred.ptr,
                                      c->stream));
     HB_TRY(hipEventRecord(e1, c->stream));
     HB_TRY(hipEventSynchronize(e1));
@@ -903,8 +1013,10 @@ extern "C" int hb_surrogate(hb_ctx *c, hb_mlp *mlp, int p_feat, float *pred, flo
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (ms) *ms = st->ms_init + st->ms_total + t;
-    if (pred) HB_TRY(hipMemcpy(pred, c->pred.ptr, c->n * 4, hipMemcpyDeviceToHost));
-    if (features) HB_TRY(hipMemcpy(features, c->feat.ptr, c->n * 24, hipMemcpyDeviceToHost));
+    if (pred)
+        if (int rc2 = copy_out(c, c->pred.ptr, pred, 4)) return rc2;
+    if (features)
+        if (int rc2 = copy_out(c, c->feat.ptr, features, 24)) return rc2;
     g_err.clear();
     return HB_OK;
 }

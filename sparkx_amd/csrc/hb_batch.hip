@@ -82,6 +82,8 @@ __device__ __forceinline__ void batch_tile(const HbBatchDev &b, const HbBatchGra
     HbView v;
     v.off = b.off + g.off_base;
     v.nbr = b.nbr + g.nbr_base;
+    v.orig = v.nbr;
+    v.node_id = nullptr;
     v.in = (g.st.parity ? b.regs1 : b.regs0) + g.reg_base;
     v.out = (g.st.parity ? b.regs0 : b.regs1) + g.reg_base;
     v.est = b.est + g.node_base;

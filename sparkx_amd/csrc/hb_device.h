@@ -49,6 +49,10 @@ struct HbIterRec {  // per-iteration counters, per kernel slot
 struct HbDev {
     const uint32_t *off;   // n+1
     const uint32_t *nbr;   // arcs
+    // ids that are hashed: the caller's id of each node (null: the node's own index)
+    // and of each arc's head (equal to nbr when the nodes keep the caller's order)
+    const uint32_t *node_id;
+    const uint32_t *orig;
     uint8_t *regs0, *regs1;
     float *est;            // estimate of each node's current sketch
     double *dist, *inv;    // accumulators

@@ -43,6 +43,15 @@ hipError_t sum_columns(const T *part, uint32_t rows, uint32_t cols, T *out, hipS
 hipError_t launch_centrality(const HbDev &d, const float *est, int kind, int normalize, double *out,
                              hipStream_t s);
 
+// Degree-ordered layout (hb_layout.hip): perm[i] = caller's id of node i, sigma its
+// inverse; roff / rnbr the renumbered CSR (nodes by descending degree, ties by id;
+// each list ascending), rorig[e] the caller's id of rnbr[e].  Synchronises s.
+hipError_t relabel_by_degree(const uint32_t *off, const uint32_t *nbr, uint32_t n, uint32_t arcs, uint32_t *perm,
+                             uint32_t *sigma, uint32_t *roff, uint32_t *rnbr, uint32_t *rorig, hipStream_t s);
+// dst row u (caller's order) = src row sigma[u]; row_bytes a multiple of 4
+hipError_t unpermute_rows(const void *src, void *dst, const uint32_t *sigma, uint32_t n, size_t row_bytes,
+                          hipStream_t s);
+
 // surrogate tail on a staged graph: features from snapshots, MLP inference
 hipError_t launch_surrogate_tail(struct ::hb_mlp *n, const uint32_t *off, const float *snaps, uint32_t nodes,
                                  float *x_dev, float *out_dev, hipStream_t s);

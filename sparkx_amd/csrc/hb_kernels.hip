@@ -87,7 +87,7 @@ __global__ __launch_bounds__(kBlock) void k_init(HbDev d)
     const uint32_t u = (uint32_t)(gtid / G), cl = (uint32_t)(gtid % G);
     if (u >= d.n) return;  // whole copies leave together (G divides the block)
     uint32_t idx, rank;
-    hb_index_rank(u, P, &idx, &rank);
+    hb_index_rank(d.node_id ? d.node_id[u] : u, P, &idx, &rank);
     uint32_t w[4] = {0, 0, 0, 0};
     if (idx / 16 == cl) w[(idx % 16) / 4] = rank << (8 * (idx % 4));
     const uint4 r = make_uint4(w[0], w[1], w[2], w[3]);
@@ -284,6 +284,8 @@ __global__ __launch_bounds__(kBlock) void k_union(HbDev d, const uint32_t *__res
     HbView v;
     v.off = d.off;
     v.nbr = d.nbr;
+    v.orig = d.orig;
+    v.node_id = d.node_id;
     v.in = st->parity ? d.regs1 : d.regs0;
     v.out = st->parity ? d.regs0 : d.regs1;
     v.est = d.est;
@@ -339,6 +341,7 @@ __global__ __launch_bounds__(kBlock) void k_chunk(HbDev d, const uint32_t *__res
         (void)tnode;
         const uint32_t eb = tbeg[task], ee = tend[task];
         const uint32_t *__restrict__ nb = d.nbr + eb;
+        const uint32_t *__restrict__ nbh = d.orig + eb;
         const uint32_t len = ee - eb;
         bool active = true;
         if (use_frontier) {
@@ -350,9 +353,9 @@ __global__ __launch_bounds__(kBlock) void k_chunk(HbDev d, const uint32_t *__res
         if (!active) continue;
         uint32_t gathered = 0;  // by this lane's copy
         uint4 acc = use_frontier
-                        ? hb_gather_union<P, S, true>(in, first, s_first, bitsR, nb, len, slot, cl,
+                        ? hb_gather_union<P, S, true>(in, first, s_first, bitsR, nb, nbh, len, slot, cl,
                                                       make_uint4(0, 0, 0, 0), gathered)
-                        : hb_gather_union<P, S, false>(in, first, s_first, nullptr, nb, len, slot, cl,
+                        : hb_gather_union<P, S, false>(in, first, s_first, nullptr, nb, nbh, len, slot, cl,
                                                        make_uint4(0, 0, 0, 0), gathered);
         if (cl == 0) c_gathered += gathered;
 #pragma unroll
@@ -486,7 +489,7 @@ __global__ __launch_bounds__(kBlock) void k_init_wide(HbDev d)
     const uint32_t u = (uint32_t)(gtid / HB_WAVE), lane = (uint32_t)(gtid % HB_WAVE);
     if (u >= d.n) return;
     uint32_t idx, rank;
-    hb_index_rank(u, P, &idx, &rank);
+    hb_index_rank(d.node_id ? d.node_id[u] : u, P, &idx, &rank);
     const bool isolated = d.off[u + 1] == d.off[u];
     for (uint32_t c = 0; c < M / 1024; ++c) {
         const uint32_t base = c * 1024 + lane * 16;

@@ -15,6 +15,8 @@ namespace hb {
 // one graph's state as seen by one iteration
 struct HbView {
     const uint32_t *off, *nbr;
+    const uint32_t *orig;      // hashed id of each arc's head (iteration 1)
+    const uint32_t *node_id;   // hashed id of each node (null: its index)
     const uint8_t *in;
     bool first;             // iteration 1 with P <= 10: sketches hold only their own insertion
     uint32_t *scr;          // LDS, 16 u32 per lane of the block (iteration 1 only)
@@ -103,7 +105,8 @@ __device__ __forceinline__ void hb_put_max(uint4 &a, uint32_t w, uint32_t cl)
 template <int P, uint32_t S, bool SEL>
 __device__ __forceinline__ uint4 hb_gather_union(const uint8_t *__restrict__ in, bool first, uint32_t *scr,
                                                  const uint32_t *__restrict__ changed,
-                                                 const uint32_t *__restrict__ nb, uint32_t len, uint32_t slot,
+                                                 const uint32_t *__restrict__ nb,
+                                                 const uint32_t *__restrict__ nbh, uint32_t len, uint32_t slot,
                                                  uint32_t cl, uint4 start, uint32_t &gathered)
 {
     constexpr uint32_t G = (1u << P) / 16;
@@ -116,7 +119,7 @@ __device__ __forceinline__ uint4 hb_gather_union(const uint8_t *__restrict__ in,
         for (int k = 0; k < 16; ++k) mine[k] = (sw[k >> 2] >> (8 * (k & 3))) & 0xffu;
         __builtin_amdgcn_wave_barrier();
         for (uint32_t q = slot + cl * S; q < len; q += G * S) {
-            const uint32_t w = hb_insertion<P>(nb[q]);
+            const uint32_t w = hb_insertion<P>(nbh[q]);
             atomicMax(copy + (w >> 6), w & 63u);
         }
         __builtin_amdgcn_wave_barrier();
@@ -183,6 +186,7 @@ __device__ __forceinline__ void union_node(const HbView &v, const HbTables &tb, 
     const uint32_t cl = tl % G, slot = tl / G;
     const uint32_t e0 = v.off[u], deg = v.off[u + 1] - e0;
     const uint32_t *__restrict__ nb = v.nbr + e0;
+    const uint32_t *__restrict__ nbh = v.orig + e0;
     if (v.use_frontier) {
         uint32_t any = 0;
         for (uint32_t j = tl; j < deg; j += T) any |= bit_test(v.bitsR, nb[j]);
@@ -194,10 +198,11 @@ __device__ __forceinline__ void union_node(const HbView &v, const HbTables &tb, 
             return;
         }
     }
-    const uint4 own = v.first ? hb_own_from_word(hb_insertion<P>(u), cl)
+    const uint4 own = v.first ? hb_own_from_word(hb_insertion<P>(v.node_id ? v.node_id[u] : u), cl)
                               : *(const uint4 *)(v.in + ((uint64_t)u << P) + cl * 16);
     uint32_t gathered = 0;  // by this lane's copy
-    uint4 acc = hb_gather_union<P, S, SEL>(v.in, v.first, v.scr, v.gather_bits, nb, deg, slot, cl, own, gathered);
+    uint4 acc = hb_gather_union<P, S, SEL>(v.in, v.first, v.scr, v.gather_bits, nb, nbh, deg, slot, cl, own,
+                                                   gathered);
     if (cl == 0) c.gathered += gathered;
 #pragma unroll
     for (uint32_t s = G; s < T; s <<= 1) acc = hb_bmax4(acc, hb_shfl_xor4<T>(acc, s));

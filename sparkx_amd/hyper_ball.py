@@ -31,7 +31,7 @@ class HyperBall:
 
     def __init__(self, g: Graph, p: int, *, max_iters: int = 0, dense: bool = False,
                  graph_capture: bool = False, timing: bool = False, nf: bool = False,
-                 ktiming: bool = False,
+                 ktiming: bool = False, layout: str = "degree",
                  device: int = 0, schedule: Optional[tuple] = None, _run: bool = True):
         if not 4 <= p <= 16:
             raise ValueError(f"precision p={p} outside 4..16")
@@ -45,6 +45,7 @@ class HyperBall:
         self._lin = None
         self.stats = None
         try:
+            self.set_layout(layout)
             if schedule is not None:
                 _lib.check(self._L.hb_set_schedule(self._ctx, *[int(x) for x in schedule]))
             self._stage()
@@ -67,6 +68,16 @@ class HyperBall:
         return cls(g, cls.PRECISION if p is None else p, **kw)
 
     # ---- engine ---------------------------------------------------------------
+    LAYOUTS = {"input": _lib.HB_LAYOUT_INPUT, "degree": _lib.HB_LAYOUT_DEGREE}
+
+    def set_layout(self, layout: str) -> None:
+        """Node numbering on the device: "degree" (default; nodes by descending degree,
+        built on the GPU at the first run) or "input".  Results are the same either way
+        and always come back in the graph's own node order."""
+        if layout not in self.LAYOUTS:
+            raise ValueError(f"layout {layout!r} not in {sorted(self.LAYOUTS)}")
+        _lib.check(self._L.hb_set_layout(self._ctx, self.LAYOUTS[layout]))
+
     def _stage(self):
         g = self.g
         off, nb = g.offsets, g.neighbours

@@ -538,3 +538,82 @@ def test_results_written_under_reference_cache_names(hb, oracle, tmp_path):
     assert np.array_equal(cache.read_result(str(tmp_path), "inv_distances", "g1"), ref.inv)
     assert np.array_equal(cache.read_result(str(tmp_path), "closeness", "g1", True), c)
     assert np.array_equal(cache.read_result(str(tmp_path), "harmonic", "g1", False), hh)
+
+
+def _layout_graph(hb, name):
+    rng = np.random.default_rng(21)
+    if name == "rmat13":
+        return hb.Graph.rmat(13, 16, seed=5)
+    if name == "kmer":
+        return hb.Graph.kmer_dbg(20000, 4, 0.01, 11, seed=3)
+    n = 4000 if name == "directed" else 1500
+    if name == "directed":  # no reverse arcs: every run takes the scanning path
+        src = rng.integers(0, n, 12000)
+        dst = rng.integers(0, n, 12000)
+    else:  # "multi": symmetric with duplicates, self-loops, isolated nodes and a hub
+        a = rng.integers(0, n // 2, 5000)
+        b = rng.integers(0, n // 2, 5000)
+        hub = np.full(900, 7)
+        leaves = rng.integers(0, n // 2, 900)
+        loops = np.arange(0, n // 2, 5)
+        src = np.concatenate([a, b, hub, leaves, loops, a[:300]])
+        dst = np.concatenate([b, a, leaves, hub, loops, b[:300]])
+    order = np.lexsort((dst, src))
+    src, dst = src[order], dst[order]
+    off = np.zeros(n + 1, np.uint64)
+    np.add.at(off, src + 1, 1)
+    return hb.Graph(np.cumsum(off).astype(np.uint64), dst.astype(np.uint32), name)
+
+
+@pytest.mark.parametrize("name,p", [("rmat13", 4), ("rmat13", 7), ("rmat13", 10), ("rmat13", 12),
+                                    ("directed", 7), ("multi", 6), ("multi", 11), ("kmer", 6)])
+def test_degree_layout_matches_input_layout_and_oracle(hb, oracle, name, p):
+    """The degree-ordered device numbering changes nothing a caller can read: both
+    layouts match the oracle bit for bit, with the same neighbourhood function and
+    centralities, and the layout is built once per staged graph."""
+    g = _layout_graph(hb, name)
+    say("layout", name, p, g.size(), g.width())
+    ref = _oracle_run(oracle, g, p)
+    out = {}
+    # the "multi" hub (degree 918) goes through the chunk / merge kernels
+    sched = (4, 256, 128) if name == "multi" else None
+    for layout in ("input", "degree"):
+        with hb.HyperBall(g, p, layout=layout, nf=True, schedule=sched) as h:
+            st = h.stats
+            assert st.layout == (1 if layout == "degree" else 0)
+            assert (st.ms_layout > 0) == (layout == "degree")
+            _assert_same(h, ref, oracle, p, f"{name} p={p} {layout}")
+            nf = list(st.nf[:st.iterations])
+            cent = [h.compute_closeness_centrality(True), h.compute_harmonic_centrality(False),
+                    h.compute_lins_centrality()]
+            st2 = h.run(nf=True)  # same staged graph: the layout is reused, not rebuilt
+            assert st2.ms_layout == st.ms_layout and st2.iterations == ref.iterations
+            out[layout] = (nf, cent)
+    assert out["input"][0] == out["degree"][0]
+    for a, b in zip(out["input"][1], out["degree"][1]):
+        assert np.array_equal(a, b)
+
+
+def test_layout_switch_and_snapshots(hb, oracle):
+    """Switching layouts on one context discards the run; snapshots come back in the
+    caller's order under both."""
+    from sparkx_amd import _lib
+    g = _layout_graph(hb, "rmat13")
+
+    def snapshots(h):
+        out = np.empty((3, g.size()), np.float32)
+        _lib.check(h._L.hb_read_snapshots(h._ctx, _lib.ptr(out, C.c_float)))
+        return out
+
+    with hb.HyperBall(g, 6, _run=False) as h:
+        _lib.check(h._L.hb_set_snapshots(h._ctx, 3))
+        h.set_layout("input")
+        h.run()
+        snaps_in = snapshots(h)
+        h.set_layout("degree")
+        with pytest.raises(hb.HyperBallError):
+            h.registers(0)
+        h.run()
+        assert np.array_equal(snapshots(h), snaps_in)
+        with pytest.raises(ValueError):
+            h.set_layout("random")
