@@ -11,6 +11,10 @@
 //            where the working set stops fitting the memory-side cache
 //   atomics: one 64-bit atomic add per block (or per wavefront) to the same address,
 //            to size the cost of per-block counters
+//   xcd    : (argument "xcd" runs only this) random 128 B records from a hot table of
+//            8 parts; every block draws from the whole table, or block b only from part
+//            b % 8, or only from part (b / 8) % 8 -- if workgroups go to the 8 XCDs
+//            round-robin, the second keeps each XCD's L2 on one part
 // Each is timed with HIP events, best of 5, over a few grid sizes.
 #include <hip/hip_runtime.h>
 
@@ -62,6 +66,22 @@ __global__ void hbm_gather(const uint4 *__restrict__ table, const uint32_t *__re
     if (acc == 0x9e3779b9u) out[blockIdx.x] = acc;
 }
 
+__global__ void xcd_gather(const uint4 *__restrict__ table, const uint32_t *__restrict__ idx, size_t n,
+                           uint32_t per_part, uint32_t parts, int mode, uint32_t *out)
+{
+    const uint32_t sub = threadIdx.x & 7u;
+    const uint32_t part = mode == 1 ? blockIdx.x % parts : (blockIdx.x / parts) % parts;
+    uint32_t acc = 0;
+    const size_t groups = (size_t)gridDim.x * blockDim.x / 8;
+    for (size_t g = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) / 8; g < n; g += groups) {
+        const uint32_t r0 = idx[g];  // < per_part * parts
+        const uint32_t r = mode == 0 ? r0 : part * per_part + r0 % per_part;
+        const uint4 v = table[(size_t)r * 8 + sub];
+        acc ^= v.x ^ v.y ^ v.z ^ v.w;
+    }
+    if (acc == 0x9e3779b9u) out[blockIdx.x] = acc;
+}
+
 __global__ void atomic_per_block(unsigned long long *p, int per_wave)
 {
     if (per_wave ? (threadIdx.x & 63) == 0 : threadIdx.x == 0) atomicAdd(p, 1ull);
@@ -94,7 +114,8 @@ static float best_ms(hipEvent_t e0, hipEvent_t e1, F launch)
 
 int main(int argc, char **argv)
 {
-    const bool copy_only = argc > 1;  // the counter-calibration runs time the copy kernel alone
+    const bool xcd_only = argc > 1 && argv[1][0] == 'x';
+    const bool copy_only = argc > 1 && !xcd_only;  // the counter-calibration runs time the copy kernel alone
     const size_t copy_bytes = size_t(1) << 30, copy_n = copy_bytes / 16;
     const size_t table_bytes = size_t(4) << 30, table_n = table_bytes / 16;
     const uint32_t records = (uint32_t)(table_bytes / 128);
@@ -108,6 +129,32 @@ int main(int argc, char **argv)
     hipEvent_t e0, e1;
     CHECK(hipEventCreate(&e0));
     CHECK(hipEventCreate(&e1));
+    if (xcd_only) {
+        const size_t n = size_t(1) << 27;
+        CHECK(hipMalloc(&table, size_t(1) << 30));
+        CHECK(hipMalloc(&idx, n * 4));
+        CHECK(hipMemset(table, 3, size_t(1) << 30));
+        for (uint32_t mib : {16u, 32u, 64u, 128u}) {
+            const uint32_t parts = 8, per_part = (mib << 20) / 128 / parts;  // 128 B records, < 1 GiB
+            make_index<<<8192, 256>>>(idx, n, per_part * parts);
+            CHECK(hipGetLastError());
+            for (int mode : {0, 1, 2}) {
+                const float ms = best_ms(e0, e1, [&] {
+                    xcd_gather<<<16384, 256>>>(table, idx, n, per_part, parts, mode, sink);
+                });
+                CHECK(hipGetLastError());
+                printf("xcd: hot table %u MiB, %s: %.3f ms, %.1f G records/s\n", mib,
+                       mode == 0 ? "every block, whole table" : mode == 1 ? "block b, part b % 8" : "block b, part (b/8) % 8",
+                       ms, n / (ms * 1e-3) / 1e9);
+            }
+        }
+        CHECK(hipFree(table));
+        CHECK(hipFree(idx));
+        CHECK(hipFree(a));
+        CHECK(hipFree(b));
+        CHECK(hipFree(sink));
+        return 0;
+    }
     for (int grid : {8192, 32768, 131072}) {
         const float ms = best_ms(e0, e1, [&] { hbm_copy<<<grid, 256>>>(a, b, copy_n); });
         CHECK(hipGetLastError());
