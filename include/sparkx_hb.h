@@ -301,6 +301,55 @@ int hb_eval_ranks(int device, const double *v, uint64_t n, uint32_t *rank);
 int hb_eval_metrics(int device, const double *a, const double *b, uint64_t n, double top_frac,
                     double *out);
 
+/* ---- graph ingestion on the device ---------------------------------------------
+ * hb_stage_mtx parses a MatrixMarket text ("%%MatrixMarket matrix coordinate
+ * pattern|integer symmetric|skew-symmetric|general", square) on the device and stages
+ * the resulting CSR as hb_stage_csr would: entry "i j" (1-based; further tokens
+ * ignored) gives i-1 -> j-1, and j-1 -> i-1 too when the file is not "general" and
+ * i != j; blank and '%' lines are skipped, a line with a single token is skipped,
+ * duplicates are kept, every list is sorted ascending.  A malformed text returns
+ * HB_EINVAL with info->err_code / err_line naming the first bad line.
+ * Replaces: GraphCache::from_mtx_file (src/graph/cache/parser_mtx/mod.rs:34-162,
+ * producers.rs:15-234, impl_sort.rs:10-51) up to the CSR it writes. */
+typedef struct {
+    uint64_t n, nnz;      /* size line                                              */
+    uint64_t entries;     /* entry lines used                                       */
+    uint64_t arcs;        /* arcs staged                                            */
+    uint64_t err_line;    /* 1-based line of the first rejected line, 0 if none     */
+    int32_t symmetric;    /* entries emit both ways (symmetric, skew-symmetric)     */
+    int32_t err_code;     /* HB_MTX_*                                               */
+    double ms_device;     /* device time of the line split, parse, sort and CSR     */
+} hb_mtx_info;
+
+#define HB_MTX_OK 0
+#define HB_MTX_BAD_HEADER 1        /* first line does not start with %%MatrixMarket */
+#define HB_MTX_MALFORMED_HEADER 2  /* fewer than 5 header tokens                    */
+#define HB_MTX_UNSUPPORTED 3       /* not matrix coordinate pattern|integer ...     */
+#define HB_MTX_NO_SIZE_LINE 4      /* text ends before the size line                */
+#define HB_MTX_BAD_SIZE_LINE 5     /* size line lacks a number                      */
+#define HB_MTX_NOT_SQUARE 6        /* rows != columns                               */
+#define HB_MTX_BAD_ENTRY 7         /* an index is not a non-negative integer        */
+#define HB_MTX_ZERO_INDEX 8        /* index 0 (MatrixMarket is 1-based)             */
+#define HB_MTX_OUT_OF_BOUNDS 9     /* index above n                                 */
+
+int hb_stage_mtx(hb_ctx *ctx, const char *text, uint64_t len, hb_mtx_info *info);
+
+/* Builds and stages a CSR from an edge list on the device: arc src[k] -> dst[k] for
+ * k < m (host or device arrays, ids < n), plus dst[k] -> src[k] when
+ * HB_COO_SYMMETRIZE and src[k] != dst[k]; HB_COO_NO_LOOPS drops u -> u, HB_COO_DEDUP
+ * keeps one copy of repeated arcs; lists sorted ascending.  No reference counterpart
+ * (the synthetic generators' symmetrise / deduplicate step, on the device). */
+#define HB_COO_SYMMETRIZE 0x1u
+#define HB_COO_DEDUP 0x2u
+#define HB_COO_NO_LOOPS 0x4u
+int hb_stage_coo(hb_ctx *ctx, uint64_t n, const uint32_t *src, const uint32_t *dst, uint64_t m,
+                 uint32_t flags);
+
+/* Copies the staged CSR, in the caller's numbering, as 8-byte offsets[n+1] and
+ * neighbours[arcs] (either may be NULL) -- the layout of the reference's index_<id> and
+ * edges_<id> cache files (src/graph/cache/utils.rs). */
+int hb_read_csr(hb_ctx *ctx, uint64_t *offsets, uint64_t *nbrs);
+
 /* Number of nodes / arcs staged (0 if none). */
 uint64_t hb_num_nodes(const hb_ctx *ctx);
 uint64_t hb_num_arcs(const hb_ctx *ctx);

@@ -12,6 +12,8 @@
  *   src/centralities/hyper_ball.rs:434-555   closeness / harmonic / Lin epilogues
  *   src/graph/cache/parser_mtx/producers.rs:228-234, impl_sort.rs:10-51
  *                                            MTX symmetric emission and per-node sort
+ *   src/graph/cache/parser_mtx/{mod.rs,producers.rs}
+ *                                            MatrixMarket text -> CSR (mtx_oracle.c)
  *   src/trails/bfs.rs:199-234, src/test_common.rs:195-249
  *                                            exact closeness (source not pre-marked)
  *   src/utils.rs:57-100                      mae / mape / ranks / spearman_rho
@@ -134,6 +136,36 @@ void hbo_ranks(const double *v, uint64_t n, uint64_t *rank_out);
  * emission (self-loop once, duplicates kept), per-node ascending sort.
  * Returns arcs; offsets must hold n+1, nbrs may be NULL to query the size. */
 uint64_t hbo_dbg_mtx_csr(int sigma, int k, uint64_t *offsets, uint32_t *nbrs);
+
+/* ---- MatrixMarket ingestion (mtx_oracle.c) ------------------------------ */
+typedef struct {
+    uint64_t n, nnz;      /* size line: rows (= columns), declared entries              */
+    uint64_t body;        /* byte offset of the first line after the size line          */
+    uint64_t body_line;   /* its 1-based line number                                    */
+    uint64_t entries;     /* entry lines used                                           */
+    uint64_t arcs;        /* arcs emitted                                               */
+    uint64_t err_line;    /* 1-based line of the first error                            */
+    int symmetric;        /* "symmetric" or "skew-symmetric": entries emit both ways    */
+} hbo_mtx_info;
+
+enum {
+    HBO_MTX_BAD_HEADER = -1,       /* first line does not start with %%MatrixMarket      */
+    HBO_MTX_MALFORMED_HEADER = -2, /* fewer than 5 header tokens                         */
+    HBO_MTX_UNSUPPORTED = -3,      /* not matrix coordinate pattern|integer sym|skew|gen */
+    HBO_MTX_NO_SIZE_LINE = -4,     /* end of text before the size line                   */
+    HBO_MTX_BAD_SIZE_LINE = -5,    /* size line missing a number or not a number         */
+    HBO_MTX_NOT_SQUARE = -6,       /* nrows != ncols                                     */
+    HBO_MTX_BAD_ENTRY = -7,        /* an index token is not a usize                      */
+    HBO_MTX_ZERO_INDEX = -8,       /* 0 index (MatrixMarket is 1-based)                  */
+    HBO_MTX_OUT_OF_BOUNDS = -9,    /* index > n                                          */
+    HBO_MTX_NOMEM = -10,
+};
+
+/* Header only (fills n, nnz, body, body_line, symmetric). */
+int hbo_mtx_header(const char *text, uint64_t len, hbo_mtx_info *info);
+/* Whole text -> CSR: offsets[n+1] (may be NULL to get sizes), nbrs[arcs] (may be NULL).
+ * Returns 0 or an HBO_MTX_* code with info->err_line set. */
+int hbo_parse_mtx(const char *text, uint64_t len, hbo_mtx_info *info, uint64_t *offsets, uint32_t *nbrs);
 
 #ifdef __cplusplus
 }

@@ -59,6 +59,24 @@ class Stats(C.Structure):
 ITER_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.POINTER(C.c_uint8),
                       C.POINTER(C.c_double), C.POINTER(C.c_double))
 
+class MtxInfo(C.Structure):
+    _fields_ = [("n", C.c_uint64), ("nnz", C.c_uint64), ("body", C.c_uint64), ("body_line", C.c_uint64),
+                ("entries", C.c_uint64), ("arcs", C.c_uint64), ("err_line", C.c_uint64),
+                ("symmetric", C.c_int)]
+
+
+MTX_ERRORS = {-1: "bad header", -2: "malformed header", -3: "unsupported format", -4: "no size line",
+              -5: "bad size line", -6: "not square", -7: "bad entry", -8: "zero index",
+              -9: "out of bounds", -10: "out of memory"}
+
+
+class MtxError(ValueError):
+    def __init__(self, code: int, line: int):
+        super().__init__(f"{MTX_ERRORS.get(code, code)} (line {line})")
+        self.code = code
+        self.line = line
+
+
 _lib = None
 
 
@@ -108,6 +126,10 @@ def lib():
         L.hbo_ranks.argtypes = [f64p, C.c_uint64, u64p]
         L.hbo_dbg_mtx_csr.restype = C.c_uint64
         L.hbo_dbg_mtx_csr.argtypes = [C.c_int, C.c_int, u64p, u32p]
+        L.hbo_mtx_header.restype = C.c_int
+        L.hbo_mtx_header.argtypes = [C.c_char_p, C.c_uint64, C.POINTER(MtxInfo)]
+        L.hbo_parse_mtx.restype = C.c_int
+        L.hbo_parse_mtx.argtypes = [C.c_char_p, C.c_uint64, C.POINTER(MtxInfo), u64p, u32p]
         _lib = L
     return _lib
 
@@ -326,3 +348,18 @@ def dbg_mtx_csr(sigma: int, k: int):
     nb = np.zeros(arcs, np.uint32)
     lib().hbo_dbg_mtx_csr(sigma, k, _p(off, C.c_uint64), _p(nb, C.c_uint32))
     return off, nb
+
+
+def parse_mtx(text: bytes):
+    """MatrixMarket text -> (offsets u64[n+1], neighbours u32[arcs], MtxInfo), following
+    parser_mtx/{mod.rs,producers.rs} and impl_sort.rs; raises MtxError."""
+    info = MtxInfo()
+    rc = lib().hbo_parse_mtx(text, len(text), C.byref(info), None, None)
+    if rc:
+        raise MtxError(rc, info.err_line)
+    off = np.zeros(info.n + 1, np.uint64)
+    nb = np.zeros(max(info.arcs, 1), np.uint32)
+    rc = lib().hbo_parse_mtx(text, len(text), C.byref(info), _p(off, C.c_uint64), _p(nb, C.c_uint32))
+    if rc:
+        raise MtxError(rc, info.err_line)
+    return off, nb[:info.arcs], info

@@ -32,7 +32,17 @@ EXPORTS = (
     "hb_set_snapshots", "hb_read_snapshots", "hb_mlp_create", "hb_mlp_destroy", "hb_mlp_set_params",
     "hb_mlp_get_params", "hb_mlp_gradient", "hb_mlp_train", "hb_mlp_predict", "hb_surrogate",
     "hb_exact_bfs", "hb_exact_centrality", "hb_eval_ranks", "hb_eval_metrics", "hb_set_layout",
+    "hb_stage_mtx", "hb_stage_coo", "hb_read_csr",
 )
+
+
+class HbMtxInfo(C.Structure):
+    _fields_ = [("n", C.c_uint64), ("nnz", C.c_uint64), ("entries", C.c_uint64), ("arcs", C.c_uint64),
+                ("err_line", C.c_uint64), ("symmetric", C.c_int32), ("err_code", C.c_int32),
+                ("ms_device", C.c_double)]
+
+
+HB_COO_SYMMETRIZE, HB_COO_DEDUP, HB_COO_NO_LOOPS = 0x1, 0x2, 0x4
 
 
 class HbGraphDesc(C.Structure):
@@ -111,6 +121,7 @@ def load() -> C.CDLL:
     L = C.CDLL(LIB_PATH)
     vp, u8p, f32p, f64p = C.c_void_p, C.POINTER(C.c_uint8), C.POINTER(C.c_float), C.POINTER(C.c_double)
     u32p = C.POINTER(C.c_uint32)
+    u64p = C.POINTER(C.c_uint64)
     sig = {
         "hb_create": (C.c_int, [C.POINTER(vp), C.c_int]),
         "hb_destroy": (None, [vp]),
@@ -162,6 +173,9 @@ def load() -> C.CDLL:
         "hb_eval_ranks": (C.c_int, [C.c_int, f64p, C.c_uint64, u32p]),
         "hb_eval_metrics": (C.c_int, [C.c_int, f64p, f64p, C.c_uint64, C.c_double, f64p]),
         "hb_set_layout": (C.c_int, [vp, C.c_int]),
+        "hb_stage_mtx": (C.c_int, [vp, C.c_char_p, C.c_uint64, C.POINTER(HbMtxInfo)]),
+        "hb_stage_coo": (C.c_int, [vp, C.c_uint64, u32p, u32p, C.c_uint64, C.c_uint32]),
+        "hb_read_csr": (C.c_int, [vp, u64p, u64p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

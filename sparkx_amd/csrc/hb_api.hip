@@ -70,6 +70,12 @@ struct DevBuf {
         ptr = nullptr;
         count = 0;
     }
+    void adopt(T *p, size_t n)  // takes a hipMalloc'd buffer
+    {
+        release();
+        ptr = p;
+        count = n;
+    }
 };
 
 }  // namespace
@@ -84,7 +90,10 @@ struct hb_ctx {
     std::vector<uint32_t> h_off;  // host copy of the offsets, for scheduling
 
     // schedule knobs
-    uint32_t small_max = 16, mid_max = 128, wave_max = 16384, chunk = 8192;  // measured best on R-MAT-24 p=7
+    // degree bounds of the bins; until hb_set_schedule is called they follow p
+    // (default_schedule)
+    uint32_t small_max = 256, mid_max = 2048, wave_max = 4096, chunk = 8192;
+    bool sched_user = false;
 
     // run state
     int p = 0;
@@ -168,6 +177,35 @@ static bool set_device(hb_ctx *c) { return hipSetDevice(c->device) == hipSuccess
 
 static int ensure_layout(hb_ctx *c);
 
+// The CSR is in c->off / c->nbr: keeps a host copy of the offsets (h_off, or read
+// back when null), tests symmetry with the arc-multiset fingerprints (symmetric graphs
+// may use push iterations: a changed node marks its neighbours) and builds the
+// degree layout.
+static int finish_stage(hb_ctx *c, uint64_t n, uint64_t arcs, std::vector<uint32_t> *h_off)
+{
+    c->sym = true;
+    if (n) {
+        unsigned long long *acc = nullptr, h_acc[4] = {};
+        HB_TRY(hipMalloc(&acc, (4 + 4 * hb::kFingerprintBlocks) * sizeof(unsigned long long)));
+        hipError_t e = hb::arc_fingerprints(c->off.ptr, c->nbr.ptr, (uint32_t)n, acc, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(h_acc, acc, sizeof h_acc, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        (void)hipFree(acc);
+        if (e != hipSuccess) return fail(HB_EHIP, "staging: symmetry check failed: %s", hipGetErrorString(e));
+        c->sym = h_acc[0] == h_acc[1] && h_acc[2] == h_acc[3];
+    }
+    std::vector<uint32_t> local;
+    if (!h_off) {
+        local.resize(n + 1);
+        HB_TRY(hipMemcpy(local.data(), c->off.ptr, (n + 1) * 4, hipMemcpyDeviceToHost));
+        h_off = &local;
+    }
+    c->h_off.swap(*h_off);
+    c->n = n;
+    c->arcs = arcs;
+    return ensure_layout(c);  // degree layout: built with the staging
+}
+
 extern "C" {
 
 const char *hb_last_error(void) { return g_err.c_str(); }
@@ -212,6 +250,7 @@ uint64_t hb_num_arcs(const hb_ctx *c) { return c ? c->arcs : 0; }
 int hb_set_schedule(hb_ctx *c, uint32_t small_max_deg, uint32_t wave_max_deg, uint32_t chunk)
 {
     if (!c) return fail(HB_EINVAL, "hb_set_schedule: ctx is NULL");
+    c->sched_user = true;
     if (small_max_deg) c->small_max = small_max_deg;
     if (wave_max_deg) c->wave_max = wave_max_deg;
     if (chunk) c->chunk = chunk;
@@ -326,27 +365,26 @@ int hb_stage_csr(hb_ctx *c, const void *offsets, int off_bytes, uint64_t n, cons
         if (done[i]) (void)hipEventDestroy(done[i]);
     }
     if (rc != HB_OK) return rc;
-    // symmetric graphs may use push iterations (a changed node marks its neighbours)
-    c->sym = true;
-    if (n) {
-        unsigned long long *acc = nullptr, h_acc[4] = {};
-        HB_TRY(hipMalloc(&acc, (4 + 4 * hb::kFingerprintBlocks) * sizeof(unsigned long long)));
-        hipError_t e = hb::arc_fingerprints(c->off.ptr, c->nbr.ptr, (uint32_t)n, acc, c->stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(h_acc, acc, sizeof h_acc, hipMemcpyDeviceToHost, c->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-        (void)hipFree(acc);
-        if (e != hipSuccess) return fail(HB_EHIP, "hb_stage_csr: symmetry check failed: %s", hipGetErrorString(e));
-        c->sym = h_acc[0] == h_acc[1] && h_acc[2] == h_acc[3];
-    }
-    c->h_off.swap(h_off);
-    c->n = n;
-    c->arcs = arcs;
-    if (int rc2 = ensure_layout(c)) return rc2;  // degree layout: built with the staging
+    if (int rc2 = finish_stage(c, n, arcs, &h_off)) return rc2;
     g_err.clear();
     return HB_OK;
 }
 
 }  // extern "C"
+
+// Default bin bounds, measured with the degree layout on R-MAT-20 / R-MAT-24 (sweeps of
+// small_max 16..512 and wave_max 4096 / 16384): at p >= 6 a node of degree <= 256 is
+// cheapest on one sketch copy (2^p/16 lanes), at p <= 5 (one or two lanes per copy)
+// the first iteration's hashing wants the wider teams from degree 64 on.
+// mid_max = min(wave_max, 8 small_max).
+static void default_schedule(hb_ctx *c, int p)
+{
+    if (c->sched_user) return;
+    c->small_max = p <= 5 ? 64 : 256;
+    c->wave_max = p <= 5 ? 16384 : 4096;
+    c->chunk = 8192;
+    c->mid_max = std::max(c->small_max, std::min<uint32_t>(c->wave_max, 8 * c->small_max));
+}
 
 // Degree-binned schedule: bins of team width G (deg <= small_max), 4G (<= mid_max)
 // and 64 (<= wave_max); higher-degree nodes are split into chunk-sized wavefront
@@ -533,6 +571,7 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
     if (push_ok) HB_TRY(c->active.ensure(words));
     {
         int rc = ensure_layout(c);  // normally built at staging already
+        default_schedule(c, p);
         if (rc == HB_OK) rc = build_schedule(c, p);
         if (rc != HB_OK) return rc;
     }
@@ -931,8 +970,9 @@ extern "C" int hb_autotune(hb_ctx *c, int p, int trial_iters, uint32_t *small_ma
         }
         return HB_OK;
     };
+    default_schedule(c, p);  // the search starts from this precision's defaults
     uint32_t knob[3] = {c->small_max, c->wave_max, c->chunk};
-    const uint32_t cand[3][4] = {{2, 4, 8, 16}, {512, 1024, 4096, 16384}, {512, 1024, 2048, 8192}};
+    const uint32_t cand[3][4] = {{16, 64, 256, 512}, {1024, 4096, 16384, 65536}, {2048, 4096, 8192, 16384}};
     double best = 0.0;
     int rc = trial(knob[0], knob[1], knob[2], &best);
     if (rc != HB_OK) return rc;
@@ -1017,6 +1057,160 @@ extern "C" int hb_surrogate(hb_ctx *c, hb_mlp *mlp, int p_feat, float *pred, flo
         if (int rc2 = copy_out(c, c->pred.ptr, pred, 4)) return rc2;
     if (features)
         if (int rc2 = copy_out(c, c->feat.ptr, features, 24)) return rc2;
+    g_err.clear();
+    return HB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// graph ingestion on the device (hb_ingest.hip)
+// ---------------------------------------------------------------------------
+static const char *mtx_reason(int code)
+{
+    switch (code) {
+    case HB_MTX_BAD_HEADER: return "first line does not start with %%MatrixMarket";
+    case HB_MTX_MALFORMED_HEADER: return "header has fewer than 5 tokens";
+    case HB_MTX_UNSUPPORTED: return "only 'matrix coordinate pattern|integer symmetric|skew-symmetric|general'";
+    case HB_MTX_NO_SIZE_LINE: return "no size line";
+    case HB_MTX_BAD_SIZE_LINE: return "size line needs nrows ncols nnz";
+    case HB_MTX_NOT_SQUARE: return "nrows != ncols";
+    case HB_MTX_BAD_ENTRY: return "index is not a non-negative integer";
+    case HB_MTX_ZERO_INDEX: return "index 0 (MatrixMarket indices are 1-based)";
+    case HB_MTX_OUT_OF_BOUNDS: return "index above n";
+    default: return "error";
+    }
+}
+
+// adopts a device-built CSR and finishes staging
+static int stage_built(hb_ctx *c, uint64_t n, hb::CsrBuild &b)
+{
+    c->off.adopt(b.off, n + 1);
+    c->nbr.adopt(b.nbr, std::max<uint64_t>(b.arcs, 1));
+    b.off = b.nbr = nullptr;
+    return finish_stage(c, n, b.arcs, nullptr);
+}
+
+static void free_build(hb::CsrBuild &b)
+{
+    if (b.off) (void)hipFree(b.off);
+    if (b.nbr) (void)hipFree(b.nbr);
+    b.off = b.nbr = nullptr;
+}
+
+extern "C" int hb_stage_mtx(hb_ctx *c, const char *text, uint64_t len, hb_mtx_info *info)
+{
+    hb_mtx_info local{};
+    if (!info) info = &local;
+    memset(info, 0, sizeof *info);
+    if (!c || (!text && len)) return fail(HB_EINVAL, "hb_stage_mtx: NULL argument");
+    hb::ingest::MtxHeader h;
+    const int code = hb::ingest::mtx_header(text ? text : "", len, &h);
+    info->n = h.n;
+    info->nnz = h.nnz;
+    info->symmetric = h.symmetric;
+    if (code) {
+        info->err_code = code;
+        info->err_line = code <= HB_MTX_NO_SIZE_LINE ? (code == HB_MTX_NO_SIZE_LINE ? 0 : 1) : h.err_line;
+        return fail(HB_EINVAL, "hb_stage_mtx: line %llu: %s", (unsigned long long)info->err_line, mtx_reason(code));
+    }
+    if (h.n >= (1ull << 32) - 1)
+        return fail(HB_ERANGE, "hb_stage_mtx: n=%llu exceeds the u32 device layout", (unsigned long long)h.n);
+    if (!set_device(c)) return fail(HB_EHIP, "hipSetDevice failed");
+    (void)hipStreamSynchronize(c->stream);
+    c->release_run();
+    c->release_layout();
+    c->n = c->arcs = 0;
+    const uint64_t body = len - h.body;
+    uint8_t *dtext = nullptr;
+    HB_TRY(hipMalloc(&dtext, std::max<uint64_t>(body, 1)));
+    hipError_t e = body ? hipMemcpyAsync(dtext, text + h.body, body, hipMemcpyDefault, c->stream) : hipSuccess;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+    hb::CsrBuild b{};
+    if (e == hipSuccess) e = hb::ingest::mtx_body_to_csr(dtext, body, (uint32_t)h.n, h.symmetric != 0, &b, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) (void)hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(dtext);
+    info->ms_device = ms;
+    if (e != hipSuccess) {
+        free_build(b);
+        if (b.emitted >= (1ull << 32))
+            return fail(HB_ERANGE, "hb_stage_mtx: %llu arcs exceed the u32 device layout",
+                        (unsigned long long)b.emitted);
+        return fail(e == hipErrorOutOfMemory ? HB_ENOMEM : HB_EHIP, "hb_stage_mtx: %s", hipGetErrorString(e));
+    }
+    if (b.err_code) {
+        free_build(b);
+        info->err_code = b.err_code;
+        info->err_line = h.body_line + b.err_index;
+        return fail(HB_EINVAL, "hb_stage_mtx: line %llu: %s", (unsigned long long)info->err_line,
+                    mtx_reason(b.err_code));
+    }
+    info->entries = b.entries;
+    info->arcs = b.arcs;
+    if (int rc = stage_built(c, h.n, b)) return rc;
+    g_err.clear();
+    return HB_OK;
+}
+
+extern "C" int hb_stage_coo(hb_ctx *c, uint64_t n, const uint32_t *src, const uint32_t *dst, uint64_t m,
+                            uint32_t flags)
+{
+    if (!c || (m && (!src || !dst))) return fail(HB_EINVAL, "hb_stage_coo: NULL argument");
+    if (flags & ~(HB_COO_SYMMETRIZE | HB_COO_DEDUP | HB_COO_NO_LOOPS))
+        return fail(HB_EINVAL, "hb_stage_coo: unknown flags 0x%x", flags);
+    if (n >= (1ull << 32) - 1 || m >= (1ull << 32))
+        return fail(HB_ERANGE, "hb_stage_coo: n=%llu m=%llu exceed the u32 device layout", (unsigned long long)n,
+                    (unsigned long long)m);
+    if (!set_device(c)) return fail(HB_EHIP, "hipSetDevice failed");
+    (void)hipStreamSynchronize(c->stream);
+    c->release_run();
+    c->release_layout();
+    c->n = c->arcs = 0;
+    uint32_t *ds = nullptr, *dd = nullptr;
+    hipError_t e = hipMalloc(&ds, std::max<uint64_t>(m, 1) * 4);
+    if (e == hipSuccess) e = hipMalloc(&dd, std::max<uint64_t>(m, 1) * 4);
+    if (e == hipSuccess && m) e = hipMemcpyAsync(ds, src, m * 4, hipMemcpyDefault, c->stream);
+    if (e == hipSuccess && m) e = hipMemcpyAsync(dd, dst, m * 4, hipMemcpyDefault, c->stream);
+    hb::CsrBuild b{};
+    if (e == hipSuccess) e = hb::ingest::coo_to_csr(ds, dd, m, (uint32_t)n, flags, &b, c->stream);
+    (void)hipStreamSynchronize(c->stream);
+    if (ds) (void)hipFree(ds);
+    if (dd) (void)hipFree(dd);
+    if (e != hipSuccess) {
+        free_build(b);
+        if (b.emitted >= (1ull << 32))
+            return fail(HB_ERANGE, "hb_stage_coo: %llu arcs exceed the u32 device layout",
+                        (unsigned long long)b.emitted);
+        return fail(e == hipErrorOutOfMemory ? HB_ENOMEM : HB_EHIP, "hb_stage_coo: %s", hipGetErrorString(e));
+    }
+    if (b.err_code) {
+        free_build(b);
+        return fail(HB_EINVAL, "hb_stage_coo: entry %llu has an id >= n=%llu", (unsigned long long)b.err_index,
+                    (unsigned long long)n);
+    }
+    if (int rc = stage_built(c, n, b)) return rc;
+    g_err.clear();
+    return HB_OK;
+}
+
+extern "C" int hb_read_csr(hb_ctx *c, uint64_t *offsets, uint64_t *nbrs)
+{
+    if (!c) return fail(HB_EINVAL, "hb_read_csr: ctx is NULL");
+    if (!c->off.ptr) return fail(HB_ESTATE, "hb_read_csr: no graph staged");
+    if (offsets)
+        for (uint64_t i = 0; i <= c->n; ++i) offsets[i] = c->h_off[i];
+    if (nbrs && c->arcs) {
+        if (!set_device(c)) return fail(HB_EHIP, "hipSetDevice failed");
+        std::vector<uint32_t> tmp(c->arcs);
+        HB_TRY(hipMemcpy(tmp.data(), c->nbr.ptr, c->arcs * 4, hipMemcpyDeviceToHost));
+        for (uint64_t i = 0; i < c->arcs; ++i) nbrs[i] = tmp[i];
+    }
     g_err.clear();
     return HB_OK;
 }

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include "hb_device.h"
+#include "../../include/sparkx_hb.h"
 
 struct hb_mlp;
 
@@ -51,6 +52,28 @@ hipError_t relabel_by_degree(const uint32_t *off, const uint32_t *nbr, uint32_t 
 // dst row u (caller's order) = src row sigma[u]; row_bytes a multiple of 4
 hipError_t unpermute_rows(const void *src, void *dst, const uint32_t *sigma, uint32_t n, size_t row_bytes,
                           hipStream_t s);
+
+// Device CSR construction (hb_ingest.hip).  off / nbr are allocated with hipMalloc
+// (the caller takes them); err_code != 0 marks a rejected input at err_index (an MTX
+// body line, 0-based, or a COO entry): 7 not a number, 8 zero index, 9 index >= n.
+struct CsrBuild {
+    uint32_t *off, *nbr;
+    uint64_t arcs, emitted, entries;
+    uint64_t err_index;
+    int err_code;
+};
+namespace ingest {
+struct MtxHeader {
+    uint64_t n, nnz, body, body_line, err_line;  // body: offset of the line after the size line
+    int symmetric;
+};
+// HB_MTX_* of the header / size line (0 if both parse)
+int mtx_header(const char *text, uint64_t len, MtxHeader *h);
+hipError_t mtx_body_to_csr(const uint8_t *text, uint64_t len, uint32_t n, bool symmetric, CsrBuild *out,
+                           hipStream_t s);
+hipError_t coo_to_csr(const uint32_t *src, const uint32_t *dst, uint64_t m, uint32_t n, uint32_t flags,
+                      CsrBuild *out, hipStream_t s);
+}  // namespace ingest
 
 // surrogate tail on a staged graph: features from snapshots, MLP inference
 hipError_t launch_surrogate_tail(struct ::hb_mlp *n, const uint32_t *off, const float *snaps, uint32_t nodes,

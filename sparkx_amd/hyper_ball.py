@@ -32,7 +32,7 @@ class HyperBall:
     def __init__(self, g: Graph, p: int, *, max_iters: int = 0, dense: bool = False,
                  graph_capture: bool = False, timing: bool = False, nf: bool = False,
                  ktiming: bool = False, layout: str = "degree",
-                 device: int = 0, schedule: Optional[tuple] = None, _run: bool = True):
+                 device: int = 0, schedule: Optional[tuple] = None, _run: bool = True, _stage_fn=None):
         if not 4 <= p <= 16:
             raise ValueError(f"precision p={p} outside 4..16")
         self.g = g
@@ -48,7 +48,10 @@ class HyperBall:
             self.set_layout(layout)
             if schedule is not None:
                 _lib.check(self._L.hb_set_schedule(self._ctx, *[int(x) for x in schedule]))
-            self._stage()
+            if _stage_fn is None:
+                self._stage()
+            else:  # staged on the device (MatrixMarket text, edge list); keep the host CSR
+                self.g, self.ingest_info = _stage_fn(self._L, self._ctx)
             if _run:
                 self.run(max_iters=max_iters, dense=dense, graph_capture=graph_capture, timing=timing,
                          nf=nf, ktiming=ktiming)
@@ -66,6 +69,14 @@ class HyperBall:
                       p: Optional[int] = None, **kw) -> "HyperBall":
         del max_depth, threads  # no effect on results in the reference either
         return cls(g, cls.PRECISION if p is None else p, **kw)
+
+    @classmethod
+    def from_mtx(cls, text, p: Optional[int] = None, **kw) -> "HyperBall":
+        """Parses a MatrixMarket text (bytes, or a path) on the GPU, stages the CSR it
+        describes and runs, as GraphCache::from_mtx_file followed by new() would
+        (parser_mtx/mod.rs:34-162).  `self.ingest_info` holds the parse summary."""
+        from .ingest import stage_mtx_fn
+        return cls(None, cls.PRECISION if p is None else p, _stage_fn=stage_mtx_fn(text), **kw)
 
     # ---- engine ---------------------------------------------------------------
     LAYOUTS = {"input": _lib.HB_LAYOUT_INPUT, "degree": _lib.HB_LAYOUT_DEGREE}
