@@ -190,22 +190,69 @@ __device__ void forward_tile(const Shared &sh, Wave &w, const Dev &d, uint64_t b
     __syncthreads();
 }
 
-__global__ __launch_bounds__(kBlock) void k_predict(Dev d)
+// Inference only, activations kept in registers.  Both layers are computed
+// transposed, z^T = W^T x^T, so the output of one MFMA (D[n][s]: lane l holds hidden
+// units 4 (l / 16) + j of sample l % 16) is exactly the B operand the next layer needs
+// (B[k][s], same lane mapping); no LDS round trip.  Layer 3 is a dot product reduced
+// over the four lane groups of each sample.  Same arithmetic as the training forward
+// pass (bf16 operands, f32 accumulation, bf16-rounded activations); the layer-3 sum
+// runs in another order.
+constexpr int kInferBlock = 256;
+
+__global__ __launch_bounds__(kInferBlock) void k_infer(Dev d)
 {
-    __shared__ Shared sh;
-    __shared__ Wave wv[kWaves];
-    load_shared(sh, d);
-    __syncthreads();
-    const int l = threadIdx.x & 63, wi = threadIdx.x / 64;
-    Wave &w = wv[wi];
+    const int l = threadIdx.x & 63, g = l >> 4, sl = l & 15;
+    // constant operands: A1[mb] = W1^T rows 16 mb .. (input dim padded to 16),
+    // A2[kc][nb] = W2^T, per-lane biases / output weights of the units it holds
+    v4s a1[4], a2[4][4];
+    float b1[4][4], b2[4][4], w3[4][4];
+    for (int mb = 0; mb < 4; ++mb) {
+        for (int j = 0; j < 4; ++j) {
+            const int k = 4 * g + j;  // input feature
+            a1[mb][j] = (short)(k < kDIn ? f2bf(d.params[oW1 + k * kDH + 16 * mb + sl]) : 0);
+            const int n = 16 * mb + 4 * g + j;
+            b1[mb][j] = d.params[oB1 + n];
+            b2[mb][j] = d.params[oB2 + n];
+            w3[mb][j] = bfr(d.params[oW3 + n]);
+        }
+        for (int nb = 0; nb < 4; ++nb)
+            for (int j = 0; j < 4; ++j)
+                a2[mb][nb][j] = (short)f2bf(d.params[oW2 + (16 * mb + 4 * g + j) * kDH + 16 * nb + sl]);
+    }
+    float mean[4], sd[4];
+    for (int j = 0; j < 4; ++j) {
+        const int k = 4 * g + j;
+        mean[j] = k < kDIn ? d.norm[k] : 0.0f;
+        sd[j] = k < kDIn ? d.norm[kDIn + k] : 1.0f;
+    }
+    const float b3 = d.params[oB3];
     const uint64_t tiles = (d.count + 15) / 16;
-    const uint64_t rounds = (tiles + (uint64_t)gridDim.x * kWaves - 1) / ((uint64_t)gridDim.x * kWaves);
-    for (uint64_t r = 0; r < rounds; ++r) {  // uniform trip count: barriers inside
-        const uint64_t tile = (r * gridDim.x + blockIdx.x) * kWaves + wi;
-        const uint64_t base = d.first + tile * 16;
-        const int valid = tile < tiles ? (int)min<uint64_t>(16, d.first + d.count - base) : 0;
-        forward_tile(sh, w, d, base, valid, l, false);
-        if (l < valid) d.out[base + l] = w.y[l];
+    const uint64_t waves = (uint64_t)gridDim.x * (kInferBlock / 64);
+    for (uint64_t tile = (uint64_t)blockIdx.x * (kInferBlock / 64) + threadIdx.x / 64; tile < tiles;
+         tile += waves) {
+        const uint64_t row = d.first + tile * 16 + sl;
+        const bool valid = row < d.first + d.count;
+        v4s xb;  // B1[k][s] = x[s][k]
+        for (int j = 0; j < 4; ++j) {
+            const int k = 4 * g + j;
+            float v = 0.0f;
+            if (valid && k < kDIn) v = bfr(__fdiv_rn(__fsub_rn(d.x[row * kDIn + k], mean[j]), sd[j]));
+            xb[j] = (short)f2bf(v);
+        }
+        v4s h1[4];
+        for (int mb = 0; mb < 4; ++mb) {
+            const v4f z = mfma(a1[mb], xb, v4f{0.0f, 0.0f, 0.0f, 0.0f});
+            for (int j = 0; j < 4; ++j) h1[mb][j] = (short)f2bf(gelu(z[j] + b1[mb][j]));
+        }
+        float part = 0.0f;
+        for (int nb = 0; nb < 4; ++nb) {
+            v4f z = {0.0f, 0.0f, 0.0f, 0.0f};
+            for (int kc = 0; kc < 4; ++kc) z = mfma(a2[kc][nb], h1[kc], z);
+            for (int j = 0; j < 4; ++j) part += bfr(gelu(z[j] + b2[nb][j])) * w3[nb][j];
+        }
+        part += __shfl_xor(part, 16);
+        part += __shfl_xor(part, 32);
+        if (g == 0 && valid) d.out[row] = part + b3;
     }
 }
 
@@ -431,6 +478,12 @@ struct hb_mlp {
     }
 };
 
+static unsigned infer_grid(uint64_t count)
+{
+    const uint64_t tiles = (count + 15) / 16, blocks = (tiles + 3) / 4;
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(blocks, 8192));
+}
+
 static unsigned mlp_grid(uint64_t count, unsigned cap)
 {
     const uint64_t tiles = (count + 15) / 16, blocks = (tiles + hb::mlp::kWaves - 1) / hb::mlp::kWaves;
@@ -586,7 +639,7 @@ int hb_mlp_predict(hb_mlp *n, const float *x, uint64_t count, float *y)
     HB_MTRY(n->ensure(count));
     HB_MTRY(hipMemcpy(n->x, x, count * 24, hipMemcpyHostToDevice));
     hb::mlp::Dev d{n->w, n->norm, n->x, nullptr, nullptr, 0, count, 0.0f, nullptr, nullptr, n->out};
-    hb::mlp::k_predict<<<mlp_grid(count, 8192), hb::mlp::kBlock, 0, n->stream>>>(d);
+    hb::mlp::k_infer<<<infer_grid(count), hb::mlp::kInferBlock, 0, n->stream>>>(d);
     HB_MTRY(hipGetLastError());
     HB_MTRY(hipMemcpyAsync(y, n->out, count * 4, hipMemcpyDeviceToHost, n->stream));
     HB_MTRY(hipStreamSynchronize(n->stream));
@@ -609,7 +662,7 @@ hipError_t launch_surrogate_tail(hb_mlp *n, const uint32_t *off, const float *sn
     if (e != hipSuccess) return e;
     mlp::k_standardize<<<g, 256, 0, s>>>(n->stats, nodes, x_dev);
     mlp::Dev d{n->w, n->norm, x_dev, nullptr, nullptr, 0, nodes, 0.0f, nullptr, nullptr, out_dev};
-    mlp::k_predict<<<mlp_grid(nodes, 8192), mlp::kBlock, 0, s>>>(d);
+    mlp::k_infer<<<infer_grid(nodes), mlp::kInferBlock, 0, s>>>(d);
     mlp::k_isolated_last<<<g, 256, 0, s>>>(off, nodes, out_dev);
     return hipGetLastError();
 }
