@@ -187,7 +187,7 @@ static int finish_stage(hb_ctx *c, uint64_t n, uint64_t arcs, std::vector<uint32
     if (n) {
         unsigned long long *acc = nullptr, h_acc[4] = {};
         HB_TRY(hipMalloc(&acc, (4 + 4 * hb::kFingerprintBlocks) * sizeof(unsigned long long)));
-        hipError_t e = hb::arc_fingerprints(c->off.ptr, c->nbr.ptr, (uint32_t)n, acc, c->stream);
+        hipError_t e = hb::arc_fingerprints(c->off.ptr, c->nbr.ptr, (uint32_t)n, arcs, acc, c->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(h_acc, acc, sizeof h_acc, hipMemcpyDeviceToHost, c->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
         (void)hipFree(acc);
@@ -373,16 +373,21 @@ int hb_stage_csr(hb_ctx *c, const void *offsets, int off_bytes, uint64_t n, cons
 }  // extern "C"
 
 // Default bin bounds, measured with the degree layout on R-MAT-20 / R-MAT-24 (sweeps of
-// small_max 16..512 and wave_max 4096 / 16384): at p >= 6 a node of degree <= 256 is
-// cheapest on one sketch copy (2^p/16 lanes), at p <= 5 (one or two lanes per copy)
-// the first iteration's hashing wants the wider teams from degree 64 on.
-// mid_max = min(wave_max, 8 small_max).
+// small_max 16..512, wave_max 256..16384, chunk 2048..8192): at p >= 6 a node of degree
+// <= 256 is cheapest on one sketch copy (2^p/16 lanes), at p <= 5 (one or two lanes per
+// copy) the first iteration's hashing wants the wider teams from degree 64 on.  Hub
+// chunks shrink with the graph so that smaller graphs still spread their hubs over the
+// whole GPU: chunk = the power of two >= arcs / 65536 within [2048, 8192] (R-MAT-24:
+// 8192, R-MAT-20: 2048), wave_max = max(2048, chunk / 2).  mid_max = min(wave_max,
+// 8 small_max).
 static void default_schedule(hb_ctx *c, int p)
 {
     if (c->sched_user) return;
     c->small_max = p <= 5 ? 64 : 256;
-    c->wave_max = p <= 5 ? 16384 : 4096;
-    c->chunk = 8192;
+    uint32_t chunk = 2048;
+    while (chunk < 8192 && (uint64_t)chunk * 65536 < c->arcs) chunk *= 2;
+    c->chunk = chunk;
+    c->wave_max = p <= 5 ? 16384 : std::max<uint32_t>(2048, c->chunk / 2);
     c->mid_max = std::max(c->small_max, std::min<uint32_t>(c->wave_max, 8 * c->small_max));
 }
 

@@ -36,8 +36,8 @@ hipError_t launch_estimates(const HbDev &d, const uint8_t *regs, float *out, hip
 // arc fingerprints into work[0..3] (work holds 4 + 4 * kFingerprintBlocks values):
 // the CSR is symmetric iff work[0] == work[1] and work[2] == work[3]
 constexpr uint32_t kFingerprintBlocks = 8192;
-hipError_t arc_fingerprints(const uint32_t *off, const uint32_t *nbr, uint32_t n, unsigned long long *work,
-                            hipStream_t s);
+hipError_t arc_fingerprints(const uint32_t *off, const uint32_t *nbr, uint32_t n, uint64_t arcs,
+                            unsigned long long *work, hipStream_t s);
 // out[k] = sum_r part[r * cols + k] in a fixed order (T = unsigned long long or double)
 template <typename T>
 hipError_t sum_columns(const T *part, uint32_t rows, uint32_t cols, T *out, hipStream_t s);

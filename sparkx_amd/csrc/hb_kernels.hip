@@ -229,28 +229,31 @@ __device__ __forceinline__ void hb_fp_add(unsigned long long *a, uint64_t u, uin
     a[3] += hb * hb;
 }
 
+// every thread takes kFpArcs consecutive arcs: a binary search over the offsets finds
+// the first one's tail, then the walk advances the tail across node boundaries (empty
+// nodes between two arcs are crossed by exactly one thread), so hubs cost no more than
+// any other arcs
+constexpr uint32_t kFpArcs = 32;
+
 __global__ __launch_bounds__(kBlock) void k_arc_fingerprint(const uint32_t *off, const uint32_t *nbr, uint32_t n,
-                                                            unsigned long long *part)
+                                                            uint64_t arcs, unsigned long long *part)
 {
     __shared__ unsigned long long red[kBlock / HB_WAVE][4];
     const uint32_t lane = threadIdx.x & (HB_WAVE - 1), wv = threadIdx.x / HB_WAVE;
-    const uint32_t waves = gridDim.x * (kBlock / HB_WAVE);
     unsigned long long a[4] = {0, 0, 0, 0};
-    for (uint32_t base = (blockIdx.x * (kBlock / HB_WAVE) + wv) * HB_WAVE; base < n; base += waves * HB_WAVE) {
-        const uint32_t u = base + lane;  // uniform loop bound: every lane takes part in the ballot
-        uint32_t e0 = 0, e1 = 0;
-        if (u < n) {
-            e0 = off[u];
-            e1 = off[u + 1];
+    const uint64_t threads = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t t = (uint64_t)blockIdx.x * kBlock + threadIdx.x; t * kFpArcs < arcs; t += threads) {
+        const uint64_t e_lo = t * kFpArcs, e_hi = min<uint64_t>(arcs, e_lo + kFpArcs);
+        uint32_t lo = 0, hi = n - 1;  // largest u with off[u] <= e_lo (then off[u + 1] > e_lo)
+        while (lo < hi) {
+            const uint32_t mid = lo + (hi - lo + 1) / 2;
+            if (off[mid] <= e_lo) lo = mid;
+            else hi = mid - 1;
         }
-        if (e1 - e0 <= HB_WAVE)
-            for (uint32_t e = e0; e < e1; ++e) hb_fp_add(a, u, nbr[e]);
-        unsigned long long big = __ballot(e1 - e0 > HB_WAVE);
-        while (big) {
-            const int i = __ffsll(big) - 1;
-            big &= big - 1;
-            const uint32_t b0 = (uint32_t)__shfl((int)e0, i), b1 = (uint32_t)__shfl((int)e1, i);
-            for (uint32_t e = b0 + lane; e < b1; e += HB_WAVE) hb_fp_add(a, base + (uint32_t)i, nbr[e]);
+        uint32_t u = lo;
+        for (uint64_t e = e_lo; e < e_hi; ++e) {
+            while (off[u + 1] <= e) ++u;
+            hb_fp_add(a, u, nbr[e]);
         }
     }
     for (int k = 0; k < 4; ++k)
@@ -259,9 +262,9 @@ __global__ __launch_bounds__(kBlock) void k_arc_fingerprint(const uint32_t *off,
         for (int k = 0; k < 4; ++k) red[wv][k] = a[k];
     __syncthreads();
     if (threadIdx.x < 4) {
-        unsigned long long t = 0;
-        for (int w = 0; w < kBlock / HB_WAVE; ++w) t += red[w][threadIdx.x];
-        part[4 * blockIdx.x + threadIdx.x] = t;
+        unsigned long long sum = 0;
+        for (int w = 0; w < kBlock / HB_WAVE; ++w) sum += red[w][threadIdx.x];
+        part[4 * blockIdx.x + threadIdx.x] = sum;
     }
 }
 
@@ -330,6 +333,8 @@ __global__ __launch_bounds__(kBlock) void k_chunk(HbDev d, const uint32_t *__res
     const uint8_t *__restrict__ in = st->parity ? d.regs1 : d.regs0;
     const uint32_t *__restrict__ bitsR = (it & 1u) ? d.bits0 : d.bits1;
     const bool use_frontier = d.frontier && it > 1 && !st->skip_scan;
+    // push iteration: a hub whose active bit is clear has no changed neighbour at all
+    const bool push = st->push != 0;
     const bool first = it == 1;
     __shared__ uint32_t s_first[kBlock * 16];
     const uint32_t tl = threadIdx.x % T, cl = tl % G, slot = tl / G;
@@ -338,13 +343,12 @@ __global__ __launch_bounds__(kBlock) void k_chunk(HbDev d, const uint32_t *__res
 
     for (uint32_t task = blockIdx.x * (kBlock / T) + threadIdx.x / T; task < ntasks;
          task += waves_per_grid) {
-        (void)tnode;
         const uint32_t eb = tbeg[task], ee = tend[task];
         const uint32_t *__restrict__ nb = d.nbr + eb;
         const uint32_t *__restrict__ nbh = d.orig + eb;
         const uint32_t len = ee - eb;
-        bool active = true;
-        if (use_frontier) {
+        bool active = !(push && !bit_test(d.active, tnode[task]));
+        if (use_frontier && active) {
             uint32_t any = 0;
             for (uint32_t j = tl; j < len; j += T) any |= bit_test(bitsR, nb[j]);
             active = hb_copy_or<T>(any) != 0;
@@ -840,11 +844,11 @@ hipError_t launch_iteration(const HbDev &d, const HbSchedule &sc, bool nf, hipSt
     return hipGetLastError();
 }
 
-hipError_t arc_fingerprints(const uint32_t *off, const uint32_t *nbr, uint32_t n, unsigned long long *work,
-                            hipStream_t s)
+hipError_t arc_fingerprints(const uint32_t *off, const uint32_t *nbr, uint32_t n, uint64_t arcs,
+                            unsigned long long *work, hipStream_t s)
 {
     if (n == 0) return hipSuccess;
-    k_arc_fingerprint<<<kFingerprintBlocks, kBlock, 0, s>>>(off, nbr, n, work + 4);
+    k_arc_fingerprint<<<kFingerprintBlocks, kBlock, 0, s>>>(off, nbr, n, arcs, work + 4);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return sum_columns(work + 4, kFingerprintBlocks, 4, work, s);
