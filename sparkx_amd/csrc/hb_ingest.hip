@@ -4,13 +4,13 @@
 //
 // MatrixMarket follows the reference's reader (src/graph/cache/parser_mtx/producers.rs,
 // mod.rs; impl_sort.rs): the header and size line are read on the host; the body is
-// split into lines on the device (a select of the positions after each '\n'), one
-// thread parses each line (trim; blank and '%' lines skipped; "i j" 1-based, further
-// tokens ignored, a line with one token skipped; optional '+'; errors for a non-number,
-// a 0 or an index above n), and every entry emits i->j plus j->i when the file is not
-// "general" and i != j.  The reference then counts degrees, scatters and sorts each
-// list; here the arcs are 64-bit keys (u << 32 | v) sorted once by rocPRIM's radix
-// sort, which yields the same CSR: lists ascending, duplicates kept.
+// split into lines on the device (positions after each '\n', by chunked count, scan
+// and write passes), one thread parses each line (trim; blank and '%' lines skipped;
+// "i j" 1-based, further tokens ignored, a line with one token skipped; optional '+';
+// errors for a non-number, a 0 or an index above n), and every entry emits i->j plus
+// j->i when the file is not "general" and i != j.  The reference then counts degrees,
+// scatters and sorts each list; here the arcs are 64-bit keys (u << 32 | v) sorted once
+// by rocPRIM's radix sort, which yields the same CSR: lists ascending, duplicates kept.
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
@@ -36,10 +36,86 @@ __device__ __forceinline__ bool is_ws(uint8_t c)
     return c == ' ' || c == '\t' || c == '\n' || c == '\v' || c == '\f' || c == '\r';
 }
 
-struct AfterNewline {
-    const uint8_t *t;
-    __device__ bool operator()(uint64_t i) const { return t[i - 1] == '\n'; }
-};
+// Line splitting in two passes over chunks of kSplitChunk bytes, one block per chunk:
+// the block reads the chunk 4 KiB at a time, 16 bytes per thread (coalesced), counts
+// the '\n' of the chunk; after a scan of the counts the second pass re-reads the chunk
+// the same way and writes the position after each '\n' (below len) in text order
+// (a block-wide exclusive scan of the per-thread counts at every 4 KiB step).
+constexpr uint32_t kSplitSteps = 16, kSplitChunk = kSplitSteps * kBlock * 16;
+
+// bit b set when byte b of the 16 at t[p..p+16) is '\n' (bytes at or past len excluded)
+__device__ __forceinline__ uint32_t nl_mask16(const uint8_t *t, uint64_t p, uint64_t len)
+{
+    uint32_t m = 0;
+    if (p + 16 <= len) {
+        const uint4 v = *(const uint4 *)(t + p);
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) m |= (uint32_t)(((w[q] >> (8 * b)) & 0xffu) == '\n') << (4 * q + b);
+    } else {
+        for (uint64_t i = p; i < len; ++i) m |= (uint32_t)(t[i] == '\n') << (i - p);
+    }
+    return m;
+}
+
+// exclusive scan of x over the block; *total gets the block sum
+__device__ __forceinline__ uint32_t block_exclusive(uint32_t x, uint32_t *lds, uint32_t *total)
+{
+    lds[threadIdx.x] = x;
+    __syncthreads();
+    for (int w = 1; w < kBlock; w <<= 1) {
+        const uint32_t v = (int)threadIdx.x >= w ? lds[threadIdx.x - w] : 0u;
+        __syncthreads();
+        lds[threadIdx.x] += v;
+        __syncthreads();
+    }
+    const uint32_t incl = lds[threadIdx.x];
+    *total = lds[kBlock - 1];
+    __syncthreads();
+    return incl - x;
+}
+
+__global__ __launch_bounds__(kBlock) void k_count_nl(const uint8_t *t, uint64_t len, unsigned long long *cnt)
+{
+    __shared__ uint32_t red[kBlock];
+    const uint64_t chunk = (uint64_t)blockIdx.x * kSplitChunk;
+    uint32_t c = 0;
+    for (uint32_t k = 0; k < kSplitSteps; ++k) {
+        const uint64_t p = chunk + ((uint64_t)k * kBlock + threadIdx.x) * 16;
+        if (p < len) c += __popc(nl_mask16(t, p, len));
+    }
+    red[threadIdx.x] = c;
+    __syncthreads();
+    for (int w = kBlock / 2; w; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) cnt[blockIdx.x] = red[0];
+}
+
+// starts[1 + base[chunk] + k] = position after the k-th '\n' of the chunk, if below len
+__global__ __launch_bounds__(kBlock) void k_line_starts(const uint8_t *t, uint64_t len,
+                                                        const unsigned long long *base, uint64_t *starts)
+{
+    __shared__ uint32_t lds[kBlock];
+    const uint64_t chunk = (uint64_t)blockIdx.x * kSplitChunk;
+    uint64_t run = 1 + base[blockIdx.x];
+    for (uint32_t k = 0; k < kSplitSteps; ++k) {  // uniform trip count: barriers inside
+        const uint64_t p = chunk + ((uint64_t)k * kBlock + threadIdx.x) * 16;
+        uint32_t m = p < len ? nl_mask16(t, p, len) : 0u;
+        uint32_t total = 0;
+        uint64_t out = run + block_exclusive(__popc(m), lds, &total);
+        while (m) {
+            const uint32_t b = (uint32_t)(__ffs(m) - 1);
+            m &= m - 1u;
+            if (p + b + 1 < len) starts[out] = p + b + 1;
+            ++out;
+        }
+        run += total;
+    }
+}
 
 // usize token [b, e): optional '+', decimal digits, no overflow; false otherwise
 __device__ bool parse_usize(const uint8_t *t, uint64_t b, uint64_t e, uint64_t *out)
@@ -339,34 +415,35 @@ hipError_t mtx_body_to_csr(const uint8_t *text, uint64_t len, uint32_t n, bool s
     out->err_index = ~0ull;
     out->err_code = 0;
     // line l = [starts[l], starts[l + 1]); starts[0] = 0, starts[lines] = len
-    Tmp cnt, tmp;
-    HBI_TRY(cnt.alloc(8));
+    Tmp tmp;
     uint64_t nl = 0;
     Tmp starts;
     if (len > 1) {
-        // newlines first (the size of the line table), then their positions
-        size_t rbytes = 0;
-        auto is_nl = rocprim::make_transform_iterator(
-            text, [] __device__(uint8_t c) { return (unsigned long long)(c == '\n'); });
-        HBI_TRY(rocprim::reduce(nullptr, rbytes, is_nl, (unsigned long long *)cnt.p, 0ull, (size_t)len,
-                                rocprim::plus<unsigned long long>(), s));
-        Tmp rtmp;
-        HBI_TRY(rtmp.alloc(rbytes));
-        HBI_TRY(rocprim::reduce(rtmp.p, rbytes, is_nl, (unsigned long long *)cnt.p, 0ull, (size_t)len,
-                                rocprim::plus<unsigned long long>(), s));
-        uint64_t total_nl = 0;
-        HBI_TRY(hipMemcpyAsync(&total_nl, cnt.p, 8, hipMemcpyDeviceToHost, s));
-        HBI_TRY(hipStreamSynchronize(s));
-        HBI_TRY(starts.alloc((total_nl + 2) * 8));  // positions after a '\n' below len, plus both ends
+        const uint64_t chunks = (len + kSplitChunk - 1) / kSplitChunk;
+        Tmp per, base;
+        HBI_TRY(per.alloc(chunks * 8));
+        HBI_TRY(base.alloc(chunks * 8));
+        k_count_nl<<<(unsigned)chunks, kBlock, 0, s>>>(text, len, (unsigned long long *)per.p);
+        HBI_TRY(hipGetLastError());
         size_t bytes = 0;
-        auto first = rocprim::make_counting_iterator<uint64_t>(1);
-        HBI_TRY(rocprim::select(nullptr, bytes, first, (uint64_t *)starts.p + 1, (unsigned long long *)cnt.p,
-                                (size_t)(len - 1), AfterNewline{text}, s));
+        HBI_TRY(rocprim::exclusive_scan(nullptr, bytes, (const unsigned long long *)per.p,
+                                        (unsigned long long *)base.p, 0ull, (size_t)chunks,
+                                        rocprim::plus<unsigned long long>(), s));
         HBI_TRY(tmp.alloc(bytes));
-        HBI_TRY(rocprim::select(tmp.p, bytes, first, (uint64_t *)starts.p + 1, (unsigned long long *)cnt.p,
-                                (size_t)(len - 1), AfterNewline{text}, s));
-        HBI_TRY(hipMemcpyAsync(&nl, cnt.p, 8, hipMemcpyDeviceToHost, s));
+        HBI_TRY(rocprim::exclusive_scan(tmp.p, bytes, (const unsigned long long *)per.p,
+                                        (unsigned long long *)base.p, 0ull, (size_t)chunks,
+                                        rocprim::plus<unsigned long long>(), s));
+        unsigned long long last[2] = {0, 0};
+        HBI_TRY(hipMemcpyAsync(&last[0], (unsigned long long *)base.p + chunks - 1, 8, hipMemcpyDeviceToHost, s));
+        HBI_TRY(hipMemcpyAsync(&last[1], (unsigned long long *)per.p + chunks - 1, 8, hipMemcpyDeviceToHost, s));
+        uint8_t tail = 0;
+        HBI_TRY(hipMemcpyAsync(&tail, text + len - 1, 1, hipMemcpyDeviceToHost, s));
         HBI_TRY(hipStreamSynchronize(s));
+        nl = last[0] + last[1] - (tail == '\n' ? 1u : 0u);  // a final '\n' starts no line
+        HBI_TRY(starts.alloc((nl + 2) * 8));
+        k_line_starts<<<(unsigned)chunks, kBlock, 0, s>>>(text, len, (const unsigned long long *)base.p,
+                                                           (uint64_t *)starts.p);
+        HBI_TRY(hipGetLastError());
     } else {
         HBI_TRY(starts.alloc(16));
     }
