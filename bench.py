@@ -35,6 +35,7 @@ CONFIGS = {
     "surrogate": (dict(kind="surrogate", train_graphs=32, train_scale=20, seed=1), 7, 0),
     "exact": (dict(kind="exact", scale=20, edge_factor=16, seed=1, sources=2048), 0, 0),
     "ingest": (dict(kind="ingest", scale=22, edge_factor=16, seed=1), 0, 0),
+    "many64": (dict(kind="many", count=64, seed=1, threads=8), 0, 0),
 }
 
 
@@ -238,6 +239,67 @@ def bench_batch(args, spec):
     print(json.dumps(out), flush=True)
 
 
+def bench_many(args, spec):
+    """The same 64 graphs (32 ER + 32 R-MAT, n log-uniform in [2^12, 2^18], average degree
+    U[4, 32], p uniform in {4..8}) run to convergence together: one engine context and
+    stream per graph, driven by `threads` host threads (hb_run_many).  value =
+    graph-iterations per second of the whole batch; the same contexts run one after the
+    other (one thread) are reported beside it."""
+    from sparkx_amd.batch import HyperBallMany, batched_workload
+    t0 = time.perf_counter()
+    items = batched_workload(spec["count"], seed=spec["seed"])
+    nodes = sum(g.size() for g, _ in items)
+    arcs = sum(g.width() for g, _ in items)
+    log(f"many: {len(items)} graphs, {nodes} nodes, {arcs} arcs, built in {time.perf_counter() - t0:.1f}s")
+    many = HyperBallMany(items, threads=spec["threads"])
+    for _ in range(args.warmup):
+        many.run()
+    walls, its, arcs_g = [], 0, 0
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        stats = many.run()
+        walls.append(time.perf_counter() - t0)
+        its += sum(st.iterations for st in stats)
+        arcs_g += sum(st.arcs_gathered for st in stats)
+        log(f"step {i}: {sum(st.iterations for st in stats)} graph-iterations (longest "
+            f"{max(st.iterations for st in stats)}) in {walls[-1] * 1e3:.2f} ms wall")
+    total = sum(walls)
+    many.threads = 1
+    t0 = time.perf_counter()
+    many.run()
+    seq = time.perf_counter() - t0
+    many.threads = spec["threads"]
+    log(f"one after the other: {seq * 1e3:.2f} ms")
+    cpu = None
+    if not args.no_cpu_baseline:
+        import numpy as np
+
+        from oracle import hb_oracle as O
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        threads = max(1, min(threads, os.cpu_count() or 1))
+        loop, n_it = 0.0, 0
+        for g, p in items[::8]:  # every 8th graph, run to convergence
+            r = O.hyperball(g.offsets.astype(np.uint64), g.neighbours, p, threads=threads, packed=True)
+            loop += r.loop_seconds
+            n_it += r.iterations
+        cpu = {"value": round(n_it / loop, 3), "unit": "graph-iter/s", "cores": threads, "kind": "port",
+               "sample": f"8 of the 64 graphs (every 8th) run to convergence, {n_it} iterations in "
+                         f"{loop:.1f}s, packed 6-bit registers, register init excluded"}
+    out = {
+        "metric": "Batched HyperBall graph-iterations/s (64 graphs, mixed p=4..8)",
+        "value": round(its / total, 2), "unit": "graph-iter/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(total / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+        "config": {"workload": "many64", "graphs": len(items), "nodes": nodes, "arcs": arcs,
+                   "host_threads": spec["threads"]},
+        "gteps": round(arcs_g / total / 1e9, 3),
+        "ms_one_after_the_other": round(seq * 1e3, 3),
+        "roofline": None, "cpu_baseline": cpu,
+    }
+    many.close()
+    print(json.dumps(out), flush=True)
+
+
 def bench_exact(args, spec):
     """Exact BFS closeness (exact.rs semantics, bfs.rs source quirk) of the first
     `sources` nodes of R-MAT scale 20: 64 bit-lane searches per pass on the GPU.
@@ -370,6 +432,8 @@ def main():
         return bench_exact(args, spec)
     if spec["kind"] == "ingest":
         return bench_ingest(args, spec)
+    if spec["kind"] == "many":
+        return bench_many(args, spec)
     if spec["kind"] == "surrogate":
         print(json.dumps(bench_surrogate(args, spec)), flush=True)
         return None

@@ -134,6 +134,15 @@ int hb_run(hb_ctx *ctx, int p, int max_iters, uint32_t flags, hb_stats *stats);
  * changes.  No reference counterpart: the reference walks the caller's CSR as given. */
 int hb_set_layout(hb_ctx *ctx, int layout);
 
+/* Runs hb_run(ctxs[i], p[i], max_iters, flags, stats + i) for i < count (stats may be
+ * NULL), each context on its own stream, from a pool of `threads` host threads (0: up
+ * to 8), so the GPU interleaves many independent graphs.  Every context runs to its
+ * own end; the first failing context (by index) is reported.  Contexts must be
+ * distinct.  The batched form of HyperBallInner::new over many graphs (no reference
+ * counterpart beyond calling it once per graph). */
+int hb_run_many(hb_ctx **ctxs, const int *p, int count, int max_iters, uint32_t flags,
+                hb_stats *stats, int threads);
+
 /* Copies n * 2^p u8 registers (node-major) of the selected buffer to dst.
  * Replaces: the counters file HyperBallInner::counters (hyper_ball.rs:92). */
 int hb_read_registers(hb_ctx *ctx, int which, uint8_t *dst);

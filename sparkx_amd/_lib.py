@@ -32,7 +32,7 @@ EXPORTS = (
     "hb_set_snapshots", "hb_read_snapshots", "hb_mlp_create", "hb_mlp_destroy", "hb_mlp_set_params",
     "hb_mlp_get_params", "hb_mlp_gradient", "hb_mlp_train", "hb_mlp_predict", "hb_surrogate",
     "hb_exact_bfs", "hb_exact_centrality", "hb_eval_ranks", "hb_eval_metrics", "hb_set_layout",
-    "hb_stage_mtx", "hb_stage_coo", "hb_read_csr",
+    "hb_stage_mtx", "hb_stage_coo", "hb_read_csr", "hb_run_many",
 )
 
 
@@ -176,6 +176,8 @@ def load() -> C.CDLL:
         "hb_stage_mtx": (C.c_int, [vp, C.c_char_p, C.c_uint64, C.POINTER(HbMtxInfo)]),
         "hb_stage_coo": (C.c_int, [vp, C.c_uint64, u32p, u32p, C.c_uint64, C.c_uint32]),
         "hb_read_csr": (C.c_int, [vp, u64p, u64p]),
+        "hb_run_many": (C.c_int, [C.POINTER(vp), C.POINTER(C.c_int), C.c_int, C.c_int, C.c_uint32,
+                                  C.POINTER(HbStats), C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -1,7 +1,13 @@
-"""Batched HyperBall: many independent graphs (p = 4..8 each) advanced together
-by the engine's work-queue kernel (include/sparkx_hb.h, hb_batch_*), plus the
-seeded 64-graph workload (half Erdos-Renyi, half R-MAT, n log-uniform in
-[2^12, 2^18], average degree uniform in [4, 32], p uniform in {4..8})."""
+"""Batched HyperBall: many independent graphs run together.
+
+- HyperBallMany: one engine context (own stream) per (graph, p), all advanced at once
+  by hb_run_many from a pool of host threads; every graph's results are exactly its
+  own single-graph run's.
+- HyperBallBatch: the persistent work-queue kernel over (graph, degree-bin) tiles
+  (hb_batch_*), switched off in this build (hb_batch_run returns HB_ESTATE).
+- batched_workload: the seeded 64-graph workload (half Erdos-Renyi, half R-MAT, n
+  log-uniform in [2^12, 2^18], average degree uniform in [4, 32], p uniform in {4..8}).
+"""
 from __future__ import annotations
 
 import ctypes as C
@@ -37,6 +43,49 @@ def _rmat(scale, deg, seed, threads):
     n = 1 << scale
     _lib.check(_lib.load().hb_graph_rmat(C.byref(h), scale, int(n * deg / 2), 0.57, 0.19, 0.19, seed, threads))
     return Graph._take(h, f"rmat{scale}_d{deg:.1f}_s{seed}")
+
+
+class HyperBallMany:
+    """Runs HyperBall on every (graph, p) of `items` concurrently; `hbs[i]` is the
+    HyperBall of item i (its accessors, epilogues and caches as usual)."""
+
+    def __init__(self, items, device: int = 0, threads: int = 0, layout: str = "degree"):
+        from .hyper_ball import HyperBall
+        self.items = list(items)
+        self.threads = threads
+        self.hbs = []
+        try:
+            for g, p in self.items:
+                self.hbs.append(HyperBall(g, p, _run=False, device=device, layout=layout))
+        except Exception:
+            self.close()
+            raise
+
+    def run(self, max_iters: int = 0, dense: bool = False, timing: bool = False):
+        n = len(self.hbs)
+        ctxs = (C.c_void_p * n)(*[h._ctx.value for h in self.hbs])
+        ps = (C.c_int * n)(*[h.p for h in self.hbs])
+        stats = (_lib.HbStats * n)()
+        flags = (_lib.HB_FLAG_DENSE if dense else 0) | (_lib.HB_FLAG_TIMING if timing else 0)
+        L = _lib.load()
+        _lib.check(L.hb_run_many(ctxs, ps, n, max_iters, flags, stats, self.threads))
+        out = []
+        for h, st in zip(self.hbs, stats):
+            own = _lib.HbStats()
+            C.memmove(C.byref(own), C.byref(st), C.sizeof(own))
+            out.append(h._ran(own))
+        return out
+
+    def close(self):
+        for h in self.hbs:
+            h.close()
+        self.hbs = []
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
 
 
 class HyperBallBatch:

@@ -9,12 +9,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/sparkx_hb.h"
@@ -1216,6 +1218,37 @@ extern "C" int hb_read_csr(hb_ctx *c, uint64_t *offsets, uint64_t *nbrs)
         HB_TRY(hipMemcpy(tmp.data(), c->nbr.ptr, c->arcs * 4, hipMemcpyDeviceToHost));
         for (uint64_t i = 0; i < c->arcs; ++i) nbrs[i] = tmp[i];
     }
+    g_err.clear();
+    return HB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// many graphs at once: every context runs the single-graph engine on its own stream,
+// driven by a pool of host threads, so the GPU interleaves the graphs' kernels
+// ---------------------------------------------------------------------------
+extern "C" int hb_run_many(hb_ctx **ctxs, const int *p, int count, int max_iters, uint32_t flags,
+                           hb_stats *stats, int threads)
+{
+    if (count < 0 || (count && (!ctxs || !p))) return fail(HB_EINVAL, "hb_run_many: bad arguments");
+    for (int i = 0; i < count; ++i)
+        if (!ctxs[i]) return fail(HB_EINVAL, "hb_run_many: context %d is NULL", i);
+    if (threads <= 0) threads = std::min(count, 8);
+    threads = std::max(1, std::min(threads, count));
+    std::vector<int> rc(count, HB_OK);
+    std::vector<std::string> msg(count);
+    std::atomic<int> next{0};
+    auto work = [&]() {
+        for (int i = next++; i < count; i = next++) {
+            rc[i] = hb_run(ctxs[i], p[i], max_iters, flags, stats ? stats + i : nullptr);
+            if (rc[i] != HB_OK) msg[i] = g_err;  // this worker's thread-local message
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; ++t) pool.emplace_back(work);
+    work();
+    for (auto &th : pool) th.join();
+    for (int i = 0; i < count; ++i)
+        if (rc[i] != HB_OK) return fail(rc[i], "hb_run_many: context %d: %s", i, msg[i].c_str());
     g_err.clear();
     return HB_OK;
 }

@@ -103,6 +103,10 @@ class HyperBall:
                  | (_lib.HB_FLAG_KTIMING if ktiming else 0))
         st = _lib.HbStats()
         _lib.check(self._L.hb_run(self._ctx, self.p, max_iters, flags, C.byref(st)))
+        return self._ran(st)
+
+    def _ran(self, st: "_lib.HbStats") -> "_lib.HbStats":
+        """A run of this context completed (here or through hb_run_many): new results."""
         self.stats = st
         self._closeness = [None, None]
         self._harmonic = [None, None]

@@ -617,3 +617,45 @@ def test_layout_switch_and_snapshots(hb, oracle):
         assert np.array_equal(snapshots(h), snaps_in)
         with pytest.raises(ValueError):
             h.set_layout("random")
+
+
+def test_run_many_matches_oracle(hb, oracle):
+    """hb_run_many: graphs of different sizes, densities and p (4..12) on their own
+    streams from 4 host threads; each one bit-identical to the oracle."""
+    from sparkx_amd.batch import HyperBallMany
+    items = []
+    for i, p in enumerate((4, 5, 6, 7, 8, 6, 7, 12, 9, 5)):
+        if i % 3 == 0:
+            g = hb.Graph.erdos_renyi(600 + 150 * i, 2500 + 700 * i, seed=70 + i)
+        elif i % 3 == 1:
+            g = hb.Graph.rmat(9 + i % 4, 8, seed=80 + i)
+        else:
+            g = _layout_graph(hb, "directed")
+        items.append((g, p))
+    refs = [_oracle_run(oracle, g, p) for g, p in items]
+    with HyperBallMany(items, threads=4) as many:
+        for rep in range(2):
+            say("run_many", rep)
+            stats = many.run()
+            for i, (h, ref, (g, p)) in enumerate(zip(many.hbs, refs, items)):
+                assert stats[i].iterations == ref.iterations
+                _assert_same(h, ref, oracle, p, f"graph {i} p={p} rep {rep}")
+
+
+def test_run_many_reports_the_failing_context(hb):
+    from sparkx_amd import _lib
+    L = _lib.load()
+    g = hb.Graph.erdos_renyi(500, 2000, seed=3)
+    ok = hb.HyperBall(g, 6, _run=False)
+    empty = C.c_void_p()
+    _lib.check(L.hb_create(C.byref(empty), 0))  # nothing staged
+    try:
+        ctxs = (C.c_void_p * 2)(ok._ctx.value, empty.value)
+        ps = (C.c_int * 2)(6, 6)
+        rc = L.hb_run_many(ctxs, ps, 2, 0, 0, None, 2)
+        assert rc == _lib.HB_ESTATE
+        assert b"context 1" in L.hb_last_error()
+        assert ok.registers(0).shape == (500, 64)  # the other context completed
+    finally:
+        L.hb_destroy(empty)
+        ok.close()
