@@ -10,7 +10,9 @@
 //              visited |= newly, frontier = newly, reachable += |newly|, total += d |newly|
 // which discovers exactly the nodes (and distances) of the reference's FIFO queue.
 // Arcs are followed forwards (push), so directed graphs are searched as the
-// reference searches them.
+// reference searches them; on symmetric graphs a level with a large frontier pulls
+// instead (each node not yet reached ORs its neighbours' frontier words), which
+// computes the same `next` without atomics.
 //
 // Per-source sums, bit-identical to the reference's:
 //   reachable : integer count (BFSDists::recheable)
@@ -93,17 +95,83 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_push(const uint32_t *__restri
     }
 }
 
-// newly = next & ~visited; visited |= newly; frontier = newly; per-source counts of
-// newly (one ballot per bit lane per wavefront; the block's 64 counts go to
-// part[64 * block], summed by sum_columns); dist (optional, [node][64] u16)
-// records the level of each newly set bit
-__global__ __launch_bounds__(kBfsBlock) void k_bfs_apply(uint64_t *frontier, uint64_t *visited, uint64_t *next,
-                                                         uint16_t *dist, uint32_t n, uint32_t level,
-                                                         unsigned long long *part)
+// Level 1: the frontier is the pass's sources s0..s0+count-1, whose lists lie side by
+// side in nbr[off[s0], off[s0 + count]).  One thread per arc (its source found by a
+// binary search over the count + 1 offsets in LDS), so a pass of hub sources spreads
+// over the whole grid instead of one wavefront.  Nothing is visited yet at level 1.
+__global__ __launch_bounds__(kBfsBlock) void k_bfs_push_sources(const uint32_t *__restrict__ off,
+                                                                const uint32_t *__restrict__ nbr, uint64_t *next,
+                                                                uint32_t s0, uint32_t count)
 {
-    __shared__ unsigned long long wc[kBfsBlock / 64][64];
+    __shared__ uint32_t so[65];
+    if (threadIdx.x <= count) so[threadIdx.x] = off[s0 + threadIdx.x];
+    __syncthreads();
+    const uint32_t e0 = so[0], e1 = so[count];
+    for (uint64_t e = (uint64_t)e0 + blockIdx.x * blockDim.x + threadIdx.x; e < e1; e += gridDim.x * blockDim.x) {
+        uint32_t lo = 0, hi = count;  // so[lo] <= e < so[hi]
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (so[mid] <= e) lo = mid; else hi = mid;
+        }
+        atomicOr((unsigned long long *)&next[nbr[e]], 1ull << lo);
+    }
+}
+
+// Pull form of one level, for symmetric CSRs (u -> v present iff v -> u): every node
+// that some source has not reached ORs the frontier words of its neighbours.  No
+// atomics; a node whose unreached sources all appear stops early.  Lane per node of
+// degree <= 64, whole wavefront per larger node (as in the push).
+__global__ __launch_bounds__(kBfsBlock) void k_bfs_pull(const uint32_t *__restrict__ off, const uint32_t *__restrict__ nbr,
+                                                        const uint64_t *__restrict__ frontier,
+                                                        const uint64_t *__restrict__ visited, uint64_t *next, uint32_t n,
+                                                        uint64_t full)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t waves = gridDim.x * (kBfsBlock / 64);
+    for (uint32_t base = (blockIdx.x * (kBfsBlock / 64) + threadIdx.x / 64) * 64; base < n; base += waves * 64) {
+        const uint32_t v = base + lane;  // uniform loop bound: every lane takes part in the ballot
+        uint64_t want = 0;                // sources that have not reached v
+        uint32_t e0 = 0, e1 = 0;
+        if (v < n) {
+            want = full & ~visited[v];
+            if (want) {
+                e0 = off[v];
+                e1 = off[v + 1];
+            }
+        }
+        uint64_t got = 0;
+        if (e1 - e0 <= 64)
+            for (uint32_t e = e0; e < e1 && (got & want) != want; ++e) got |= frontier[nbr[e]];
+        if (v < n && e1 - e0 <= 64) next[v] = got & want;
+        unsigned long long big = __ballot(e1 - e0 > 64);
+        while (big) {
+            const int i = __ffsll(big) - 1;
+            big &= big - 1;
+            const uint32_t b0 = (uint32_t)__shfl((int)e0, i), b1 = (uint32_t)__shfl((int)e1, i);
+            const uint64_t wi = (uint64_t)__shfl((long long)want, i);
+            uint64_t g = 0;
+            for (uint32_t e = b0 + lane; e < b1; e += 64) g |= frontier[nbr[e]];
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) g |= (uint64_t)__shfl_xor((long long)g, o);
+            if ((int)lane == i) next[base + (uint32_t)i] = g & wi;
+        }
+    }
+}
+
+// newly = next & ~visited; visited |= newly; frontier = newly; per-source counts of
+// newly (one ballot per bit lane per wavefront) and the arcs of the new frontier's
+// nodes: the block's 65 values go to part[65 * block], summed by sum_columns; dist
+// (optional, [node][64] u16) records the level of each newly set bit
+constexpr uint32_t kBfsCols = 65;
+
+__global__ __launch_bounds__(kBfsBlock) void k_bfs_apply(const uint32_t *__restrict__ off, uint64_t *frontier,
+                                                         uint64_t *visited, uint64_t *next, uint16_t *dist, uint32_t n,
+                                                         uint32_t level, unsigned long long *part)
+{
+    __shared__ unsigned long long wc[kBfsBlock / 64][kBfsCols];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     unsigned long long mine = 0;  // count of bit `lane` over this wavefront's nodes
+    unsigned long long farcs = 0;  // arcs of this lane's nodes that joined the frontier
     // the loop bound is uniform across the wavefront, so every lane takes part in the ballots
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         const uint32_t v = base + threadIdx.x;
@@ -116,6 +184,7 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_apply(uint64_t *frontier, uin
                 next[v] = 0ull;
                 if (newly) {
                     visited[v] = vis | newly;
+                    farcs += off[v + 1] - off[v];
                     if (dist) {
                         uint64_t b = newly;
                         while (b) {
@@ -135,33 +204,59 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_apply(uint64_t *frontier, uin
             }
         }
     }
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) farcs += (unsigned long long)__shfl_xor((long long)farcs, o);
     wc[wave][lane] = mine;
+    if (lane == 0) wc[wave][64] = farcs;
     __syncthreads();
-    if (threadIdx.x < 64) {
+    if (threadIdx.x < kBfsCols) {
         unsigned long long s = 0;
         for (int w = 0; w < kBfsBlock / 64; ++w) s += wc[w][threadIdx.x];
-        part[64 * blockIdx.x + threadIdx.x] = s;
+        part[kBfsCols * blockIdx.x + threadIdx.x] = s;
     }
 }
 
-// inv_sum[k] = Σ_{v=0..n-1} inv[dist[v][k]] as a sequential f64 sum in node order;
-// one wavefront, lane k = source k; inv[0] = 0 (unreached), inv[d] = 1/d
+// inv_sum[64 b + k] = Σ_{v=0..n-1} inv[dist_b[v][k]] as a sequential f64 sum in node
+// order, dist_b = dist + b n 64 the distances of pass b of a group; inv[0] = 0
+// (unreached), inv[d] = 1/d for d < entries.  One wavefront per pass (lane k = source
+// k): the adds of a source are one dependent chain, so the passes of a group run side
+// by side, and each chain is kept fed by loading the next U distances while the
+// current U are added.  A table of at most kInvLds entries sits in LDS.
+constexpr uint32_t kInvLds = 4096;
+
+template <bool kLds>
 __global__ __launch_bounds__(64) void k_bfs_inv_sum(const uint16_t *__restrict__ dist, const double *__restrict__ inv,
-                                                    uint32_t n, uint32_t count, double *out)
+                                                    uint32_t entries, uint32_t n, const uint32_t *__restrict__ counts,
+                                                    double *out)
 {
-    const uint32_t k = threadIdx.x;
-    constexpr uint32_t U = 16;
-    double acc = 0.0;
-    uint32_t v = 0;
-    for (; v + U <= n; v += U) {
-        double t[U];
-#pragma unroll
-        for (uint32_t j = 0; j < U; ++j) t[j] = inv[dist[(size_t)(v + j) * 64 + k]];
-#pragma unroll
-        for (uint32_t j = 0; j < U; ++j) acc = __dadd_rn(acc, t[j]);
+    __shared__ double tab[kLds ? kInvLds : 1];
+    const uint32_t k = threadIdx.x, b = blockIdx.x;
+    if (kLds) {
+        for (uint32_t i = k; i < entries; i += 64) tab[i] = inv[i];
+        __syncthreads();
     }
-    for (; v < n; ++v) acc = __dadd_rn(acc, inv[dist[(size_t)v * 64 + k]]);
-    if (k < count) out[k] = acc;
+    auto look = [&](uint16_t x) -> double { return kLds ? tab[x] : inv[x]; };
+    const uint16_t *__restrict__ d = dist + (size_t)b * n * 64 + k;
+    constexpr uint32_t U = 32;
+    double acc = 0.0;
+    uint16_t cur[U], nxt[U];
+    uint32_t v = 0;
+    if (n >= U) {
+#pragma unroll
+        for (uint32_t j = 0; j < U; ++j) cur[j] = d[(size_t)j * 64];
+    }
+    for (; v + U <= n; v += U) {
+        if (v + 2 * U <= n) {  // the next block exists iff the loop runs again
+#pragma unroll
+            for (uint32_t j = 0; j < U; ++j) nxt[j] = d[(size_t)(v + U + j) * 64];
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < U; ++j) acc = __dadd_rn(acc, look(cur[j]));
+#pragma unroll
+        for (uint32_t j = 0; j < U; ++j) cur[j] = nxt[j];
+    }
+    for (; v < n; ++v) acc = __dadd_rn(acc, look(d[(size_t)v * 64]));
+    if (k < counts[b]) out[64 * b + k] = acc;
 }
 
 }  // namespace hb
@@ -197,50 +292,86 @@ struct Scoped {
 
 namespace hb {
 
-int exact_bfs(const uint32_t *off, const uint32_t *nbr, uint32_t n, uint64_t s_begin, uint64_t s_end,
-              uint64_t *reach, uint64_t *total, double *inv_sum, hipStream_t s)
+int exact_bfs(const uint32_t *off, const uint32_t *nbr, uint32_t n, bool symmetric, uint64_t s_begin,
+              uint64_t s_end, uint64_t *reach, uint64_t *total, double *inv_sum, hipStream_t s)
 {
     if (s_begin > s_end || s_end > n) return efail(HB_EINVAL, "hb_exact_bfs: sources [%llu, %llu) outside [0, %u)",
                                                    (unsigned long long)s_begin, (unsigned long long)s_end, n);
     if (s_begin == s_end) return HB_OK;
-    Scoped bits, cnt, dist, inv, sums;
+    Scoped bits, cnt, dist, inv, sums, cnts;
     const unsigned grid = (unsigned)std::min<uint64_t>((n + kBfsBlock - 1) / kBfsBlock, 4096);
+    // passes of 64 sources; with harmonic sums, the distances of up to `group` passes are
+    // kept (u16 per node and source) so their sequential sums run side by side
+    const uint64_t passes = (s_end - s_begin + 63) / 64;
+    const uint64_t pass_bytes = (uint64_t)n * 64 * 2;
+    const uint32_t group = inv_sum ? (uint32_t)std::max<uint64_t>(
+                                         1, std::min<uint64_t>({32, passes, (16ull << 30) / pass_bytes}))
+                                   : 1;
     HBX_TRY(hipMalloc(&bits.p, (size_t)3 * n * 8));
-    HBX_TRY(hipMalloc(&cnt.p, (size_t)(64 + 64 * grid) * 8));  // 64 totals, then 64 per block
+    HBX_TRY(hipMalloc(&cnt.p, (size_t)kBfsCols * (1 + grid) * 8));  // totals, then kBfsCols per block
+    uint32_t arcs = 0;
+    HBX_TRY(hipMemcpy(&arcs, off + n, 4, hipMemcpyDeviceToHost));
     if (inv_sum) {
-        HBX_TRY(hipMalloc(&dist.p, (size_t)n * 64 * 2));
-        HBX_TRY(hipMalloc(&sums.p, 64 * 8));
+        HBX_TRY(hipMalloc(&dist.p, (size_t)group * pass_bytes));
+        HBX_TRY(hipMalloc(&sums.p, (size_t)group * 64 * 8));
+        HBX_TRY(hipMalloc(&cnts.p, (size_t)group * 4));
     }
     uint64_t *fr = (uint64_t *)bits.p, *vis = fr + n, *nx = fr + 2 * (size_t)n;
     unsigned long long *d_cnt = (unsigned long long *)cnt.p;
-    std::vector<unsigned long long> hc(64);
+    std::vector<unsigned long long> hc(kBfsCols);
     std::vector<double> h_inv;
-    for (uint64_t s0 = s_begin; s0 < s_end; s0 += 64) {
-        const uint32_t count = (uint32_t)std::min<uint64_t>(64, s_end - s0);
-        uint64_t r[64] = {}, t[64] = {};
+    std::vector<uint32_t> h_counts(group);
+    for (uint64_t g0 = 0; g0 < passes; g0 += group) {
+        const uint32_t gn = (uint32_t)std::min<uint64_t>(group, passes - g0);
         uint32_t levels = 0;
-        k_bfs_init<<<grid, kBfsBlock, 0, s>>>(fr, vis, nx, n, (uint32_t)s0, count);
-        if (dist.p) k_bfs_zero<uint16_t><<<4096, kBfsBlock, 0, s>>>((uint16_t *)dist.p, (uint64_t)n * 64);
-        for (uint32_t level = 1;; ++level) {
-            if (dist.p && level > 65535u)
-                return efail(HB_ERANGE, "hb_exact_bfs: eccentricity above 65535 (harmonic sums need u16 distances)");
-            k_bfs_push<<<grid, kBfsBlock, 0, s>>>(off, nbr, fr, vis, nx, n);
-            k_bfs_apply<<<grid, kBfsBlock, 0, s>>>(fr, vis, nx, (uint16_t *)dist.p, n, level, d_cnt + 64);
-            HBX_TRY(sum_columns(d_cnt + 64, grid, 64, d_cnt, s));
-            HBX_TRY(hipMemcpyAsync(hc.data(), d_cnt, 64 * 8, hipMemcpyDeviceToHost, s));
-            HBX_TRY(hipStreamSynchronize(s));
-            bool any = false;
-            for (int k = 0; k < 64; ++k) any = any || hc[k] != 0;
-            if (!any) break;
-            levels = level;
-            for (int k = 0; k < 64; ++k) {
-                r[k] += hc[k];
-                t[k] += (uint64_t)level * hc[k];
+        for (uint32_t j = 0; j < gn; ++j) {
+            const uint64_t s0 = s_begin + (g0 + j) * 64;
+            const uint32_t count = (uint32_t)std::min<uint64_t>(64, s_end - s0);
+            uint16_t *dj = dist.p ? (uint16_t *)dist.p + (size_t)j * n * 64 : nullptr;
+            h_counts[j] = count;
+            uint64_t r[64] = {}, t[64] = {};
+            const uint64_t full = count == 64 ? ~0ull : ((1ull << count) - 1);
+            // arcs leaving the frontier: the sources' own lists first
+            uint64_t frontier_arcs = 0;
+            {
+                std::vector<uint32_t> so(count + 1);
+                HBX_TRY(hipMemcpy(so.data(), off + s0, (count + 1) * 4, hipMemcpyDeviceToHost));
+                frontier_arcs = so[count] - so[0];
             }
-        }
-        for (uint32_t k = 0; k < count; ++k) {
-            if (reach) reach[s0 - s_begin + k] = r[k];
-            if (total) total[s0 - s_begin + k] = t[k];
+            k_bfs_init<<<grid, kBfsBlock, 0, s>>>(fr, vis, nx, n, (uint32_t)s0, count);
+            if (dj) k_bfs_zero<uint16_t><<<4096, kBfsBlock, 0, s>>>(dj, (uint64_t)n * 64);
+            for (uint32_t level = 1;; ++level) {
+                if (dj && level > 65535u)
+                    return efail(HB_ERANGE, "hb_exact_bfs: eccentricity above 65535 (harmonic sums need u16 distances)");
+                // direction (symmetric graphs only): pull when the frontier's arcs exceed
+                // 1/8 of all arcs -- a pull scans every unreached node's list once without
+                // atomics, a push walks the frontier's lists with an atomic OR per arc
+                if (level == 1) {
+                    const unsigned sg = (unsigned)std::max<uint64_t>(
+                        1, std::min<uint64_t>((frontier_arcs + kBfsBlock - 1) / kBfsBlock, 4096));
+                    k_bfs_push_sources<<<sg, kBfsBlock, 0, s>>>(off, nbr, nx, (uint32_t)s0, count);
+                } else if (symmetric && frontier_arcs * 8 > arcs)
+                    k_bfs_pull<<<grid, kBfsBlock, 0, s>>>(off, nbr, fr, vis, nx, n, full);
+                else
+                    k_bfs_push<<<grid, kBfsBlock, 0, s>>>(off, nbr, fr, vis, nx, n);
+                k_bfs_apply<<<grid, kBfsBlock, 0, s>>>(off, fr, vis, nx, dj, n, level, d_cnt + kBfsCols);
+                HBX_TRY(sum_columns(d_cnt + kBfsCols, grid, kBfsCols, d_cnt, s));
+                HBX_TRY(hipMemcpyAsync(hc.data(), d_cnt, kBfsCols * 8, hipMemcpyDeviceToHost, s));
+                HBX_TRY(hipStreamSynchronize(s));
+                bool any = false;
+                for (int k = 0; k < 64; ++k) any = any || hc[k] != 0;
+                frontier_arcs = hc[64];
+                if (!any) break;
+                levels = std::max(levels, level);
+                for (int k = 0; k < 64; ++k) {
+                    r[k] += hc[k];
+                    t[k] += (uint64_t)level * hc[k];
+                }
+            }
+            for (uint32_t k = 0; k < count; ++k) {
+                if (reach) reach[s0 - s_begin + k] = r[k];
+                if (total) total[s0 - s_begin + k] = t[k];
+            }
         }
         if (inv_sum) {
             // 1/d as correctly rounded host divisions, d = 1..levels; entry 0 is an unreached node
@@ -253,9 +384,21 @@ int exact_bfs(const uint32_t *off, const uint32_t *nbr, uint32_t n, uint64_t s_b
                 HBX_TRY(hipMalloc(&inv.p, h_inv.size() * 8));
                 HBX_TRY(hipMemcpyAsync(inv.p, h_inv.data(), h_inv.size() * 8, hipMemcpyHostToDevice, s));
             }
-            k_bfs_inv_sum<<<1, 64, 0, s>>>((const uint16_t *)dist.p, (const double *)inv.p, n, count, (double *)sums.p);
-            HBX_TRY(hipMemcpyAsync(inv_sum + (s0 - s_begin), sums.p, (size_t)count * 8, hipMemcpyDeviceToHost, s));
+            HBX_TRY(hipMemcpyAsync(cnts.p, h_counts.data(), (size_t)gn * 4, hipMemcpyHostToDevice, s));
+            const uint32_t entries = (uint32_t)h_inv.size();
+            if (entries <= kInvLds)
+                k_bfs_inv_sum<true><<<gn, 64, 0, s>>>((const uint16_t *)dist.p, (const double *)inv.p, entries, n,
+                                                      (const uint32_t *)cnts.p, (double *)sums.p);
+            else
+                k_bfs_inv_sum<false><<<gn, 64, 0, s>>>((const uint16_t *)dist.p, (const double *)inv.p, entries, n,
+                                                       (const uint32_t *)cnts.p, (double *)sums.p);
+            HBX_TRY(hipGetLastError());
+            std::vector<double> hs((size_t)gn * 64);
+            HBX_TRY(hipMemcpyAsync(hs.data(), sums.p, hs.size() * 8, hipMemcpyDeviceToHost, s));
             HBX_TRY(hipStreamSynchronize(s));
+            for (uint32_t j = 0; j < gn; ++j)
+                for (uint32_t k = 0; k < h_counts[j]; ++k)
+                    inv_sum[(g0 + j) * 64 + k] = hs[(size_t)j * 64 + k];
         }
     }
     return HB_OK;

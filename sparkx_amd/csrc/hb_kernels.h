@@ -79,8 +79,9 @@ hipError_t coo_to_csr(const uint32_t *src, const uint32_t *dst, uint64_t m, uint
 hipError_t launch_surrogate_tail(struct ::hb_mlp *n, const uint32_t *off, const float *snaps, uint32_t nodes,
                                  float *x_dev, float *out_dev, hipStream_t s);
 int mlp_device(const struct ::hb_mlp *n);
-// exact BFS sums of sources [s_begin, s_end) over a staged CSR (hb_exact.hip)
-int exact_bfs(const uint32_t *off, const uint32_t *nbr, uint32_t n, uint64_t s_begin, uint64_t s_end,
-              uint64_t *reach, uint64_t *total, double *inv_sum, hipStream_t s);
+// exact BFS sums of sources [s_begin, s_end) over a staged CSR (hb_exact.hip);
+// symmetric: the CSR holds v -> u for every u -> v (levels may then pull)
+int exact_bfs(const uint32_t *off, const uint32_t *nbr, uint32_t n, bool symmetric, uint64_t s_begin,
+              uint64_t s_end, uint64_t *reach, uint64_t *total, double *inv_sum, hipStream_t s);
 
 }  // namespace hb

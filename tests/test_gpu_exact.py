@@ -74,7 +74,8 @@ def _csr(n, arcs):
     return off, np.array([v for x in adj for v in x], np.uint64)
 
 
-@pytest.mark.parametrize("name", ["directed_random", "directed_cycle", "self_loops", "isolated", "path", "star"])
+@pytest.mark.parametrize("name", ["directed_random", "directed_cycle", "self_loops", "isolated", "path", "star",
+                                  "long_path"])
 def test_exact_bfs_edge_cases(hb, oracle, name):
     rng = np.random.default_rng(7)
     if name == "directed_random":
@@ -89,8 +90,9 @@ def test_exact_bfs_edge_cases(hb, oracle, name):
     elif name == "isolated":
         n = 100
         arcs = [(3, 4), (4, 3)]
-    elif name == "path":
-        n = 700
+    elif name in ("path", "long_path"):
+        # 4500 levels: the 1/d table of the harmonic sums no longer fits in LDS
+        n = 700 if name == "path" else 4500
         arcs = [(i, i + 1) for i in range(n - 1)] + [(i + 1, i) for i in range(n - 1)]
     else:
         n = 5000
