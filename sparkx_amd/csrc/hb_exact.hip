@@ -57,7 +57,8 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_zero(T *p, uint64_t count)
 // next[v] |= frontier[u] for every arc u -> v whose bits are not all visited at v.
 // A wavefront takes 64 consecutive nodes at a time: a frontier node of degree <= 64
 // is walked by its own lane, a larger one by the whole wavefront, so hubs do not
-// serialise on one lane.
+// serialise on one lane; nodes of degree > hub_deg are left to k_bfs_hubs, which
+// spreads their lists over many wavefronts.
 __device__ __forceinline__ void hb_bfs_push_arc(const uint64_t *__restrict__ visited, uint64_t *next, uint32_t v,
                                                 uint64_t f)
 {
@@ -67,7 +68,8 @@ __device__ __forceinline__ void hb_bfs_push_arc(const uint64_t *__restrict__ vis
 
 __global__ __launch_bounds__(kBfsBlock) void k_bfs_push(const uint32_t *__restrict__ off, const uint32_t *__restrict__ nbr,
                                                         const uint64_t *__restrict__ frontier,
-                                                        const uint64_t *__restrict__ visited, uint64_t *next, uint32_t n)
+                                                        const uint64_t *__restrict__ visited, uint64_t *next, uint32_t n,
+                                                        uint32_t hub_deg)
 {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t waves = gridDim.x * (kBfsBlock / 64);
@@ -84,7 +86,7 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_push(const uint32_t *__restri
         }
         if (e1 - e0 <= 64)
             for (uint32_t e = e0; e < e1; ++e) hb_bfs_push_arc(visited, next, nbr[e], f);
-        unsigned long long big = __ballot(e1 - e0 > 64);
+        unsigned long long big = __ballot(e1 - e0 > 64 && e1 - e0 <= hub_deg);
         while (big) {
             const int i = __ffsll(big) - 1;
             big &= big - 1;
@@ -117,14 +119,31 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_push_sources(const uint32_t *
     }
 }
 
+// OR of frontier[nbr[e]] over e = b0 + lane, b0 + lane + 64, ... < b1 (this lane's
+// share of a list walked by a whole wavefront), four arcs' loads in flight at a time
+__device__ __forceinline__ uint64_t hb_bfs_or_range(const uint32_t *__restrict__ nbr,
+                                                    const uint64_t *__restrict__ frontier, uint32_t b0, uint32_t b1,
+                                                    uint32_t lane)
+{
+    uint64_t g = 0;
+    uint64_t e = (uint64_t)b0 + lane;
+    for (; e + 3 * 64 < b1; e += 4 * 64) {
+        const uint32_t w0 = nbr[e], w1 = nbr[e + 64], w2 = nbr[e + 128], w3 = nbr[e + 192];
+        g |= frontier[w0] | frontier[w1] | frontier[w2] | frontier[w3];
+    }
+    for (; e < b1; e += 64) g |= frontier[nbr[e]];
+    return g;
+}
+
 // Pull form of one level, for symmetric CSRs (u -> v present iff v -> u): every node
 // that some source has not reached ORs the frontier words of its neighbours.  No
 // atomics; a node whose unreached sources all appear stops early.  Lane per node of
-// degree <= 64, whole wavefront per larger node (as in the push).
+// degree <= 64, whole wavefront per node of degree <= hub_deg (as in the push); the
+// larger ones are k_bfs_hubs' (their next[] word is left at the 0 that apply wrote).
 __global__ __launch_bounds__(kBfsBlock) void k_bfs_pull(const uint32_t *__restrict__ off, const uint32_t *__restrict__ nbr,
                                                         const uint64_t *__restrict__ frontier,
                                                         const uint64_t *__restrict__ visited, uint64_t *next, uint32_t n,
-                                                        uint64_t full)
+                                                        uint64_t full, uint32_t hub_deg)
 {
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t waves = gridDim.x * (kBfsBlock / 64);
@@ -140,20 +159,61 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_pull(const uint32_t *__restri
             }
         }
         uint64_t got = 0;
-        if (e1 - e0 <= 64)
-            for (uint32_t e = e0; e < e1 && (got & want) != want; ++e) got |= frontier[nbr[e]];
+        if (e1 - e0 <= 64) {
+            // 8 arcs per step, loads issued together; the early stop is checked per step
+            constexpr uint32_t B = 8;
+            for (uint64_t e = e0; e < e1 && (got & want) != want; e += B) {
+                uint32_t w[B];
+#pragma unroll
+                for (uint32_t j = 0; j < B; ++j) w[j] = e + j < e1 ? nbr[e + j] : 0u;
+#pragma unroll
+                for (uint32_t j = 0; j < B; ++j)
+                    if (e + j < e1) got |= frontier[w[j]];
+            }
+        }
         if (v < n && e1 - e0 <= 64) next[v] = got & want;
-        unsigned long long big = __ballot(e1 - e0 > 64);
+        unsigned long long big = __ballot(e1 - e0 > 64 && e1 - e0 <= hub_deg);
         while (big) {
             const int i = __ffsll(big) - 1;
             big &= big - 1;
             const uint32_t b0 = (uint32_t)__shfl((int)e0, i), b1 = (uint32_t)__shfl((int)e1, i);
             const uint64_t wi = (uint64_t)__shfl((long long)want, i);
-            uint64_t g = 0;
-            for (uint32_t e = b0 + lane; e < b1; e += 64) g |= frontier[nbr[e]];
+            uint64_t g = hb_bfs_or_range(nbr, frontier, b0, b1, lane);
 #pragma unroll
             for (int o = 1; o < 64; o <<= 1) g |= (uint64_t)__shfl_xor((long long)g, o);
             if ((int)lane == i) next[base + (uint32_t)i] = g & wi;
+        }
+    }
+}
+
+// The lists of nodes of degree > hub_deg, cut into chunks (hub_node[c], arcs
+// [hub_beg[c], hub_end[c])), one wavefront per chunk.  Pull: a chunk ORs its frontier
+// words and ORs the bits its hub still wants into next[hub] (zero before the level).
+// Push: a chunk of a frontier hub pushes the hub's word along its arcs.
+template <bool kPull>
+__global__ __launch_bounds__(kBfsBlock) void k_bfs_hubs(const uint32_t *__restrict__ nbr,
+                                                        const uint32_t *__restrict__ hub_node,
+                                                        const uint32_t *__restrict__ hub_beg,
+                                                        const uint32_t *__restrict__ hub_end, uint32_t chunks,
+                                                        const uint64_t *__restrict__ frontier,
+                                                        const uint64_t *__restrict__ visited, uint64_t *next,
+                                                        uint64_t full)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t waves = gridDim.x * (kBfsBlock / 64);
+    for (uint32_t c = blockIdx.x * (kBfsBlock / 64) + threadIdx.x / 64; c < chunks; c += waves) {
+        const uint32_t v = hub_node[c], b0 = hub_beg[c], b1 = hub_end[c];
+        if (kPull) {
+            const uint64_t want = full & ~visited[v];
+            if (!want) continue;
+            uint64_t g = hb_bfs_or_range(nbr, frontier, b0, b1, lane);
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) g |= (uint64_t)__shfl_xor((long long)g, o);
+            if (lane == 0 && (g & want)) atomicOr((unsigned long long *)&next[v], (unsigned long long)(g & want));
+        } else {
+            const uint64_t f = frontier[v];
+            if (!f) continue;
+            for (uint32_t e = b0 + lane; e < b1; e += 64) hb_bfs_push_arc(visited, next, nbr[e], f);
         }
     }
 }
@@ -309,8 +369,33 @@ int exact_bfs(const uint32_t *off, const uint32_t *nbr, uint32_t n, bool symmetr
                                    : 1;
     HBX_TRY(hipMalloc(&bits.p, (size_t)3 * n * 8));
     HBX_TRY(hipMalloc(&cnt.p, (size_t)kBfsCols * (1 + grid) * 8));  // totals, then kBfsCols per block
-    uint32_t arcs = 0;
-    HBX_TRY(hipMemcpy(&arcs, off + n, 4, hipMemcpyDeviceToHost));
+    // hubs: nodes of degree > kHubDeg, their lists cut into chunks of kHubDeg arcs
+    constexpr uint32_t kHubDeg = 4096;
+    std::vector<uint32_t> h_off((size_t)n + 1);
+    HBX_TRY(hipMemcpy(h_off.data(), off, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost));
+    const uint32_t arcs = h_off[n];
+    std::vector<uint32_t> h_hub;  // node, beg, end per chunk, as three runs
+    {
+        std::vector<uint32_t> hn, hb_, he;
+        for (uint32_t v = 0; v < n; ++v)
+            if (h_off[v + 1] - h_off[v] > kHubDeg)
+                for (uint64_t b = h_off[v]; b < h_off[v + 1]; b += kHubDeg) {
+                    hn.push_back(v);
+                    hb_.push_back((uint32_t)b);
+                    he.push_back((uint32_t)std::min<uint64_t>(b + kHubDeg, h_off[v + 1]));
+                }
+        h_hub = hn;
+        h_hub.insert(h_hub.end(), hb_.begin(), hb_.end());
+        h_hub.insert(h_hub.end(), he.begin(), he.end());
+    }
+    const uint32_t chunks = (uint32_t)(h_hub.size() / 3);
+    Scoped hub;
+    if (chunks) {
+        HBX_TRY(hipMalloc(&hub.p, h_hub.size() * 4));
+        HBX_TRY(hipMemcpy(hub.p, h_hub.data(), h_hub.size() * 4, hipMemcpyHostToDevice));
+    }
+    const uint32_t *hub_node = (const uint32_t *)hub.p, *hub_beg = hub_node + chunks, *hub_end = hub_beg + chunks;
+    const unsigned hub_grid = (unsigned)std::min<uint64_t>((chunks + 3) / 4, 4096);
     if (inv_sum) {
         HBX_TRY(hipMalloc(&dist.p, (size_t)group * pass_bytes));
         HBX_TRY(hipMalloc(&sums.p, (size_t)group * 64 * 8));
@@ -332,12 +417,7 @@ int exact_bfs(const uint32_t *off, const uint32_t *nbr, uint32_t n, bool symmetr
             uint64_t r[64] = {}, t[64] = {};
             const uint64_t full = count == 64 ? ~0ull : ((1ull << count) - 1);
             // arcs leaving the frontier: the sources' own lists first
-            uint64_t frontier_arcs = 0;
-            {
-                std::vector<uint32_t> so(count + 1);
-                HBX_TRY(hipMemcpy(so.data(), off + s0, (count + 1) * 4, hipMemcpyDeviceToHost));
-                frontier_arcs = so[count] - so[0];
-            }
+            uint64_t frontier_arcs = h_off[s0 + count] - h_off[s0];
             k_bfs_init<<<grid, kBfsBlock, 0, s>>>(fr, vis, nx, n, (uint32_t)s0, count);
             if (dj) k_bfs_zero<uint16_t><<<4096, kBfsBlock, 0, s>>>(dj, (uint64_t)n * 64);
             for (uint32_t level = 1;; ++level) {
@@ -350,10 +430,17 @@ int exact_bfs(const uint32_t *off, const uint32_t *nbr, uint32_t n, bool symmetr
                     const unsigned sg = (unsigned)std::max<uint64_t>(
                         1, std::min<uint64_t>((frontier_arcs + kBfsBlock - 1) / kBfsBlock, 4096));
                     k_bfs_push_sources<<<sg, kBfsBlock, 0, s>>>(off, nbr, nx, (uint32_t)s0, count);
-                } else if (symmetric && frontier_arcs * 8 > arcs)
-                    k_bfs_pull<<<grid, kBfsBlock, 0, s>>>(off, nbr, fr, vis, nx, n, full);
-                else
-                    k_bfs_push<<<grid, kBfsBlock, 0, s>>>(off, nbr, fr, vis, nx, n);
+                } else if (symmetric && frontier_arcs * 8 > arcs) {
+                    k_bfs_pull<<<grid, kBfsBlock, 0, s>>>(off, nbr, fr, vis, nx, n, full, kHubDeg);
+                    if (chunks)
+                        k_bfs_hubs<true><<<hub_grid, kBfsBlock, 0, s>>>(nbr, hub_node, hub_beg, hub_end, chunks, fr,
+                                                                        vis, nx, full);
+                } else {
+                    k_bfs_push<<<grid, kBfsBlock, 0, s>>>(off, nbr, fr, vis, nx, n, kHubDeg);
+                    if (chunks)
+                        k_bfs_hubs<false><<<hub_grid, kBfsBlock, 0, s>>>(nbr, hub_node, hub_beg, hub_end, chunks, fr,
+                                                                         vis, nx, full);
+                }
                 k_bfs_apply<<<grid, kBfsBlock, 0, s>>>(off, fr, vis, nx, dj, n, level, d_cnt + kBfsCols);
                 HBX_TRY(sum_columns(d_cnt + kBfsCols, grid, kBfsCols, d_cnt, s));
                 HBX_TRY(hipMemcpyAsync(hc.data(), d_cnt, kBfsCols * 8, hipMemcpyDeviceToHost, s));

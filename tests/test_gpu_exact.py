@@ -75,7 +75,7 @@ def _csr(n, arcs):
 
 
 @pytest.mark.parametrize("name", ["directed_random", "directed_cycle", "self_loops", "isolated", "path", "star",
-                                  "long_path"])
+                                  "long_path", "directed_hub"])
 def test_exact_bfs_edge_cases(hb, oracle, name):
     rng = np.random.default_rng(7)
     if name == "directed_random":
@@ -94,6 +94,10 @@ def test_exact_bfs_edge_cases(hb, oracle, name):
         # 4500 levels: the 1/d table of the harmonic sums no longer fits in LDS
         n = 700 if name == "path" else 4500
         arcs = [(i, i + 1) for i in range(n - 1)] + [(i + 1, i) for i in range(n - 1)]
+    elif name == "directed_hub":
+        # out-degree 9000 (> 4096: its list is pushed in chunks) and a random sparse rest
+        n = 9001
+        arcs = [(0, i) for i in range(1, n)] + [(int(a), int(b)) for a, b in rng.integers(1, n, size=(6000, 2))]
     else:
         n = 5000
         arcs = [(0, i) for i in range(1, n)] + [(i, 0) for i in range(1, n)]
