@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -278,6 +279,58 @@ int hb_set_layout(hb_ctx *c, int layout)
     return HB_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// Two pinned host buffers shared by every staging call of the process (allocated on
+// first use, kept to the end: pinning host pages costs milliseconds per call);
+// g_pin_mu serialises the calls that use them.
+constexpr size_t kPinElems = size_t(8) << 20;  // u32 per buffer (32 MiB)
+std::mutex g_pin_mu;
+uint32_t *g_pin[2] = {nullptr, nullptr};
+
+// dst[i] = src[pos + i] as u32 for i < len, split over host threads; returns the
+// first arc index whose neighbour is >= n, or UINT64_MAX
+uint64_t convert_neighbours(const void *nbrs, int nbr_bytes, uint64_t pos, uint64_t len, uint64_t n,
+                            uint32_t *dst)
+{
+    auto run = [&](uint64_t a, uint64_t b) -> uint64_t {
+        uint64_t bad = UINT64_MAX;
+        if (nbr_bytes == 8) {
+            const uint64_t *src = (const uint64_t *)nbrs + pos;
+            for (uint64_t i = a; i < b; ++i) {
+                const uint64_t v = src[i];
+                if (v >= n && bad == UINT64_MAX) bad = pos + i;
+                dst[i] = (uint32_t)v;
+            }
+        } else {
+            const uint32_t *src = (const uint32_t *)nbrs + pos;
+            uint32_t hi = 0;
+            for (uint64_t i = a; i < b; ++i) {
+                const uint32_t v = src[i];
+                hi = std::max(hi, v);
+                dst[i] = v;
+            }
+            if (hi >= n)
+                for (uint64_t i = a; i < b; ++i)
+                    if (src[i] >= n) return pos + i;
+        }
+        return bad;
+    };
+    const unsigned hw = std::max(1u, std::min(8u, std::thread::hardware_concurrency()));
+    const unsigned T = len >= (1u << 20) ? hw : 1u;
+    if (T == 1) return run(0, len);
+    std::vector<uint64_t> bad(T, UINT64_MAX);
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < T; ++t)
+        pool.emplace_back([&, t] { bad[t] = run(len * t / T, len * (t + 1) / T); });
+    for (auto &th : pool) th.join();
+    return *std::min_element(bad.begin(), bad.end());
+}
+}  // namespace
+
+extern "C" {
+
 // Copies offsets/neighbours into HBM through two pinned staging buffers, converting
 // to u32 and validating on the way (every neighbour id < n, offsets monotone).
 int hb_stage_csr(hb_ctx *c, const void *offsets, int off_bytes, uint64_t n, const void *nbrs,
@@ -313,58 +366,44 @@ int hb_stage_csr(hb_ctx *c, const void *offsets, int off_bytes, uint64_t n, cons
     HB_TRY(c->nbr.ensure(arcs));
     HB_TRY(hipMemcpyAsync(c->off.ptr, h_off.data(), (n + 1) * 4, hipMemcpyHostToDevice, c->stream));
 
-    const size_t kStage = size_t(16) << 20;  // u32 elements per staging buffer (64 MiB)
-    uint32_t *pinned[2] = {nullptr, nullptr};
-    hipEvent_t done[2] = {nullptr, nullptr};
     int rc = HB_OK;
-    for (int b = 0; b < 2 && rc == HB_OK; ++b) {
-        if (hipHostMalloc((void **)&pinned[b], kStage * 4, hipHostMallocDefault) != hipSuccess)
-            rc = fail(HB_ENOMEM, "hipHostMalloc of %zu bytes failed", kStage * 4);
-        else if (hipEventCreateWithFlags(&done[b], hipEventDisableTiming) != hipSuccess)
-            rc = fail(HB_EHIP, "hipEventCreate failed");
-    }
-    uint64_t pos = 0;
-    int b = 0;
-    while (rc == HB_OK && pos < arcs) {
-        const uint64_t len = std::min<uint64_t>(kStage, arcs - pos);
-        if (hipEventSynchronize(done[b]) != hipSuccess) {
-            rc = fail(HB_EHIP, "hipEventSynchronize failed");
-            break;
-        }
-        uint32_t *dst = pinned[b];
-        uint64_t bad = 0, bad_at = 0;
-        if (nbr_bytes == 8) {
-            const uint64_t *src = (const uint64_t *)nbrs + pos;
-            for (uint64_t i = 0; i < len; ++i) {
-                const uint64_t v = src[i];
-                if (v >= n && !bad) { bad = v + 1; bad_at = pos + i; }
-                dst[i] = (uint32_t)v;
-            }
-        } else {
-            const uint32_t *src = (const uint32_t *)nbrs + pos;
-            for (uint64_t i = 0; i < len; ++i) {
-                const uint32_t v = src[i];
-                if (v >= n && !bad) { bad = (uint64_t)v + 1; bad_at = pos + i; }
-                dst[i] = v;
+    if (arcs) {
+        std::lock_guard<std::mutex> lock(g_pin_mu);
+        hipEvent_t done[2] = {nullptr, nullptr};
+        for (int b = 0; b < 2 && rc == HB_OK; ++b) {
+            if (!g_pin[b] && hipHostMalloc((void **)&g_pin[b], kPinElems * 4, hipHostMallocPortable) != hipSuccess) {
+                g_pin[b] = nullptr;
+                rc = fail(HB_ENOMEM, "hipHostMalloc of %zu bytes failed", kPinElems * 4);
+            } else if (hipEventCreateWithFlags(&done[b], hipEventDisableTiming) != hipSuccess) {
+                rc = fail(HB_EHIP, "hipEventCreate failed");
             }
         }
-        if (bad) {
-            rc = fail(HB_EINVAL, "hb_stage_csr: neighbour %llu at arc %llu is not < n=%llu",
-                      (unsigned long long)(bad - 1), (unsigned long long)bad_at, (unsigned long long)n);
-            break;
+        uint64_t pos = 0;
+        int b = 0;
+        while (rc == HB_OK && pos < arcs) {
+            const uint64_t len = std::min<uint64_t>(kPinElems, arcs - pos);
+            if (hipEventSynchronize(done[b]) != hipSuccess) {  // the copy out of g_pin[b] is done
+                rc = fail(HB_EHIP, "hipEventSynchronize failed");
+                break;
+            }
+            const uint64_t bad_at = convert_neighbours(nbrs, nbr_bytes, pos, len, n, g_pin[b]);
+            if (bad_at != UINT64_MAX) {
+                const uint64_t v = nbr_bytes == 8 ? ((const uint64_t *)nbrs)[bad_at] : ((const uint32_t *)nbrs)[bad_at];
+                rc = fail(HB_EINVAL, "hb_stage_csr: neighbour %llu at arc %llu is not < n=%llu", (unsigned long long)v,
+                          (unsigned long long)bad_at, (unsigned long long)n);
+                break;
+            }
+            if (hipMemcpyAsync(c->nbr.ptr + pos, g_pin[b], len * 4, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+                hipEventRecord(done[b], c->stream) != hipSuccess) {
+                rc = fail(HB_EHIP, "staging copy failed");
+                break;
+            }
+            pos += len;
+            b ^= 1;
         }
-        if (hipMemcpyAsync(c->nbr.ptr + pos, dst, len * 4, hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-            hipEventRecord(done[b], c->stream) != hipSuccess) {
-            rc = fail(HB_EHIP, "staging copy failed");
-            break;
-        }
-        pos += len;
-        b ^= 1;
-    }
-    (void)hipStreamSynchronize(c->stream);
-    for (int i = 0; i < 2; ++i) {
-        if (pinned[i]) (void)hipHostFree(pinned[i]);
-        if (done[i]) (void)hipEventDestroy(done[i]);
+        (void)hipStreamSynchronize(c->stream);  // no copy still reads the shared buffers
+        for (int i = 0; i < 2; ++i)
+            if (done[i]) (void)hipEventDestroy(done[i]);
     }
     if (rc != HB_OK) return rc;
     if (int rc2 = finish_stage(c, n, arcs, &h_off)) return rc2;

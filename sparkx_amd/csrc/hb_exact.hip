@@ -278,45 +278,81 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_apply(const uint32_t *__restr
 
 // inv_sum[64 b + k] = Σ_{v=0..n-1} inv[dist_b[v][k]] as a sequential f64 sum in node
 // order, dist_b = dist + b n 64 the distances of pass b of a group; inv[0] = 0
-// (unreached), inv[d] = 1/d for d < entries.  One wavefront per pass (lane k = source
-// k): the adds of a source are one dependent chain, so the passes of a group run side
-// by side, and each chain is kept fed by loading the next U distances while the
-// current U are added.  A table of at most kInvLds entries sits in LDS.
+// (unreached), inv[d] = 1/d for d < entries.  One block per pass: the adds of a source
+// are one dependent chain (lane k of wavefront 0 = source k), so the passes of a group
+// run side by side, and the whole block streams the distances through LDS in tiles of
+// kInvTile nodes -- tile t + 1 is loaded while tile t is summed -- so the summing
+// wavefront never waits on HBM.  A 1/d table of at most kInvLds entries sits in LDS.
 constexpr uint32_t kInvLds = 4096;
+constexpr uint32_t kInvBlock = 256;
+constexpr uint32_t kInvTile = 256;                    // nodes per tile: 32 KiB of u16 distances
+constexpr uint32_t kInvVec = kInvTile * 64 * 2 / 16;  // uint4 per tile
+constexpr uint32_t kInvPer = kInvVec / kInvBlock;     // uint4 per thread per tile
 
 template <bool kLds>
-__global__ __launch_bounds__(64) void k_bfs_inv_sum(const uint16_t *__restrict__ dist, const double *__restrict__ inv,
-                                                    uint32_t entries, uint32_t n, const uint32_t *__restrict__ counts,
-                                                    double *out)
+__global__ __launch_bounds__(kInvBlock) void k_bfs_inv_sum(const uint16_t *__restrict__ dist,
+                                                           const double *__restrict__ inv, uint32_t entries, uint32_t n,
+                                                           const uint32_t *__restrict__ counts, double *out)
 {
     __shared__ double tab[kLds ? kInvLds : 1];
-    const uint32_t k = threadIdx.x, b = blockIdx.x;
-    if (kLds) {
-        for (uint32_t i = k; i < entries; i += 64) tab[i] = inv[i];
+    __shared__ uint4 tile[2][kInvVec];
+    const uint32_t b = blockIdx.x, tid = threadIdx.x;
+    if (kLds)
+        for (uint32_t i = tid; i < entries; i += kInvBlock) tab[i] = inv[i];
+    const uint4 *__restrict__ src = (const uint4 *)(dist + (size_t)b * n * 64);  // 8 uint4 per node
+    const uint32_t tiles = (n + kInvTile - 1) / kInvTile;
+    auto vecs = [&](uint32_t t) -> uint32_t {
+        return (uint32_t)(std::min<uint64_t>(n, (uint64_t)(t + 1) * kInvTile) - (uint64_t)t * kInvTile) * 8u;
+    };
+    uint4 r[kInvPer];
+    auto load = [&](uint32_t t) {
+        const uint32_t m = vecs(t);
+        const uint4 *p = src + (size_t)t * kInvVec;
+#pragma unroll
+        for (uint32_t j = 0; j < kInvPer; ++j) {
+            const uint32_t i = tid + j * kInvBlock;
+            if (i < m) r[j] = p[i];
+        }
+    };
+    auto store = [&](uint32_t buf, uint32_t t) {
+        const uint32_t m = vecs(t);
+#pragma unroll
+        for (uint32_t j = 0; j < kInvPer; ++j) {
+            const uint32_t i = tid + j * kInvBlock;
+            if (i < m) tile[buf][i] = r[j];
+        }
+    };
+    double acc = 0.0;
+    if (tiles) {
+        load(0);
+        store(0, 0);
+    }
+    __syncthreads();
+    for (uint32_t t = 0; t < tiles; ++t) {
+        if (t + 1 < tiles) load(t + 1);  // in flight while wavefront 0 sums tile t
+        if (tid < 64) {
+            const uint16_t *d = (const uint16_t *)tile[t & 1];
+            const uint32_t m = vecs(t) / 8u;
+            uint32_t j = 0;
+            for (; j + 16 <= m; j += 16) {
+                double x[16];
+#pragma unroll
+                for (uint32_t q = 0; q < 16; ++q) {
+                    const uint16_t dv = d[(j + q) * 64 + tid];
+                    x[q] = kLds ? tab[dv] : inv[dv];
+                }
+#pragma unroll
+                for (uint32_t q = 0; q < 16; ++q) acc = __dadd_rn(acc, x[q]);
+            }
+            for (; j < m; ++j) {
+                const uint16_t dv = d[j * 64 + tid];
+                acc = __dadd_rn(acc, kLds ? tab[dv] : inv[dv]);
+            }
+        }
+        if (t + 1 < tiles) store((t + 1) & 1, t + 1);
         __syncthreads();
     }
-    auto look = [&](uint16_t x) -> double { return kLds ? tab[x] : inv[x]; };
-    const uint16_t *__restrict__ d = dist + (size_t)b * n * 64 + k;
-    constexpr uint32_t U = 32;
-    double acc = 0.0;
-    uint16_t cur[U], nxt[U];
-    uint32_t v = 0;
-    if (n >= U) {
-#pragma unroll
-        for (uint32_t j = 0; j < U; ++j) cur[j] = d[(size_t)j * 64];
-    }
-    for (; v + U <= n; v += U) {
-        if (v + 2 * U <= n) {  // the next block exists iff the loop runs again
-#pragma unroll
-            for (uint32_t j = 0; j < U; ++j) nxt[j] = d[(size_t)(v + U + j) * 64];
-        }
-#pragma unroll
-        for (uint32_t j = 0; j < U; ++j) acc = __dadd_rn(acc, look(cur[j]));
-#pragma unroll
-        for (uint32_t j = 0; j < U; ++j) cur[j] = nxt[j];
-    }
-    for (; v < n; ++v) acc = __dadd_rn(acc, look(d[(size_t)v * 64]));
-    if (k < counts[b]) out[64 * b + k] = acc;
+    if (tid < counts[b]) out[64 * b + tid] = acc;
 }
 
 }  // namespace hb
@@ -474,10 +510,10 @@ int exact_bfs(const uint32_t *off, const uint32_t *nbr, uint32_t n, bool symmetr
             HBX_TRY(hipMemcpyAsync(cnts.p, h_counts.data(), (size_t)gn * 4, hipMemcpyHostToDevice, s));
             const uint32_t entries = (uint32_t)h_inv.size();
             if (entries <= kInvLds)
-                k_bfs_inv_sum<true><<<gn, 64, 0, s>>>((const uint16_t *)dist.p, (const double *)inv.p, entries, n,
+                k_bfs_inv_sum<true><<<gn, kInvBlock, 0, s>>>((const uint16_t *)dist.p, (const double *)inv.p, entries, n,
                                                       (const uint32_t *)cnts.p, (double *)sums.p);
             else
-                k_bfs_inv_sum<false><<<gn, 64, 0, s>>>((const uint16_t *)dist.p, (const double *)inv.p, entries, n,
+                k_bfs_inv_sum<false><<<gn, kInvBlock, 0, s>>>((const uint16_t *)dist.p, (const double *)inv.p, entries, n,
                                                        (const uint32_t *)cnts.p, (double *)sums.p);
             HBX_TRY(hipGetLastError());
             std::vector<double> hs((size_t)gn * 64);

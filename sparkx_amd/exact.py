@@ -40,6 +40,8 @@ class _Exact:
         self._ctx = C.c_void_p()
         _lib.check(self._L.hb_create(C.byref(self._ctx), device))
         try:
+            # the searches walk the caller's CSR: no degree-ordered copy is built
+            _lib.check(self._L.hb_set_layout(self._ctx, _lib.HB_LAYOUT_INPUT))
             off, nb = g.offsets, g.neighbours
             _lib.check(self._L.hb_stage_csr(self._ctx, off.ctypes.data, off.itemsize, n,
                                             nb.ctypes.data if nb.size else None, nb.itemsize, g.width()))
@@ -98,6 +100,7 @@ def exact_bfs(g: Graph, sources: Optional[Tuple[int, int]] = None, inv_sum: bool
     ctx = C.c_void_p()
     _lib.check(L.hb_create(C.byref(ctx), device))
     try:
+        _lib.check(L.hb_set_layout(ctx, _lib.HB_LAYOUT_INPUT))
         off, nb = g.offsets, g.neighbours
         _lib.check(L.hb_stage_csr(ctx, off.ctypes.data, off.itemsize, n, nb.ctypes.data if nb.size else None,
                                   nb.itemsize, g.width()))
