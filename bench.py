@@ -123,13 +123,17 @@ def bench_surrogate(args, spec, eval_graphs=None):
     import numpy as np
 
     from sparkx_amd import Graph, HyperBall, metrics
-    from sparkx_amd.surrogate import Surrogate, labelled_graph
+    from sparkx_amd.surrogate import Surrogate, labelled_graph, standardize, walk_features
     t0 = time.perf_counter()
     xs, ys = [], []
+    xws = {2: [], 3: []}
     for i in range(spec["train_graphs"]):
         g = Graph.rmat(spec["train_scale"], 16, seed=spec["seed"] * 1000 + i)
         x, y, _, _, _ = labelled_graph(g)
         linked = g.degrees() > 0
+        with HyperBall(g, 4, _run=False) as h:
+            for passes in xws:
+                xws[passes].append(standardize(walk_features(h, passes)[0], linked)[linked])
         xs.append(x[linked])
         ys.append(y[linked])
     x = np.concatenate(xs)
@@ -139,8 +143,14 @@ def bench_surrogate(args, spec, eval_graphs=None):
     t0 = time.perf_counter()
     losses = s.train(x, y, epochs=4, batch=8192, lr=1e-3, seed=spec["seed"])
     train_s = time.perf_counter() - t0
-    log(f"trained in {train_s:.1f}s, epoch losses {[round(float(v), 5) for v in losses]}")
-    del xs, ys, x, y
+    log(f"trained (snapshot inputs) in {train_s:.1f}s, epoch losses {[round(float(v), 5) for v in losses]}")
+    walk_nets, losses_w = {}, {}
+    for passes, parts in xws.items():
+        walk_nets[passes] = Surrogate(seed=spec["seed"])
+        losses_w[passes] = walk_nets[passes].train(np.concatenate(parts), y, epochs=4, batch=8192, lr=1e-3,
+                                                   seed=spec["seed"])
+        log(f"trained (walk inputs, {passes} passes), epoch losses {[round(float(v), 5) for v in losses_w[passes]]}")
+    del xs, xws, ys, x, y
     results = {}
     evals = eval_graphs or [("rmat24", CONFIGS["rmat24"][0], 7, 0), ("kmer", CONFIGS["kmer"][0], 6, 64)]
     for name, gspec, p_full, mi in evals:
@@ -154,6 +164,10 @@ def bench_surrogate(args, spec, eval_graphs=None):
             s.rank(h)  # warm
             pred, _, sur_ms = s.rank(h)
             trunc = h.harmonic_geodesic_distances()
+            walk_out = {}
+            for passes, net in walk_nets.items():
+                net.rank_walks(h, passes)  # warm
+                walk_out[passes] = net.rank_walks(h, passes)
         linked = g.degrees() > 0
         t0 = time.perf_counter()
         m_all = metrics.evaluate(pred, truth, 0.01)  # on the GPU, reference tie semantics
@@ -161,6 +175,15 @@ def bench_surrogate(args, spec, eval_graphs=None):
         b_lnk = metrics.evaluate(trunc[linked], truth[linked], 0.01)
         b_all = metrics.evaluate(trunc, truth, 0.01)
         eval_s = time.perf_counter() - t0
+        walks = {}
+        for passes, (pred_w, _, walk_ms) in walk_out.items():
+            w_all = metrics.evaluate(pred_w, truth, 0.01)
+            w_lnk = metrics.evaluate(pred_w[linked], truth[linked], 0.01)
+            walks[f"{passes}_passes"] = {
+                "surrogate_ms": round(walk_ms, 3), "time_ratio": round(walk_ms / full_ms, 4),
+                "spearman_all_nodes": round(w_all["spearman"], 5),
+                "spearman_linked_nodes": round(w_lnk["spearman"], 5),
+                "top1pct_overlap": round(w_all["top_overlap"], 5)}
         results[name] = {
             "nodes": g.size(), "arcs": g.width(), "full_p": p_full, "full_iterations": int(st.iterations),
             "full_ms": round(full_ms, 3), "surrogate_ms": round(sur_ms, 3),
@@ -171,11 +194,15 @@ def bench_surrogate(args, spec, eval_graphs=None):
             "baseline_4iter_spearman_linked": round(b_lnk["spearman"], 5),
             "baseline_4iter_top1pct_overlap": round(b_all["top_overlap"], 5),
             "evaluation_seconds_4_metric_sets": round(eval_s, 3),
+            "walks": walks,
         }
         log(f"surrogate on {name}: {results[name]}")
     s.close()
+    for net in walk_nets.values():
+        net.close()
     return {"train_graphs": spec["train_graphs"], "train_scale": spec["train_scale"],
             "train_seconds": round(train_s, 2), "epoch_losses": [round(float(v), 6) for v in losses],
+            "walks_epoch_losses": {f"{k}_passes": [round(float(v), 6) for v in w] for k, w in losses_w.items()},
             "eval": results}
 
 

@@ -264,6 +264,24 @@ int hb_mlp_predict(hb_mlp *m, const float *x, uint64_t count, float *y);
  * (host set-up excluded).  Replaces this context's previous run. */
 int hb_surrogate(hb_ctx *ctx, hb_mlp *m, int p_feat, float *pred, float *features, double *ms);
 
+/* Walk-count inputs (alternative input sets for the same MLP, no HyperBall
+ * iteration): S_0 = degree, S_k(u) the sum of S_{k-1}(v) over u's arcs u -> v (walks of
+ * k+1 arcs from u; u64, saturating at 2^64-1, so exact and order-free), M_1(u) the
+ * largest degree among u's heads (0 without arcs), L(x) = f64 log1p(x).  Per node, 6
+ * f32 in the caller's order, each rounded once from f64:
+ *   passes = 2: L(deg), L(S_1), L(S_2), L(M_1), L(S_2) - L(S_1), L(S_1) - L(deg)
+ *   passes = 3: L(deg), L(S_1), L(S_2), L(S_3), L(M_1), L(S_3) - L(S_2)
+ * (one gather pass per S_k).  ms = device time.  Leaves the context's HyperBall run in
+ * place.  No reference counterpart. */
+int hb_walk_features(hb_ctx *ctx, int passes, float *features, double *ms);
+
+/* Anytime ranking from walk-count inputs: hb_walk_features, standardised per graph as
+ * in hb_surrogate, then the MLP (which must have been trained on the same input set);
+ * pred and features (standardised, may be NULL) as in hb_surrogate; ms = device time
+ * of the passes, the inputs and inference.  Leaves the context's HyperBall run in
+ * place.  No reference counterpart. */
+int hb_surrogate_walks(hb_ctx *ctx, hb_mlp *m, int passes, float *pred, float *features, double *ms);
+
 /* ---- exact BFS centralities ---------------------------------------------------
  * One breadth-first search per source over the staged graph, 64 sources per pass
  * (one bit lane each), following arcs forwards as the reference's BFSDists does

@@ -13,6 +13,10 @@ repo root):
   mae/mape/spearman    src/utils.rs:57-100
   dbg_mtx_csr()        gen_dbg_mtx.py:4-22 + parser_mtx/producers.rs:228-234
                        + impl_sort.rs:10-51
+  walk_counts(), walk_inputs()
+                       no reference counterpart (inputs of the learned surrogate);
+                       pinned by brute-force walk enumeration, exact sparse powers
+                       A^k deg and a closed-form saturating case (tests/test_cpu_oracle.py)
 """
 from __future__ import annotations
 
@@ -363,3 +367,47 @@ def parse_mtx(text: bytes):
     if rc:
         raise MtxError(rc, info.err_line)
     return off, nb[:info.arcs], info
+
+
+U64_MAX = (1 << 64) - 1
+
+
+def walk_counts(offsets, nbrs):
+    """(degree, S_1, S_2, S_3, M_1) as uint64 arrays: S_0 = degree, S_k(u) = sum of
+    S_{k-1}(v) over u's arcs u -> v (walks of k + 1 arcs leaving u), saturating at
+    2^64 - 1 (min(2^64 - 1, exact sum)); M_1(u) = largest degree among u's heads, 0
+    for a node without arcs."""
+    off = np.asarray(offsets, np.int64)
+    nb = np.asarray(nbrs, np.int64)
+    n = off.size - 1
+    deg = np.diff(off).astype(np.uint64)
+    tails = np.repeat(np.arange(n), np.diff(off))
+    sums = []
+    s = deg
+    for _ in range(3):
+        x = s[nb]
+        # the sum modulo 2^64 from wrapping prefix sums; exact unless the true sum
+        # reaches 2^64, which the f64 bound flags (those rows are redone in Python ints)
+        cs = np.concatenate([np.zeros(1, np.uint64), np.cumsum(x, dtype=np.uint64)])
+        tot = cs[off[1:]] - cs[off[:-1]]
+        approx = np.bincount(tails, weights=x.astype(np.float64), minlength=n)
+        for u in np.nonzero(approx >= 2.0 ** 63)[0]:
+            exact = sum(int(t) for t in x[off[u]:off[u + 1]])
+            tot[u] = np.uint64(min(exact, U64_MAX))
+        sums.append(tot)
+        s = tot
+    m1 = np.zeros(n, np.uint64)
+    np.maximum.at(m1, tails, deg[nb])
+    return deg, sums[0], sums[1], sums[2], m1
+
+
+def walk_inputs(offsets, nbrs, passes: int = 3) -> np.ndarray:
+    """Surrogate inputs of every node, f32 [n, 6], from L = f64 log1p of the counts,
+    each value rounded to f32 once:
+      passes 2: L(deg), L(S_1), L(S_2), L(M_1), L(S_2) - L(S_1), L(S_1) - L(deg)
+      passes 3: L(deg), L(S_1), L(S_2), L(S_3), L(M_1), L(S_3) - L(S_2)"""
+    deg, s1, s2, s3, m1 = walk_counts(offsets, nbrs)
+    L = lambda v: np.log1p(v.astype(np.float64))  # noqa: E731
+    ld, l1, l2, l3, lm = L(deg), L(s1), L(s2), L(s3), L(m1)
+    cols = [ld, l1, l2, lm, l2 - l1, l1 - ld] if passes == 2 else [ld, l1, l2, l3, lm, l3 - l2]
+    return np.column_stack(cols).astype(np.float32)

@@ -31,6 +31,7 @@ EXPORTS = (
     "hb_batch_read_accumulators", "hb_autotune", "hb_batch_autotune",
     "hb_set_snapshots", "hb_read_snapshots", "hb_mlp_create", "hb_mlp_destroy", "hb_mlp_set_params",
     "hb_mlp_get_params", "hb_mlp_gradient", "hb_mlp_train", "hb_mlp_predict", "hb_surrogate",
+    "hb_walk_features", "hb_surrogate_walks",
     "hb_exact_bfs", "hb_exact_centrality", "hb_eval_ranks", "hb_eval_metrics", "hb_set_layout",
     "hb_stage_mtx", "hb_stage_coo", "hb_read_csr", "hb_run_many",
 )
@@ -167,6 +168,8 @@ def load() -> C.CDLL:
         "hb_mlp_train": (C.c_int, [vp, f32p, f32p, C.c_uint64, C.c_int, C.c_int, C.c_float, C.c_uint64, f32p]),
         "hb_mlp_predict": (C.c_int, [vp, f32p, C.c_uint64, f32p]),
         "hb_surrogate": (C.c_int, [vp, vp, C.c_int, f32p, f32p, C.POINTER(C.c_double)]),
+        "hb_walk_features": (C.c_int, [vp, C.c_int, f32p, C.POINTER(C.c_double)]),
+        "hb_surrogate_walks": (C.c_int, [vp, vp, C.c_int, f32p, f32p, C.POINTER(C.c_double)]),
         "hb_exact_bfs": (C.c_int, [vp, C.c_uint64, C.c_uint64, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                    f64p]),
         "hb_exact_centrality": (C.c_int, [vp, C.c_int, C.c_int, C.c_uint64, C.c_uint64, f64p]),

@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsparkx_hb.so")
 PROBE = os.path.join(HERE, "hbm_copy_probe")  # PMC calibration / streaming-bandwidth probe
 
-SOURCES = ["hb_kernels.hip", "hb_api.hip", "hb_batch.hip", "hb_mlp.hip", "hb_exact.hip", "hb_eval.hip", "hb_layout.hip", "hb_ingest.hip", "hb_graphgen.cpp"]
+SOURCES = ["hb_kernels.hip", "hb_api.hip", "hb_batch.hip", "hb_mlp.hip", "hb_exact.hip", "hb_eval.hip", "hb_layout.hip", "hb_ingest.hip", "hb_walk.hip", "hb_graphgen.cpp"]
 HEADERS = ["hb_device.h", "hb_kernels.h", "hb_union.h", "hll_bias_table.inc"]
 
 # -ffp-contract=off: the f64 accumulators and f32 estimates must not fuse a*b+c

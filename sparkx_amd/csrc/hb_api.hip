@@ -118,6 +118,8 @@ struct hb_ctx {
     uint32_t n_linked = 0;
     uint32_t snap_k = 0;
     DevBuf<float> snaps, feat, pred;
+    DevBuf<uint64_t> walk;      // walk-count passes: S_1, S_2, M_1, M_2, hub partials
+    DevBuf<uint32_t> walk_hub;  // hub chunk table of the walk passes, then the u32 degrees
 
     // node order on the device: HB_LAYOUT_DEGREE renumbers the nodes by descending
     // degree at the first run after staging; reads return the caller's order
@@ -170,6 +172,8 @@ struct hb_ctx {
         snaps.release();
         feat.release();
         pred.release();
+        walk.release();
+        walk_hub.release();
         readback.release();
         ran = false;
         sched_p = 0;
@@ -1070,6 +1074,21 @@ extern "C" int hb_read_snapshots(hb_ctx *c, float *dst)
     return HB_OK;
 }
 
+// nodes the surrogate network runs on: in the degree layout the nodes without a
+// neighbour are the last ones, and their output is fixed without it
+static uint32_t infer_nodes(const hb_ctx *c)
+{
+    if (!c->relabelled()) return (uint32_t)c->n;
+    const std::vector<uint32_t> &ho = c->run_hoff();
+    uint32_t lo = 0, hi = (uint32_t)c->n;  // first node of degree 0 (degrees descend)
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (ho[mid + 1] > ho[mid]) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
 extern "C" int hb_surrogate(hb_ctx *c, hb_mlp *mlp, int p_feat, float *pred, float *features, double *ms)
 {
     if (!c || !mlp) return fail(HB_EINVAL, "hb_surrogate: NULL argument");
@@ -1090,8 +1109,8 @@ extern "C" int hb_surrogate(hb_ctx *c, hb_mlp *mlp, int p_feat, float *pred, flo
     HB_TRY(hipEventCreate(&e0));
     HB_TRY(hipEventCreate(&e1));
     HB_TRY(hipEventRecord(e0, c->stream));
-    HB_TRY(hb::launch_surrogate_tail(mlp, c->run_off(), c->snaps.ptr, (uint32_t)c->n, c->feat.ptr, c->pred.ptr,
-                                     c->stream));
+    HB_TRY(hb::launch_surrogate_tail(mlp, c->run_off(), c->snaps.ptr, (uint32_t)c->n, infer_nodes(c), c->feat.ptr,
+                                     c->pred.ptr, c->stream));
     HB_TRY(hipEventRecord(e1, c->stream));
     HB_TRY(hipEventSynchronize(e1));
     float t = 0.0f;
@@ -1103,6 +1122,101 @@ extern "C" int hb_surrogate(hb_ctx *c, hb_mlp *mlp, int p_feat, float *pred, flo
         if (int rc2 = copy_out(c, c->pred.ptr, pred, 4)) return rc2;
     if (features)
         if (int rc2 = copy_out(c, c->feat.ptr, features, 24)) return rc2;
+    g_err.clear();
+    return HB_OK;
+}
+
+// walk-count inputs of every node into c->feat (device numbering): the hub chunk table
+// comes from the host offsets; *ms = device time of the passes and the inputs, plus
+// standardisation and inference when mlp is given (then c->pred holds the ranking)
+static int walk_surrogate(hb_ctx *c, hb_mlp *mlp, int passes, float *ms)
+{
+    const uint32_t n = (uint32_t)c->n;
+    const std::vector<uint32_t> &ho = c->run_hoff();
+    std::vector<uint32_t> beg, end, node, first;
+    for (uint32_t u = 0; u < n; ++u) {
+        if (ho[u + 1] - ho[u] <= hb::kWalkHub) continue;
+        node.push_back(u);
+        first.push_back((uint32_t)beg.size());
+        for (uint64_t b = ho[u]; b < ho[u + 1]; b += hb::kWalkHub) {
+            beg.push_back((uint32_t)b);
+            end.push_back((uint32_t)std::min<uint64_t>(b + hb::kWalkHub, ho[u + 1]));
+        }
+    }
+    first.push_back((uint32_t)beg.size());
+    hb::WalkHubs hubs;
+    hubs.chunks = (uint32_t)beg.size();
+    hubs.hubs = (uint32_t)node.size();
+    std::vector<uint32_t> table;
+    table.reserve(2 * beg.size() + 2 * node.size() + 1);
+    table.insert(table.end(), beg.begin(), beg.end());
+    table.insert(table.end(), end.begin(), end.end());
+    table.insert(table.end(), node.begin(), node.end());
+    table.insert(table.end(), first.begin(), first.end());
+    HB_TRY(c->walk_hub.ensure(table.size() + n));
+    HB_TRY(hipMemcpyAsync(c->walk_hub.ptr, table.data(), table.size() * 4, hipMemcpyHostToDevice, c->stream));
+    hubs.beg = c->walk_hub.ptr;
+    hubs.end = hubs.beg + hubs.chunks;
+    hubs.node = hubs.end + hubs.chunks;
+    hubs.first = hubs.node + hubs.hubs;
+    uint32_t *deg = c->walk_hub.ptr + table.size();
+    HB_TRY(c->walk.ensure(4 * (size_t)n + 2 * (size_t)hubs.chunks));
+    HB_TRY(c->feat.ensure((size_t)n * 6));
+    if (mlp) HB_TRY(c->pred.ensure(n));
+    hipEvent_t e0, e1;
+    HB_TRY(hipEventCreate(&e0));
+    HB_TRY(hipEventCreate(&e1));
+    hipError_t e = hipEventRecord(e0, c->stream);
+    if (e == hipSuccess)
+        e = hb::walk_inputs(c->run_off(), c->run_nbr(), n, hubs, passes, c->walk.ptr, deg,
+                            c->walk.ptr + 4 * (size_t)n, c->feat.ptr, c->stream);
+    if (e == hipSuccess && mlp)
+        e = hb::launch_surrogate_infer(mlp, c->run_off(), n, infer_nodes(c), c->feat.ptr, c->pred.ptr, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0.0f;
+    if (e == hipSuccess) (void)hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return fail(HB_EHIP, "walk-count surrogate: %s", hipGetErrorString(e));
+    if (ms) *ms = t;
+    return HB_OK;
+}
+
+extern "C" int hb_walk_features(hb_ctx *c, int passes, float *features, double *ms)
+{
+    if (!c) return fail(HB_EINVAL, "hb_walk_features: NULL argument");
+    if (passes != 2 && passes != 3) return fail(HB_EINVAL, "hb_walk_features: passes=%d (2 or 3)", passes);
+    if (!c->off.ptr) return fail(HB_ESTATE, "hb_walk_features: no graph staged");
+    if (!set_device(c)) return fail(HB_EHIP, "hipSetDevice failed");
+    if (c->n == 0) {
+        if (ms) *ms = 0.0;
+        return HB_OK;
+    }
+    float t = 0.0f;
+    if (int rc = walk_surrogate(c, nullptr, passes, &t)) return rc;
+    if (ms) *ms = t;
+    if (features)
+        if (int rc = copy_out(c, c->feat.ptr, features, 24)) return rc;
+    g_err.clear();
+    return HB_OK;
+}
+
+extern "C" int hb_surrogate_walks(hb_ctx *c, hb_mlp *mlp, int passes, float *pred, float *features, double *ms)
+{
+    if (!c || !mlp) return fail(HB_EINVAL, "hb_surrogate_walks: NULL argument");
+    if (passes != 2 && passes != 3) return fail(HB_EINVAL, "hb_surrogate_walks: passes=%d (2 or 3)", passes);
+    if (!c->off.ptr || c->n == 0) return fail(HB_ESTATE, "hb_surrogate_walks: no graph staged");
+    if (hb::mlp_device(mlp) != c->device)
+        return fail(HB_EINVAL, "hb_surrogate_walks: model and graph on different devices");
+    if (!set_device(c)) return fail(HB_EHIP, "hipSetDevice failed");
+    float t = 0.0f;
+    if (int rc = walk_surrogate(c, mlp, passes, &t)) return rc;
+    if (ms) *ms = t;
+    if (pred)
+        if (int rc = copy_out(c, c->pred.ptr, pred, 4)) return rc;
+    if (features)
+        if (int rc = copy_out(c, c->feat.ptr, features, 24)) return rc;
     g_err.clear();
     return HB_OK;
 }

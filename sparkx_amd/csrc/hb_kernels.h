@@ -75,9 +75,28 @@ hipError_t coo_to_csr(const uint32_t *src, const uint32_t *dst, uint64_t m, uint
                       CsrBuild *out, hipStream_t s);
 }  // namespace ingest
 
-// surrogate tail on a staged graph: features from snapshots, MLP inference
+// Walk-count surrogate inputs (hb_walk.hip).  Lists longer than kWalkHub arcs are cut
+// into kWalkHub-arc chunks: chunk c = arcs [beg[c], end[c]); hub h = node[h] owns
+// chunks [first[h], first[h + 1]) (device arrays).
+constexpr uint32_t kWalkHub = 4096;
+struct WalkHubs {
+    const uint32_t *beg = nullptr, *end = nullptr, *node = nullptr, *first = nullptr;
+    uint32_t chunks = 0, hubs = 0;
+};
+// passes 2 or 3; s: 4 n u64 (S_1, S_2, S_3, M_1, each n long); deg: n u32; part:
+// 2 * chunks u64; x: n * 6 f32 (the inputs listed in hb_walk_features)
+hipError_t walk_inputs(const uint32_t *off, const uint32_t *nbr, uint32_t n, const WalkHubs &hubs, int passes,
+                       uint64_t *s, uint32_t *deg, uint64_t *part, float *x, hipStream_t st);
+// per-graph standardisation of inputs x, MLP inference and isolated-last ranking (the
+// surrogate tail after its inputs); nodes >= infer_nodes have no neighbour (their
+// output is the isolated mark without running the network)
+hipError_t launch_surrogate_infer(struct ::hb_mlp *n, const uint32_t *off, uint32_t nodes, uint32_t infer_nodes,
+                                  float *x_dev, float *out_dev, hipStream_t s);
+
+// surrogate tail on a staged graph: features from snapshots, MLP inference (nodes
+// >= infer_nodes have no neighbour)
 hipError_t launch_surrogate_tail(struct ::hb_mlp *n, const uint32_t *off, const float *snaps, uint32_t nodes,
-                                 float *x_dev, float *out_dev, hipStream_t s);
+                                 uint32_t infer_nodes, float *x_dev, float *out_dev, hipStream_t s);
 int mlp_device(const struct ::hb_mlp *n);
 // exact BFS sums of sources [s_begin, s_end) over a staged CSR (hb_exact.hip);
 // symmetric: the CSR holds v -> u for every u -> v (levels may then pull)

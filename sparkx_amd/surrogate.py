@@ -1,15 +1,22 @@
 """Learned surrogate for anytime centrality rankings.
 
 An MLP 6 -> 64 -> 64 -> 1 (GELU; BF16 MFMA matmuls with f32 accumulation; f32
-master weights and Adam) maps each node's degree, log(1 + degree) and log N_1..N_4
--- N_t its HyperBall estimate after t iterations -- to its log harmonic
-centrality log(1 + H), H the full run's inverse-distance accumulator
-(hyper_ball.rs:824-825).  Inputs and target are standardised per graph (mean and
-standard deviation over the nodes that have a neighbour), so that a network trained
-on small graphs ranks the nodes of a larger one: the prediction is the node's
-standardised log harmonic centrality, which orders the graph's nodes.  Training,
-inference and the inputs run on the GPU (include/sparkx_hb.h, hb_mlp_*,
-hb_surrogate)."""
+master weights and Adam) maps six inputs per node to its log harmonic centrality
+log(1 + H), H the full run's inverse-distance accumulator (hyper_ball.rs:824-825).
+Two input sets:
+
+* snapshots: degree, log(1 + degree) and log N_1..N_4, N_t the node's HyperBall
+  estimate after t iterations (hb_surrogate);
+* walks: logs of S_1..S_passes (S_k the number of walks of k + 1 arcs leaving the
+  node), of its degree and of its largest neighbour degree, with differences of
+  them -- two or three gather passes, no HyperBall iteration (hb_surrogate_walks;
+  the sets are listed in include/sparkx_hb.h).
+
+Inputs and target are standardised per graph (mean and standard deviation over the
+nodes that have a neighbour), so that a network trained on small graphs ranks the
+nodes of a larger one: the prediction is the node's standardised log harmonic
+centrality, which orders the graph's nodes.  Training, inference and the inputs run
+on the GPU (include/sparkx_hb.h, hb_mlp_*, hb_surrogate*, hb_walk_features)."""
 from __future__ import annotations
 
 import ctypes as C
@@ -102,6 +109,20 @@ class Surrogate:
                                         _lib.ptr(feat, C.c_float) if feat is not None else None, C.byref(ms)))
         return pred, feat, ms.value
 
+    def rank_walks(self, h: HyperBall, passes: int = 3, want_features: bool = False):
+        """Anytime ranking of the graph staged in `h` from walk-count inputs of
+        `passes` gather passes (the network must be trained on the same set; its
+        HyperBall run, if any, is kept): (prediction as in rank(); standardised inputs
+        or None; device ms)."""
+        n = h.g.size()
+        pred = np.empty(n, np.float32)
+        feat = np.empty((n, 6), np.float32) if want_features else None
+        ms = C.c_double()
+        _lib.check(self._L.hb_surrogate_walks(h._ctx, self._m, passes, _lib.ptr(pred, C.c_float),
+                                              _lib.ptr(feat, C.c_float) if feat is not None else None,
+                                              C.byref(ms)))
+        return pred, feat, ms.value
+
     def close(self):
         if getattr(self, "_m", None):
             self._L.hb_mlp_destroy(self._m)
@@ -112,6 +133,27 @@ class Surrogate:
             self.close()
         except Exception:
             pass
+
+
+def walk_features(h: HyperBall, passes: int = 3):
+    """Unstandardised walk-count inputs of the graph staged in `h` (f32 [n, 6], the
+    caller's node order) and the device ms of the passes."""
+    x = np.empty((h.g.size(), 6), np.float32)
+    ms = C.c_double()
+    _lib.check(h._L.hb_walk_features(h._ctx, passes, _lib.ptr(x, C.c_float), C.byref(ms)))
+    return x, ms.value
+
+
+def labelled_walks(g: Graph, passes: int = 3, p_label: int = 7, device: int = 0):
+    """(standardised walk-count inputs, target, full-run ms, H) of one graph, the
+    target as in labelled_graph."""
+    with HyperBall(g, p_label, device=device) as full:
+        inv = full.harmonic_geodesic_distances()
+        ms_full = full.stats.ms_total + full.stats.ms_init
+        x, _ = walk_features(full, passes)
+    linked = g.degrees() > 0
+    y = standardize(np.log1p(np.asarray(inv, np.float64))[:, None], linked)[:, 0]
+    return standardize(x, linked), y, ms_full, inv
 
 
 def labelled_graph(g: Graph, p_label: int = 7, p_feat: int = 4, device: int = 0):

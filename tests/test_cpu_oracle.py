@@ -452,3 +452,78 @@ def test_oracle_top_overlap(oracle):
     v = np.random.default_rng(2).normal(size=1000)
     assert oracle.top_overlap(v, v, 0.013) == 1.0
     assert oracle.top_overlap(v, -v, 0.5) == 0.0
+
+
+# ---------------------------------------------------------------------------
+# walk counts (surrogate inputs): brute force, exact sparse powers, saturation
+# ---------------------------------------------------------------------------
+def _random_multigraph(rng, n, m):
+    src = rng.integers(0, n, m)
+    dst = rng.integers(0, n, m)  # self-loops and repeated arcs included
+    order = np.argsort(src, kind="stable")
+    off = np.zeros(n + 1, np.uint64)
+    off[1:] = np.cumsum(np.bincount(src, minlength=n))
+    return off, dst[order].astype(np.uint32)
+
+
+def test_walk_counts_brute_force(oracle):
+    """S_k(u) counts the walks u -> ... of k + 1 arcs (each arc of a multigraph a
+    separate walk step): enumerate them on small multigraphs with self-loops."""
+    rng = np.random.default_rng(3)
+    for n, m in ((1, 0), (5, 9), (12, 30), (9, 40)):
+        off, nb = _random_multigraph(rng, n, m)
+        adj = [[int(v) for v in nb[int(off[u]):int(off[u + 1])]] for u in range(n)]
+
+        def walks(u, length):
+            return 1 if length == 0 else sum(walks(v, length - 1) for v in adj[u])
+
+        deg, s1, s2, s3, m1 = oracle.walk_counts(off, nb)
+        for u in range(n):
+            assert int(deg[u]) == walks(u, 1)
+            assert int(s1[u]) == walks(u, 2)
+            assert int(s2[u]) == walks(u, 3)
+            assert int(s3[u]) == walks(u, 4)
+            assert int(m1[u]) == max((walks(v, 1) for v in adj[u]), default=0)
+
+
+def test_walk_counts_match_sparse_powers(oracle):
+    """S_k = A^k deg on an R-MAT graph (integer sparse products, far below 2^63);
+    M_1 the row maxima of A diag(deg)."""
+    from scipy.sparse import csr_matrix
+
+    from sparkx_amd import Graph
+    g = Graph.rmat(12, 8, seed=4)
+    n = g.size()
+    off = g.offsets.astype(np.int64)
+    A = csr_matrix((np.ones(g.width(), np.int64), g.neighbours.astype(np.int64), off), shape=(n, n))
+    deg = np.diff(off)
+    got = oracle.walk_counts(g.offsets, g.neighbours)
+    v = deg
+    for k in range(1, 4):
+        v = A @ v
+        assert v.max() < 2 ** 62
+        assert np.array_equal(got[k].astype(np.int64), v), k
+    m = A.multiply(deg[None, :]).max(axis=1).toarray()[:, 0]
+    assert np.array_equal(got[4].astype(np.int64), m)
+    assert np.array_equal(got[0].astype(np.int64), deg)
+    for passes in (2, 3):  # the inputs are f64 log1p values of these counts
+        x = oracle.walk_inputs(g.offsets, g.neighbours, passes)
+        ld, l1, l2 = (np.log1p(got[k].astype(np.float64)) for k in range(3))
+        assert np.array_equal(x[:, 0], ld.astype(np.float32))
+        assert np.array_equal(x[:, 2], l2.astype(np.float32))
+        last = (l1 - ld) if passes == 2 else (np.log1p(got[3].astype(np.float64)) - l2)
+        assert np.array_equal(x[:, 5], last.astype(np.float32))
+
+
+def test_walk_counts_saturate():
+    """Two nodes joined by d parallel arcs each way: S_k = d^(k+1), clipped at 2^64 - 1
+    (d = 2.7e6: S_2 = 1.97e19 and S_3 saturate, S_1 = 7.29e12 does not); M_1 = d."""
+    from oracle import hb_oracle as O
+    d = 2_700_000
+    off = np.array([0, d, 2 * d], np.uint64)
+    nb = np.concatenate([np.ones(d, np.uint32), np.zeros(d, np.uint32)])
+    deg, s1, s2, s3, m1 = O.walk_counts(off, nb)
+    assert [int(x) for x in (deg[0], s1[0], s2[0], s3[0], m1[0])] == [d, d ** 2, O.U64_MAX, O.U64_MAX, d]
+    x = O.walk_inputs(off, nb, 3)
+    assert x[0, 3] == np.float32(np.log1p(float(O.U64_MAX))) and x[0, 5] == 0.0
+    assert x[1, 1] == np.float32(math.log1p(d ** 2))
