@@ -36,6 +36,7 @@ CONFIGS = {
     "exact": (dict(kind="exact", scale=20, edge_factor=16, seed=1, sources=2048), 0, 0),
     "ingest": (dict(kind="ingest", scale=22, edge_factor=16, seed=1), 0, 0),
     "many64": (dict(kind="many", count=64, seed=1, threads=8), 0, 0),
+    "walks": (dict(kind="walks", scale=24, edge_factor=16, seed=1), 7, 0),
 }
 
 
@@ -266,6 +267,36 @@ def bench_batch(args, spec):
     print(json.dumps(out), flush=True)
 
 
+def bench_walks(args, spec):
+    """Device time of the walk-count surrogate (hb_surrogate_walks: the gather passes,
+    the inputs, standardisation, inference) on R-MAT-24 for 2 and 3 passes, against
+    the full p=7 run (register init + iterations).  The network's weights do not change
+    the work, so it is left at its seeded initialisation; rankings are measured by
+    --config surrogate."""
+    from sparkx_amd import HyperBall
+    from sparkx_amd.surrogate import Surrogate
+    g = make_graph(dict(kind="rmat", scale=spec["scale"], edge_factor=spec["edge_factor"], seed=spec["seed"]),
+                   args.graph_cache)
+    log(f"walks: R-MAT {spec['scale']} n={g.size()} arcs={g.width()}")
+    with HyperBall(g, 7) as full:
+        st = full.run()
+        full_ms = st.ms_total + st.ms_init
+    net = Surrogate(seed=spec["seed"])
+    out = {"metric": "walk-count surrogate device ms (R-MAT-24)", "unit": "ms", "higher_is_better": False,
+           "full_run_ms": round(full_ms, 3), "config": {"workload": "walks", **spec, "nodes": g.size(),
+                                                       "arcs": g.width()}}
+    with HyperBall(g, 4, _run=False) as h:
+        for passes in (2, 3):
+            for _ in range(args.warmup):
+                net.rank_walks(h, passes)
+            ms = [net.rank_walks(h, passes)[2] for _ in range(args.steps)]
+            avg = sum(ms) / len(ms)
+            log(f"{passes} passes: {[round(v, 3) for v in ms]} ms")
+            out[f"{passes}_passes"] = {"ms": round(avg, 3), "time_ratio": round(avg / full_ms, 4)}
+    net.close()
+    print(json.dumps(out), flush=True)
+
+
 def bench_many(args, spec):
     """The same 64 graphs (32 ER + 32 R-MAT, n log-uniform in [2^12, 2^18], average degree
     U[4, 32], p uniform in {4..8}) run to convergence together: one engine context and
@@ -461,6 +492,8 @@ def main():
         return bench_ingest(args, spec)
     if spec["kind"] == "many":
         return bench_many(args, spec)
+    if spec["kind"] == "walks":
+        return bench_walks(args, spec)
     if spec["kind"] == "surrogate":
         print(json.dumps(bench_surrogate(args, spec)), flush=True)
         return None
