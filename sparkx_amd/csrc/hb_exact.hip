@@ -405,8 +405,10 @@ int exact_bfs(const uint32_t *off, const uint32_t *nbr, uint32_t n, bool symmetr
                                    : 1;
     HBX_TRY(hipMalloc(&bits.p, (size_t)3 * n * 8));
     HBX_TRY(hipMalloc(&cnt.p, (size_t)kBfsCols * (1 + grid) * 8));  // totals, then kBfsCols per block
-    // hubs: nodes of degree > kHubDeg, their lists cut into chunks of kHubDeg arcs
-    constexpr uint32_t kHubDeg = 4096;
+    // hubs: nodes of degree > kHubDeg, their lists cut into chunks of kHubChunk arcs
+    // (one wavefront each).  A wavefront walks the lists of degree 65..kHubDeg of its
+    // 64 nodes one after another, so that range is kept short.
+    constexpr uint32_t kHubDeg = 128, kHubChunk = 1024;
     std::vector<uint32_t> h_off((size_t)n + 1);
     HBX_TRY(hipMemcpy(h_off.data(), off, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost));
     const uint32_t arcs = h_off[n];
@@ -415,10 +417,10 @@ int exact_bfs(const uint32_t *off, const uint32_t *nbr, uint32_t n, bool symmetr
         std::vector<uint32_t> hn, hb_, he;
         for (uint32_t v = 0; v < n; ++v)
             if (h_off[v + 1] - h_off[v] > kHubDeg)
-                for (uint64_t b = h_off[v]; b < h_off[v + 1]; b += kHubDeg) {
+                for (uint64_t b = h_off[v]; b < h_off[v + 1]; b += kHubChunk) {
                     hn.push_back(v);
                     hb_.push_back((uint32_t)b);
-                    he.push_back((uint32_t)std::min<uint64_t>(b + kHubDeg, h_off[v + 1]));
+                    he.push_back((uint32_t)std::min<uint64_t>(b + kHubChunk, h_off[v + 1]));
                 }
         h_hub = hn;
         h_hub.insert(h_hub.end(), hb_.begin(), hb_.end());
