@@ -118,8 +118,9 @@ struct hb_ctx {
     uint32_t n_linked = 0;
     uint32_t snap_k = 0;
     DevBuf<float> snaps, feat, pred;
-    DevBuf<uint64_t> walk;      // walk-count passes: S_1, S_2, M_1, M_2, hub partials
+    DevBuf<uint64_t> walk;      // walk-count passes: S_1, S_2, S_3, M_1, hub partials
     DevBuf<uint32_t> walk_hub;  // hub chunk table of the walk passes, then the u32 degrees
+    uint32_t walk_chunks = 0, walk_hubs = 0;  // of the table in walk_hub (valid while allocated)
 
     // node order on the device: HB_LAYOUT_DEGREE renumbers the nodes by descending
     // degree at the first run after staging; reads return the caller's order
@@ -1132,34 +1133,39 @@ extern "C" int hb_surrogate(hb_ctx *c, hb_mlp *mlp, int p_feat, float *pred, flo
 static int walk_surrogate(hb_ctx *c, hb_mlp *mlp, int passes, float *ms)
 {
     const uint32_t n = (uint32_t)c->n;
-    const std::vector<uint32_t> &ho = c->run_hoff();
-    std::vector<uint32_t> beg, end, node, first;
-    for (uint32_t u = 0; u < n; ++u) {
-        if (ho[u + 1] - ho[u] <= hb::kWalkHub) continue;
-        node.push_back(u);
-        first.push_back((uint32_t)beg.size());
-        for (uint64_t b = ho[u]; b < ho[u + 1]; b += hb::kWalkHub) {
-            beg.push_back((uint32_t)b);
-            end.push_back((uint32_t)std::min<uint64_t>(b + hb::kWalkHub, ho[u + 1]));
+    if (!c->walk_hub.ptr) {  // built once per staged graph and numbering (release_run drops it)
+        const std::vector<uint32_t> &ho = c->run_hoff();
+        std::vector<uint32_t> beg, end, node, first;
+        for (uint32_t u = 0; u < n; ++u) {
+            if (ho[u + 1] - ho[u] <= hb::kWalkHub) continue;
+            node.push_back(u);
+            first.push_back((uint32_t)beg.size());
+            for (uint64_t b = ho[u]; b < ho[u + 1]; b += hb::kWalkHub) {
+                beg.push_back((uint32_t)b);
+                end.push_back((uint32_t)std::min<uint64_t>(b + hb::kWalkHub, ho[u + 1]));
+            }
         }
+        first.push_back((uint32_t)beg.size());
+        std::vector<uint32_t> table;
+        table.reserve(2 * beg.size() + 2 * node.size() + 1);
+        table.insert(table.end(), beg.begin(), beg.end());
+        table.insert(table.end(), end.begin(), end.end());
+        table.insert(table.end(), node.begin(), node.end());
+        table.insert(table.end(), first.begin(), first.end());
+        HB_TRY(c->walk_hub.ensure(table.size() + n));
+        HB_TRY(hipMemcpyAsync(c->walk_hub.ptr, table.data(), table.size() * 4, hipMemcpyHostToDevice, c->stream));
+        HB_TRY(hipStreamSynchronize(c->stream));  // `table` is a pageable host buffer
+        c->walk_chunks = (uint32_t)beg.size();
+        c->walk_hubs = (uint32_t)node.size();
     }
-    first.push_back((uint32_t)beg.size());
     hb::WalkHubs hubs;
-    hubs.chunks = (uint32_t)beg.size();
-    hubs.hubs = (uint32_t)node.size();
-    std::vector<uint32_t> table;
-    table.reserve(2 * beg.size() + 2 * node.size() + 1);
-    table.insert(table.end(), beg.begin(), beg.end());
-    table.insert(table.end(), end.begin(), end.end());
-    table.insert(table.end(), node.begin(), node.end());
-    table.insert(table.end(), first.begin(), first.end());
-    HB_TRY(c->walk_hub.ensure(table.size() + n));
-    HB_TRY(hipMemcpyAsync(c->walk_hub.ptr, table.data(), table.size() * 4, hipMemcpyHostToDevice, c->stream));
+    hubs.chunks = c->walk_chunks;
+    hubs.hubs = c->walk_hubs;
     hubs.beg = c->walk_hub.ptr;
     hubs.end = hubs.beg + hubs.chunks;
     hubs.node = hubs.end + hubs.chunks;
     hubs.first = hubs.node + hubs.hubs;
-    uint32_t *deg = c->walk_hub.ptr + table.size();
+    uint32_t *deg = c->walk_hub.ptr + 2 * (size_t)hubs.chunks + 2 * (size_t)hubs.hubs + 1;
     HB_TRY(c->walk.ensure(4 * (size_t)n + 2 * (size_t)hubs.chunks));
     HB_TRY(c->feat.ensure((size_t)n * 6));
     if (mlp) HB_TRY(c->pred.ensure(n));

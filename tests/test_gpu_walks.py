@@ -64,12 +64,18 @@ def test_walk_features_match_oracle(hb, oracle, layout, passes):
     for name, g in _graphs(hb):
         say("walk features", name, layout, passes, g.size(), g.width())
         want = oracle.walk_inputs(g.offsets, g.neighbours, passes)
+        logs = [k for k in range(6) if k not in diff_cols]
         with hb.HyperBall(g, 4, _run=False, layout=layout) as h:
             got, ms = walk_features(h, passes)
-        assert ms >= 0
-        logs = [k for k in range(6) if k not in diff_cols]
-        np.testing.assert_array_max_ulp(got[:, logs], want[:, logs], maxulp=1)
-        np.testing.assert_allclose(got[:, diff_cols], want[:, diff_cols], rtol=1e-6, atol=1e-9)
+            assert ms >= 0
+            np.testing.assert_array_max_ulp(got[:, logs], want[:, logs], maxulp=1)
+            np.testing.assert_allclose(got[:, diff_cols], want[:, diff_cols], rtol=1e-6, atol=1e-9)
+            # again on the same context (its hub table kept), then in the other numbering
+            assert np.array_equal(walk_features(h, passes)[0], got)
+            h.set_layout("input" if layout == "degree" else "degree")
+            other, _ = walk_features(h, passes)
+            np.testing.assert_array_max_ulp(other[:, logs], want[:, logs], maxulp=1)
+            np.testing.assert_allclose(other[:, diff_cols], want[:, diff_cols], rtol=1e-6, atol=1e-9)
 
 
 def test_walk_features_empty_graph(hb):
