@@ -946,8 +946,8 @@ extern "C" int hb_exact_bfs(hb_ctx *c, uint64_t src_begin, uint64_t src_end, uin
         return fail(HB_EINVAL, "hb_exact_bfs: sources [%llu, %llu) outside [0, %llu)", (unsigned long long)src_begin,
                     (unsigned long long)src_end, (unsigned long long)c->n);
     if (!set_device(c)) return fail(HB_EHIP, "hipSetDevice failed");
-    const int rc = hb::exact_bfs(c->off.ptr, c->nbr.ptr, (uint32_t)c->n, c->sym, src_begin, src_end, reach, total,
-                                 inv_sum, c->stream);
+    const int rc = hb::exact_bfs(c->off.ptr, c->h_off.data(), c->nbr.ptr, (uint32_t)c->n, c->sym, src_begin, src_end,
+                                 reach, total, inv_sum, c->stream);
     if (rc == HB_OK) g_err.clear();
     return rc;
 }

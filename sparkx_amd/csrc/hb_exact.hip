@@ -388,7 +388,8 @@ struct Scoped {
 
 namespace hb {
 
-int exact_bfs(const uint32_t *off, const uint32_t *nbr, uint32_t n, bool symmetric, uint64_t s_begin,
+int exact_bfs(const uint32_t *off, const uint32_t *h_off, const uint32_t *nbr, uint32_t n, bool symmetric,
+              uint64_t s_begin,
               uint64_t s_end, uint64_t *reach, uint64_t *total, double *inv_sum, hipStream_t s)
 {
     if (s_begin > s_end || s_end > n) return efail(HB_EINVAL, "hb_exact_bfs: sources [%llu, %llu) outside [0, %u)",
@@ -409,8 +410,6 @@ int exact_bfs(const uint32_t *off, const uint32_t *nbr, uint32_t n, bool symmetr
     // (one wavefront each).  A wavefront walks the lists of degree 65..kHubDeg of its
     // 64 nodes one after another, so that range is kept short.
     constexpr uint32_t kHubDeg = 128, kHubChunk = 1024;
-    std::vector<uint32_t> h_off((size_t)n + 1);
-    HBX_TRY(hipMemcpy(h_off.data(), off, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost));
     const uint32_t arcs = h_off[n];
     std::vector<uint32_t> h_hub;  // node, beg, end per chunk, as three runs
     {

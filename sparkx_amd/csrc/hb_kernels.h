@@ -98,9 +98,11 @@ hipError_t launch_surrogate_infer(struct ::hb_mlp *n, const uint32_t *off, uint3
 hipError_t launch_surrogate_tail(struct ::hb_mlp *n, const uint32_t *off, const float *snaps, uint32_t nodes,
                                  uint32_t infer_nodes, float *x_dev, float *out_dev, hipStream_t s);
 int mlp_device(const struct ::hb_mlp *n);
-// exact BFS sums of sources [s_begin, s_end) over a staged CSR (hb_exact.hip);
-// symmetric: the CSR holds v -> u for every u -> v (levels may then pull)
-int exact_bfs(const uint32_t *off, const uint32_t *nbr, uint32_t n, bool symmetric, uint64_t s_begin,
+// exact BFS sums of sources [s_begin, s_end) over a staged CSR (hb_exact.hip), h_off
+// the host copy of its offsets; symmetric: the CSR holds v -> u for every u -> v (levels
+// may then pull)
+int exact_bfs(const uint32_t *off, const uint32_t *h_off, const uint32_t *nbr, uint32_t n, bool symmetric,
+              uint64_t s_begin,
               uint64_t s_end, uint64_t *reach, uint64_t *total, double *inv_sum, hipStream_t s);
 
 }  // namespace hb
