@@ -245,12 +245,20 @@ __global__ __launch_bounds__(kBfsBlock) void k_bfs_apply(const uint32_t *__restr
                 if (newly) {
                     visited[v] = vis | newly;
                     farcs += off[v + 1] - off[v];
-                    if (dist) {
-                        uint64_t b = newly;
-                        while (b) {
-                            const int k = __ffsll((unsigned long long)b) - 1;
-                            dist[(size_t)v * 64 + k] = (uint16_t)level;
-                            b &= b - 1;
+                    if (dist) {  // the row's 16-byte pieces holding newly set bits, read-modify-write
+                        uint4 *row = (uint4 *)(dist + (size_t)v * 64);
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) {
+                            const uint32_t bits = (uint32_t)(newly >> (8 * j)) & 0xffu;
+                            if (!bits) continue;
+                            uint4 w = row[j];
+                            uint32_t q[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+                            for (int h = 0; h < 8; ++h) {
+                                const uint32_t sh = 16u * (uint32_t)(h & 1);
+                                if ((bits >> h) & 1u) q[h >> 1] = (q[h >> 1] & ~(0xffffu << sh)) | (level << sh);
+                            }
+                            row[j] = make_uint4(q[0], q[1], q[2], q[3]);
                         }
                     }
                 }
