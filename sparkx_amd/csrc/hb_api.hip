@@ -1110,8 +1110,8 @@ extern "C" int hb_surrogate(hb_ctx *c, hb_mlp *mlp, int p_feat, float *pred, flo
     HB_TRY(hipEventCreate(&e0));
     HB_TRY(hipEventCreate(&e1));
     HB_TRY(hipEventRecord(e0, c->stream));
-    HB_TRY(hb::launch_surrogate_tail(mlp, c->run_off(), c->snaps.ptr, (uint32_t)c->n, infer_nodes(c), c->feat.ptr,
-                                     c->pred.ptr, c->stream));
+    HB_TRY(hb::launch_surrogate_tail(mlp, c->run_off(), c->snaps.ptr, (uint32_t)c->n, infer_nodes(c),
+                                     features != nullptr, c->feat.ptr, c->pred.ptr, c->stream));
     HB_TRY(hipEventRecord(e1, c->stream));
     HB_TRY(hipEventSynchronize(e1));
     float t = 0.0f;
@@ -1130,7 +1130,8 @@ extern "C" int hb_surrogate(hb_ctx *c, hb_mlp *mlp, int p_feat, float *pred, flo
 // walk-count inputs of every node into c->feat (device numbering): the hub chunk table
 // comes from the host offsets; *ms = device time of the passes and the inputs, plus
 // standardisation and inference when mlp is given (then c->pred holds the ranking)
-static int walk_surrogate(hb_ctx *c, hb_mlp *mlp, int passes, float *ms)
+// want_x: the inputs of every node are read back (raw, or standardised when mlp is given)
+static int walk_surrogate(hb_ctx *c, hb_mlp *mlp, int passes, bool want_x, float *ms)
 {
     const uint32_t n = (uint32_t)c->n;
     if (!c->walk_hub.ptr) {  // built once per staged graph and numbering (release_run drops it)
@@ -1174,10 +1175,14 @@ static int walk_surrogate(hb_ctx *c, hb_mlp *mlp, int passes, float *ms)
     HB_TRY(hipEventCreate(&e1));
     hipError_t e = hipEventRecord(e0, c->stream);
     if (e == hipSuccess)
+        // without a read-back only the nodes the network runs on need inputs (the
+        // degree layout puts the nodes without a neighbour last)
         e = hb::walk_inputs(c->run_off(), c->run_nbr(), n, hubs, passes, c->walk.ptr, deg,
-                            c->walk.ptr + 4 * (size_t)n, c->feat.ptr, c->stream);
+                            c->walk.ptr + 4 * (size_t)n, c->feat.ptr, want_x || !mlp ? n : infer_nodes(c),
+                            c->stream);
     if (e == hipSuccess && mlp)
-        e = hb::launch_surrogate_infer(mlp, c->run_off(), n, infer_nodes(c), c->feat.ptr, c->pred.ptr, c->stream);
+        e = hb::launch_surrogate_infer(mlp, c->run_off(), n, infer_nodes(c), want_x, c->feat.ptr, c->pred.ptr,
+                                       c->stream);
     if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
     if (e == hipSuccess) e = hipEventSynchronize(e1);
     float t = 0.0f;
@@ -1200,7 +1205,7 @@ extern "C" int hb_walk_features(hb_ctx *c, int passes, float *features, double *
         return HB_OK;
     }
     float t = 0.0f;
-    if (int rc = walk_surrogate(c, nullptr, passes, &t)) return rc;
+    if (int rc = walk_surrogate(c, nullptr, passes, features != nullptr, &t)) return rc;
     if (ms) *ms = t;
     if (features)
         if (int rc = copy_out(c, c->feat.ptr, features, 24)) return rc;
@@ -1217,7 +1222,7 @@ extern "C" int hb_surrogate_walks(hb_ctx *c, hb_mlp *mlp, int passes, float *pre
         return fail(HB_EINVAL, "hb_surrogate_walks: model and graph on different devices");
     if (!set_device(c)) return fail(HB_EHIP, "hipSetDevice failed");
     float t = 0.0f;
-    if (int rc = walk_surrogate(c, mlp, passes, &t)) return rc;
+    if (int rc = walk_surrogate(c, mlp, passes, features != nullptr, &t)) return rc;
     if (ms) *ms = t;
     if (pred)
         if (int rc = copy_out(c, c->pred.ptr, pred, 4)) return rc;

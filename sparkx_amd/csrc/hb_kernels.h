@@ -84,19 +84,22 @@ struct WalkHubs {
     uint32_t chunks = 0, hubs = 0;
 };
 // passes 2 or 3; s: 4 n u64 (S_1, S_2, S_3, M_1, each n long); deg: n u32; part:
-// 2 * chunks u64; x: n * 6 f32 (the inputs listed in hb_walk_features)
+// 2 * chunks u64; x: n * 6 f32 (the inputs listed in hb_walk_features), written for
+// nodes < x_nodes
 hipError_t walk_inputs(const uint32_t *off, const uint32_t *nbr, uint32_t n, const WalkHubs &hubs, int passes,
-                       uint64_t *s, uint32_t *deg, uint64_t *part, float *x, hipStream_t st);
+                       uint64_t *s, uint32_t *deg, uint64_t *part, float *x, uint32_t x_nodes, hipStream_t st);
 // per-graph standardisation of inputs x, MLP inference and isolated-last ranking (the
 // surrogate tail after its inputs); nodes >= infer_nodes have no neighbour (their
-// output is the isolated mark without running the network)
+// output is the isolated mark without running the network) and their rows of x are
+// only read when standardized_x asks for the standardised inputs back in x
 hipError_t launch_surrogate_infer(struct ::hb_mlp *n, const uint32_t *off, uint32_t nodes, uint32_t infer_nodes,
-                                  float *x_dev, float *out_dev, hipStream_t s);
+                                  bool standardized_x, float *x_dev, float *out_dev, hipStream_t s);
 
 // surrogate tail on a staged graph: features from snapshots, MLP inference (nodes
 // >= infer_nodes have no neighbour)
 hipError_t launch_surrogate_tail(struct ::hb_mlp *n, const uint32_t *off, const float *snaps, uint32_t nodes,
-                                 uint32_t infer_nodes, float *x_dev, float *out_dev, hipStream_t s);
+                                 uint32_t infer_nodes, bool standardized_x, float *x_dev, float *out_dev,
+                                 hipStream_t s);
 int mlp_device(const struct ::hb_mlp *n);
 // exact BFS sums of sources [s_begin, s_end) over a staged CSR (hb_exact.hip), h_off
 // the host copy of its offsets; symmetric: the CSR holds v -> u for every u -> v (levels

@@ -272,11 +272,13 @@ __global__ __launch_bounds__(kBlock) void k_arc_fingerprint(const uint32_t *off,
 // union over the neighbours of listed nodes; a team of T lanes per node
 // (T = G: one sketch copy; T > G: S = T/G copies split the neighbours)
 // ---------------------------------------------------------------------------
-// 8 wavefronts per SIMD (at most 64 VGPRs; the compiler spills a few rarely used values
-// to scratch): the gathers are latency-bound, and the extra wavefronts in flight took
-// the k-mer graph (p = 6) from 59.5 to 49.5 ms per run with R-MAT-24 (p = 7) unchanged
+// Wavefronts per SIMD: the gathers are latency-bound, so more wavefronts in flight
+// help until the register bound makes the compiler spill.  8 (at most 64 VGPRs) took the
+// k-mer graph (p = 6) from 59.5 to 49.5 ms per run; at p >= 7 the register rows are
+// wider and 7 (at most 72 VGPRs, no spill in the degree <= 256 bin) is faster: R-MAT-24
+// (p = 7) 41.8 ms at 8, 41.0 at 7, while the k-mer graph at 7 takes 57.0.
 template <int P, int T>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8))) void k_union(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(P >= 7 ? 7 : 8))) void k_union(
     HbDev d, const uint32_t *__restrict__ list, uint32_t count, int slot_id)
 {
     const HbIterState *st = d.st;

@@ -651,14 +651,14 @@ int hb_mlp_predict(hb_mlp *n, const float *x, uint64_t count, float *y)
 // device-resident surrogate on a staged graph (declared in sparkx_hb.h)
 namespace hb {
 hipError_t launch_surrogate_tail(hb_mlp *n, const uint32_t *off, const float *snaps, uint32_t nodes,
-                                 uint32_t infer_nodes, float *x_dev, float *out_dev, hipStream_t s)
+                                 uint32_t infer_nodes, bool standardized_x, float *x_dev, float *out_dev, hipStream_t s)
 {
     const unsigned g = (nodes + 255) / 256;
     mlp::k_features<<<g, 256, 0, s>>>(off, snaps, nodes, x_dev);
-    return launch_surrogate_infer(n, off, nodes, infer_nodes, x_dev, out_dev, s);
+    return launch_surrogate_infer(n, off, nodes, infer_nodes, standardized_x, x_dev, out_dev, s);
 }
-hipError_t launch_surrogate_infer(hb_mlp *n, const uint32_t *off, uint32_t nodes, uint32_t infer_nodes, float *x_dev,
-                                  float *out_dev, hipStream_t s)
+hipError_t launch_surrogate_infer(hb_mlp *n, const uint32_t *off, uint32_t nodes, uint32_t infer_nodes,
+                                  bool standardized_x, float *x_dev, float *out_dev, hipStream_t s)
 {
     const unsigned g = (nodes + 255) / 256;
     mlp::k_feature_sums<<<mlp::kSumBlocks, 256, 0, s>>>(off, x_dev, nodes, n->stats + 13);
@@ -666,7 +666,10 @@ hipError_t launch_surrogate_infer(hb_mlp *n, const uint32_t *off, uint32_t nodes
     if (e != hipSuccess) return e;
     e = sum_columns(n->stats + 13, mlp::kSumBlocks, 13, n->stats, s);
     if (e != hipSuccess) return e;
-    mlp::k_standardize<<<g, 256, 0, s>>>(n->stats, nodes, x_dev);
+    // every row when the caller reads the standardised inputs back, else only the rows
+    // the network runs on
+    const uint32_t rows = standardized_x ? nodes : infer_nodes;
+    if (rows) mlp::k_standardize<<<(rows + 255) / 256, 256, 0, s>>>(n->stats, rows, x_dev);
     if (infer_nodes) {
         mlp::Dev d{n->w, n->norm, x_dev, nullptr, nullptr, 0, infer_nodes, 0.0f, nullptr, nullptr, out_dev};
         mlp::k_infer<<<infer_grid(infer_nodes), mlp::kInferBlock, 0, s>>>(d);

@@ -157,10 +157,10 @@ __global__ __launch_bounds__(kBlock) void k_degrees(const uint32_t *__restrict__
 //   2 passes: L(deg), L(S_1), L(S_2), L(M_1), L(S_2) - L(S_1), L(S_1) - L(deg)
 //   3 passes: L(deg), L(S_1), L(S_2), L(S_3), L(M_1), L(S_3) - L(S_2)
 __global__ __launch_bounds__(kBlock) void k_walk_inputs(const uint32_t *__restrict__ deg,
-                                                        const uint64_t *__restrict__ s, uint32_t n, int passes,
-                                                        float *x)
+                                                        const uint64_t *__restrict__ s, uint32_t n, uint32_t x_nodes,
+                                                        int passes, float *x)
 {
-    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < n; u += gridDim.x * blockDim.x) {
+    for (uint32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < x_nodes; u += gridDim.x * blockDim.x) {
         const double ld = log1p((double)deg[u]);
         const double l1 = log1p((double)s[u]);
         const double l2 = log1p((double)s[(uint64_t)n + u]);
@@ -190,7 +190,7 @@ unsigned grid_for(uint64_t items)
 }  // namespace
 
 hipError_t walk_inputs(const uint32_t *off, const uint32_t *nbr, uint32_t n, const WalkHubs &hubs, int passes,
-                       uint64_t *s, uint32_t *deg, uint64_t *part, float *x, hipStream_t st)
+                       uint64_t *s, uint32_t *deg, uint64_t *part, float *x, uint32_t x_nodes, hipStream_t st)
 {
     if (n == 0) return hipSuccess;
     if (passes != 2 && passes != 3) return hipErrorInvalidValue;
@@ -216,7 +216,7 @@ hipError_t walk_inputs(const uint32_t *off, const uint32_t *nbr, uint32_t n, con
                                                   nullptr);
         }
     }
-    k_walk_inputs<<<grid, kBlock, 0, st>>>(deg, s, n, passes, x);
+    if (x_nodes) k_walk_inputs<<<grid_for(x_nodes), kBlock, 0, st>>>(deg, s, n, x_nodes, passes, x);
     return hipGetLastError();
 }
 
