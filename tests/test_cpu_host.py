@@ -203,3 +203,15 @@ def test_drop_cache_follows_the_reference_quirks(tmp_path):
     with pytest.raises(FileNotFoundError):
         cache.drop_cache(hb, d, gid)  # the second slot names seq 1 again
     assert os.path.exists(cache.cache_file(d, "closeness", gid, 2))
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus N > 1 exits with status 2 and says so, before
+    touching the library or a device."""
+    import subprocess
+    import sys as _sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([_sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU only" in r.stderr
