@@ -132,3 +132,29 @@ def test_surrogate_walks_ranks_held_out_graph(hb, oracle, passes):
     say("walks rho", rho, "degree", rho_deg)
     assert rho >= 0.95 and rho > rho_deg
     s.close()
+
+
+def test_walk_errors(hb):
+    """Bad arguments come back as status codes with a message, nothing runs."""
+    import ctypes as C
+    from sparkx_amd import _lib
+    from sparkx_amd.surrogate import Surrogate
+    L = _lib.load()
+    g = hb.Graph.erdos_renyi(500, 1500, seed=3)
+    x = np.empty((g.size(), 6), np.float32)
+    ms = C.c_double()
+    s = Surrogate(seed=1)
+    with hb.HyperBall(g, 4, _run=False) as h:
+        for passes in (0, 1, 4):
+            assert L.hb_walk_features(h._ctx, passes, _lib.ptr(x, C.c_float), C.byref(ms)) == _lib.HB_EINVAL
+            assert b"passes" in L.hb_last_error()
+            assert L.hb_surrogate_walks(h._ctx, s._m, passes, None, None, C.byref(ms)) == _lib.HB_EINVAL
+        assert L.hb_surrogate_walks(h._ctx, None, 2, None, None, C.byref(ms)) == _lib.HB_EINVAL
+    ctx = C.c_void_p()
+    _lib.check(L.hb_create(C.byref(ctx), 0))
+    try:  # nothing staged
+        assert L.hb_walk_features(ctx, 2, None, C.byref(ms)) == _lib.HB_ESTATE
+        assert L.hb_surrogate_walks(ctx, s._m, 2, None, None, C.byref(ms)) == _lib.HB_ESTATE
+    finally:
+        L.hb_destroy(ctx)
+        s.close()
