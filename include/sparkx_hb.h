@@ -230,11 +230,20 @@ int hb_batch_autotune(hb_batch *b, uint32_t *small_max_deg, uint32_t *mid_max_de
 
 /* ---- estimate snapshots ------------------------------------------------------
  * Record every node's estimate after iterations 1..k of the following runs
- * (k <= 16; 0 turns it off).  hb_read_snapshots copies k*n floats, iteration-major;
- * rows past the last executed iteration repeat its estimates.  No reference
- * counterpart (they are the learned surrogate's inputs N_1..N_k). */
+ * (k <= 16; 0 turns it off).  hb_read_snapshots copies k*n floats, iteration-major,
+ * with k the count in force when the completed run started (hb_result_info reports
+ * it; a later hb_set_snapshots changes the next run only); rows past the last
+ * executed iteration repeat its estimates.  No reference counterpart (they are the
+ * learned surrogate's inputs N_1..N_k). */
 int hb_set_snapshots(hb_ctx *ctx, int k);
 int hb_read_snapshots(hb_ctx *ctx, float *dst);
+
+/* Describes the results the context holds (each pointer may be NULL): the precision
+ * p of the completed run (hb_read_registers copies n * 2^p bytes), its iteration
+ * count and its snapshot rows.  HB_ESTATE without a completed run.  hb_surrogate
+ * replaces the run with its own 4 iterations at p_feat.  No reference counterpart
+ * (the reference's P is a type parameter, hyper_ball.rs:81). */
+int hb_result_info(const hb_ctx *ctx, int *p, int *iterations, int *snapshots);
 
 /* ---- learned surrogate ---------------------------------------------------------
  * An MLP 6 -> 64 -> 64 -> 1 with GELU (BF16 MFMA matmuls, f32 accumulation, f32

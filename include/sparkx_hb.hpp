@@ -157,6 +157,9 @@ public:
     // the counters (u8 registers, node-major) the reference leaves in its file
     std::vector<uint8_t> counters(int which = HB_REGS_REFERENCE) const
     {
+        int run_p = 0;  // the engine's completed run must be this mirror's precision
+        check(hb_result_info(ctx_, &run_p, nullptr, nullptr));
+        if (run_p != P) throw HyperBallError(HB_ESTATE, "the context holds a run of another precision");
         std::vector<uint8_t> r(g_.nodes << P);
         check(hb_read_registers(ctx_, which, r.data()));
         return r;

@@ -33,7 +33,7 @@ EXPORTS = (
     "hb_mlp_get_params", "hb_mlp_gradient", "hb_mlp_train", "hb_mlp_predict", "hb_surrogate",
     "hb_walk_features", "hb_surrogate_walks",
     "hb_exact_bfs", "hb_exact_centrality", "hb_eval_ranks", "hb_eval_metrics", "hb_set_layout",
-    "hb_stage_mtx", "hb_stage_coo", "hb_read_csr", "hb_run_many",
+    "hb_stage_mtx", "hb_stage_coo", "hb_read_csr", "hb_run_many", "hb_result_info",
 )
 
 
@@ -181,6 +181,7 @@ def load() -> C.CDLL:
         "hb_read_csr": (C.c_int, [vp, u64p, u64p]),
         "hb_run_many": (C.c_int, [C.POINTER(vp), C.POINTER(C.c_int), C.c_int, C.c_int, C.c_uint32,
                                   C.POINTER(HbStats), C.c_int]),
+        "hb_result_info": (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

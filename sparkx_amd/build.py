@@ -11,7 +11,6 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsparkx_hb.so")
-PROBE = os.path.join(HERE, "hbm_copy_probe")  # PMC calibration / streaming-bandwidth probe
 
 SOURCES = ["hb_kernels.hip", "hb_api.hip", "hb_batch.hip", "hb_mlp.hip", "hb_exact.hip", "hb_eval.hip", "hb_layout.hip", "hb_ingest.hip", "hb_walk.hip", "hb_graphgen.cpp"]
 HEADERS = ["hb_device.h", "hb_kernels.h", "hb_union.h", "hll_bias_table.inc"]
@@ -53,11 +52,6 @@ def build(verbose: bool = False) -> str:
         if res.returncode != 0:
             raise RuntimeError(f"link failed ({res.returncode}):\n{res.stdout}\n{res.stderr}")
     os.replace(LIB + ".tmp", LIB)
-    probe = [hipcc(), "--offload-arch=gfx950", "-O3", "-Xclang", "-target-feature", "-Xclang",
-             "-packed-fp32-ops", os.path.join(CSRC, "hbm_copy_probe.hip"), "-o", PROBE]
-    res = subprocess.run(probe, capture_output=True, text=True)
-    if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed on the copy probe:\n{res.stderr}")
     return LIB
 
 

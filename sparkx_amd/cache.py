@@ -111,30 +111,64 @@ def read_result(cache_dir: str, kind: str, gid: str, normalize: bool | None = No
 
 
 def read_mtx(path: str) -> Graph:
-    """CSR of a Matrix Market `coordinate` file with the reference's rules."""
-    with open(path) as fh:
-        header = fh.readline().lower().split()
-        if len(header) < 5 or header[0] != "%%matrixmarket" or header[2] != "coordinate":
-            raise ValueError(f"{path}: not a MatrixMarket coordinate file")
-        symmetric = header[4] == "symmetric"
-        line = fh.readline()
-        while line.startswith("%"):
-            line = fh.readline()
-        nrows, ncols, _ = (int(v) for v in line.split()[:3])
-        data = np.loadtxt(fh, dtype=np.int64, usecols=(0, 1), ndmin=2)
-    n = max(nrows, ncols)
-    i = data[:, 0] - 1
-    j = data[:, 1] - 1
-    if data.size and (i.min() < 0 or j.min() < 0 or i.max() >= nrows or j.max() >= ncols):
-        raise ValueError(f"{path}: entry out of bounds")
+    """CSR of a MatrixMarket file with the reference's rules (parser_mtx/mod.rs:34-162,
+    producers.rs:228-234, impl_sort.rs:10-51), the same ones hb_stage_mtx applies on the
+    device: the first line starts with "%%MatrixMarket" (case-sensitive) and names
+    "matrix coordinate pattern|integer symmetric|skew-symmetric|general" (case-
+    insensitive); every kind but "general" also emits j -> i for i != j; rows must equal
+    columns; blank and '%' lines and lines with a single token are skipped; indices are
+    1-based, 0 and values above n are errors; duplicates are kept and every list is
+    sorted ascending.  Raises ValueError naming the first bad line."""
+    with open(path, "rb") as fh:
+        lines = fh.read().decode("ascii").split("\n")
+    head = lines[0].rstrip("\r")
+    if not head.startswith("%%MatrixMarket"):
+        raise ValueError(f"{path}: line 1: not a MatrixMarket file")
+    tok = head.split()
+    if len(tok) < 5:
+        raise ValueError(f"{path}: line 1: malformed header")
+    kind = [t.lower() for t in tok[1:5]]
+    if (kind[0] != "matrix" or kind[1] != "coordinate" or kind[2] not in ("pattern", "integer")
+            or kind[3] not in ("symmetric", "skew-symmetric", "general")):
+        raise ValueError(f"{path}: line 1: unsupported MatrixMarket format {' '.join(tok[1:5])}")
+    symmetric = kind[3] != "general"
+    ln = 1
+    while True:
+        if ln >= len(lines):
+            raise ValueError(f"{path}: no size line")
+        size = lines[ln].split()
+        ln += 1
+        if size and not size[0].startswith("%"):
+            break
+    try:
+        nrows, ncols = int(size[0]), int(size[1])
+    except (IndexError, ValueError):
+        raise ValueError(f"{path}: line {ln}: bad size line") from None
+    if nrows != ncols:
+        raise ValueError(f"{path}: line {ln}: matrix is not square ({nrows} x {ncols})")
+    n = nrows
+    src, dst = [], []
+    for k in range(ln, len(lines)):
+        t = lines[k].split()
+        if len(t) < 2 or t[0].startswith("%"):
+            continue
+        a, b = t[0].lstrip("+"), t[1].lstrip("+")
+        if not (a.isdigit() and b.isdigit() and t[0].count("+") <= 1 and t[1].count("+") <= 1):
+            raise ValueError(f"{path}: line {k + 1}: index is not a number")
+        i, j = int(a), int(b)
+        if i == 0 or j == 0:
+            raise ValueError(f"{path}: line {k + 1}: index 0 (MatrixMarket is 1-based)")
+        if i > n or j > n:
+            raise ValueError(f"{path}: line {k + 1}: index above n = {n}")
+        src.append(i - 1)
+        dst.append(j - 1)
+    i = np.asarray(src, np.int64)
+    j = np.asarray(dst, np.int64)
     if symmetric:
         off_diag = i != j
-        src = np.concatenate([i, j[off_diag]])
-        dst = np.concatenate([j, i[off_diag]])
-    else:
-        src, dst = i, j
-    order = np.lexsort((dst, src))
-    nbr = dst[order].astype(np.uint64)
+        i, j = np.concatenate([i, j[off_diag]]), np.concatenate([j, i[off_diag]])
+    order = np.lexsort((j, i))
+    nbr = j[order].astype(np.uint64)
     off = np.zeros(n + 1, np.uint64)
-    np.cumsum(np.bincount(src, minlength=n), out=off[1:])
+    np.cumsum(np.bincount(i, minlength=n), out=off[1:])
     return Graph(off, nbr, os.path.basename(path))

@@ -116,7 +116,8 @@ struct hb_ctx {
     std::vector<HbIterRec> h_rec;
     uint32_t sched_p = 0, sched_key[4] = {};
     uint32_t n_linked = 0;
-    uint32_t snap_k = 0;
+    uint32_t snap_k = 0;      // snapshots the next run records (hb_set_snapshots)
+    uint32_t run_snap_k = 0;  // snapshots the completed run recorded (rows in snaps)
     DevBuf<float> snaps, feat, pred;
     DevBuf<uint64_t> walk;      // walk-count passes: S_1, S_2, S_3, M_1, hub partials
     DevBuf<uint32_t> walk_hub;  // hub chunk table of the walk passes, then the u32 degrees
@@ -592,6 +593,7 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
         c->h_st.converged = 1;
         c->h_rec.assign(2, HbIterRec{});
         c->p = p;
+        c->run_snap_k = 0;
         c->ran = true;
         if (stats) {
             memset(stats, 0, sizeof *stats);
@@ -813,6 +815,7 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
     c->h_rec.assign(executed + 1, HbIterRec{});
     HB_TRY(hipMemcpy(c->h_rec.data(), c->rec.ptr, (executed + 1) * sizeof(HbIterRec), hipMemcpyDeviceToHost));
     c->p = p;
+    c->run_snap_k = c->snap_k;
     c->ran = true;
     if (stats) {
         memset(stats, 0, sizeof *stats);
@@ -1055,6 +1058,17 @@ extern "C" int hb_autotune(hb_ctx *c, int p, int trial_iters, uint32_t *small_ma
     return HB_OK;
 }
 
+extern "C" int hb_result_info(const hb_ctx *c, int *p, int *iterations, int *snapshots)
+{
+    if (!c) return fail(HB_EINVAL, "hb_result_info: ctx is NULL");
+    if (!c->ran) return fail(HB_ESTATE, "hb_result_info: no completed run");
+    if (p) *p = c->p;
+    if (iterations) *iterations = (int)c->h_st.executed;
+    if (snapshots) *snapshots = (int)c->run_snap_k;
+    g_err.clear();
+    return HB_OK;
+}
+
 extern "C" int hb_set_snapshots(hb_ctx *c, int k)
 {
     if (!c) return fail(HB_EINVAL, "hb_set_snapshots: ctx is NULL");
@@ -1067,9 +1081,11 @@ extern "C" int hb_set_snapshots(hb_ctx *c, int k)
 extern "C" int hb_read_snapshots(hb_ctx *c, float *dst)
 {
     if (!c || !dst) return fail(HB_EINVAL, "hb_read_snapshots: NULL argument");
-    if (!c->ran || !c->snap_k || !c->snaps.ptr) return fail(HB_ESTATE, "hb_read_snapshots: no run with snapshots");
+    // rows of the completed run: hb_set_snapshots after it changes the next run only
+    if (!c->ran || !c->run_snap_k || !c->snaps.ptr || c->snaps.count < (size_t)c->run_snap_k * c->n)
+        return fail(HB_ESTATE, "hb_read_snapshots: no run with snapshots");
     if (!set_device(c)) return fail(HB_EHIP, "hipSetDevice failed");
-    for (uint32_t t = 0; t < c->snap_k; ++t)
+    for (uint32_t t = 0; t < c->run_snap_k; ++t)
         if (int rc = copy_out(c, c->snaps.ptr + (size_t)t * c->n, dst + (size_t)t * c->n, 4)) return rc;
     g_err.clear();
     return HB_OK;

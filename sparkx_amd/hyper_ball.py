@@ -133,10 +133,27 @@ class HyperBall:
     # ---- results --------------------------------------------------------------
     def get_iters(self) -> int:
         """Iterations executed, including the final one with no increase (hyper_ball.rs:557-560)."""
-        return int(self.stats.iterations)
+        return int(self.result_info()[1])
+
+    def _replaced(self) -> None:
+        """The engine's run on this graph may have been replaced by one whose statistics
+        are not returned (the surrogate's 4 iterations): drop the cached results and
+        take the precision of whatever run the engine now holds."""
+        self._ran(None)
+        try:
+            self.p = self.result_info()[0]
+        except _lib.HyperBallError:
+            pass  # no completed run: the accessors raise HB_ESTATE
+
+    def result_info(self):
+        """(p, iterations, snapshot rows) of the results the engine holds for this graph."""
+        p, it, snaps = C.c_int(), C.c_int(), C.c_int()
+        _lib.check(self._L.hb_result_info(self._ctx, C.byref(p), C.byref(it), C.byref(snaps)))
+        return p.value, it.value, snaps.value
 
     def registers(self, which: int = _lib.HB_REGS_REFERENCE) -> np.ndarray:
-        out = np.empty((self.g.size(), 1 << self.p), np.uint8)
+        # sized from the engine's completed run, whose p is the one the registers have
+        out = np.empty((self.g.size(), 1 << self.result_info()[0]), np.uint8)
         _lib.check(self._L.hb_read_registers(self._ctx, which, _lib.ptr(out, C.c_uint8)))
         return out
 

@@ -98,15 +98,20 @@ class Surrogate:
         return out
 
     def rank(self, h: HyperBall, p_feat: int = 4, want_features: bool = False):
-        """Anytime ranking of the graph staged in `h` (its run is replaced): returns
-        (predicted standardised log harmonic centrality, nodes without neighbours at
-        -3e38; standardised inputs or None; device ms)."""
+        """Anytime ranking of the graph staged in `h` (its run is replaced by the 4
+        iterations at p_feat, and `h` then describes that run): returns (predicted
+        standardised log harmonic centrality, nodes without neighbours at -3e38;
+        standardised inputs or None; device ms)."""
         n = h.g.size()
         pred = np.empty(n, np.float32)
         feat = np.empty((n, 6), np.float32) if want_features else None
         ms = C.c_double()
-        _lib.check(self._L.hb_surrogate(h._ctx, self._m, p_feat, _lib.ptr(pred, C.c_float),
-                                        _lib.ptr(feat, C.c_float) if feat is not None else None, C.byref(ms)))
+        try:
+            _lib.check(self._L.hb_surrogate(h._ctx, self._m, p_feat, _lib.ptr(pred, C.c_float),
+                                            _lib.ptr(feat, C.c_float) if feat is not None else None,
+                                            C.byref(ms)))
+        finally:
+            h._replaced()  # the context's previous run is gone (also when the call failed midway)
         return pred, feat, ms.value
 
     def rank_walks(self, h: HyperBall, passes: int = 3, want_features: bool = False):

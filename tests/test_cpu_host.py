@@ -153,6 +153,43 @@ def test_mtx_reader_matches_reference_rules(tmp_path, oracle):
     assert np.array_equal(g.offsets, off) and np.array_equal(g.neighbours, nb.astype(np.uint64))
 
 
+@pytest.mark.parametrize("kind", ["general", "symmetric", "skew-symmetric", "Symmetric"])
+def test_mtx_reader_symmetry_kinds_match_the_oracle(tmp_path, oracle, kind):
+    """Every header but "general" emits j -> i too (self-loops once), as the reference's
+    producer and the device parser do; comments, blank and single-token lines skip."""
+    from sparkx_amd.cache import read_mtx
+    rng = np.random.default_rng(len(kind))
+    n = 30
+    body = "\n".join(f"{a} {b}" for a, b in rng.integers(1, n + 1, size=(80, 2)))
+    text = f"%%MatrixMarket matrix coordinate pattern {kind}\n% c\n{n} {n} 80\n\n7\n% mid\n{body}\n"
+    path = tmp_path / "g.mtx"
+    path.write_text(text)
+    g = read_mtx(str(path))
+    off, nb, info = oracle.parse_mtx(text.encode())
+    assert info.symmetric == (kind.lower() != "general")
+    assert np.array_equal(g.offsets, off) and np.array_equal(g.neighbours, nb.astype(np.uint64))
+
+
+@pytest.mark.parametrize("text,what", [
+    ("%%MatrixMarket matrix coordinate pattern general\n3 4 1\n1 2\n", "not square"),
+    ("%%MatrixMarket matrix array pattern general\n3 3 1\n1 2\n", "unsupported"),
+    ("%%MatrixMarket matrix coordinate real general\n3 3 1\n1 2\n", "unsupported"),
+    ("%%MatrixMarket matrix coordinate pattern\n3 3 1\n1 2\n", "malformed"),
+    ("%%matrixmarket matrix coordinate pattern general\n3 3 1\n1 2\n", "not a MatrixMarket"),
+    ("%%MatrixMarket matrix coordinate pattern general\n3 3 1\n0 2\n", "index 0"),
+    ("%%MatrixMarket matrix coordinate pattern general\n3 3 1\n1 4\n", "above n"),
+    ("%%MatrixMarket matrix coordinate pattern general\n3 3 1\n1 x\n", "not a number"),
+])
+def test_mtx_reader_rejects_what_the_reference_rejects(tmp_path, oracle, text, what):
+    from sparkx_amd.cache import read_mtx
+    path = tmp_path / "bad.mtx"
+    path.write_text(text)
+    with pytest.raises(ValueError, match=what):
+        read_mtx(str(path))
+    with pytest.raises(oracle.MtxError):
+        oracle.parse_mtx(text.encode())
+
+
 def test_cache_ids_and_names():
     from sparkx_amd import cache
     # the id of the reference's committed exact-closeness fixture for this dataset

@@ -1,6 +1,7 @@
 """CPU tests: pin the oracle (the checker of the HIP path) against the
 reference's fixtures, published numbers and the mathematics, and check the
 host-side logic.  No GPU calls."""
+import functools
 import json
 import math
 import os
@@ -178,22 +179,65 @@ def test_published_numbers_exact_in_linear_counting_regime(oracle, p):
     assert rho == PUBLISHED[p]["rho"][0]
 
 
-def test_published_numbers_close_in_bias_regime(oracle):
-    """Rows that pass through the HLL++ bias-corrected regime: the reference's
-    empirical bias table is not available, ours is derived from the Poisson model;
-    all such rows agree within 2.5% (MAE/MAPE) and 1e-3 relative (rho)."""
-    from tests.golden.pin_hll_variant import PUBLISHED
+def _recorded_rows():
     with open(os.path.join(GOLDEN, "published_accuracy.json")) as fh:
-        recorded = json.load(fh)["rows"]
-    assert len(recorded) == 12
-    for row in recorded:
-        for key in ("mae", "mape"):
-            assert row[key]["rel_diff"] < 0.025, row
-        assert row["rho"]["rel_diff"] < 1e-3, row
-    # re-run two rows live
-    mae, mape, rho = _profile(oracle, 20, 2, 8, "exact_closeness_dbg_sigma20_k2.f64")
-    assert abs(mae - PUBLISHED[8]["mae"][0]) / PUBLISHED[8]["mae"][0] < 0.01
-    assert rho == PUBLISHED[8]["rho"][0]
+        return json.load(fh)["rows"]
+
+
+_ROWS = _recorded_rows()
+_FIXTURE = {"dbg_sigma20_k2": (20, 2, "exact_closeness_dbg_sigma20_k2.f64"),
+            "dbg_sigma4_k7": (4, 7, "exact_closeness_dbg_sigma4_k7.f64")}
+_METRICS = ("mae", "mape", "rho")
+
+
+@functools.lru_cache(maxsize=None)
+def _profile_row(p, graph):
+    from oracle import hb_oracle as O
+    sigma, k, fixture = _FIXTURE[graph]
+    return dict(zip(_METRICS, _profile(O, sigma, k, p, fixture)))
+
+
+def _row_cases(published: bool):
+    cases = []
+    for row in _ROWS:
+        for key in _METRICS:
+            marks = ()
+            if published and row[key]["rel_diff"] != 0.0:
+                # the estimate passes through the HLL++ bias-corrected regime, whose
+                # empirical table lives only in the absent crate (parity unpinned)
+                marks = (pytest.mark.xfail(strict=True, reason=(
+                    f"bias-regime miss: ours {row[key]['ours']!r} vs published {row[key]['published']!r} "
+                    f"(rel {row[key]['rel_diff']:.2e})")),)
+            cases.append(pytest.param(row["p"], row["graph"], key, id=f"p{row['p']}-{row['graph']}-{key}",
+                                      marks=marks))
+    return cases
+
+
+@pytest.mark.parametrize("p,graph,key", _row_cases(published=False))
+def test_published_rows_reproduce_recorded_values(oracle, p, graph, key):
+    """Regression guard on the estimator: every one of the 36 numbers equals, to the
+    last bit, the value recorded in published_accuracy.json, so any change of the
+    bias-corrected branch shows up here."""
+    row = next(r for r in _ROWS if r["p"] == p and r["graph"] == graph)
+    assert _profile_row(p, graph)[key] == row[key]["ours"]
+
+
+@pytest.mark.parametrize("p,graph,key", _row_cases(published=True))
+def test_published_rows_against_the_reference(oracle, p, graph, key):
+    """Each published number (plot_hyperball_results.py:38-57) against the oracle.
+    Numbers whose estimates stay in the linear-counting regime match exactly; the
+    known misses are listed one by one as strict xfails with their values, so a
+    change that makes one match (or breaks a matching one) is reported."""
+    from tests.golden.pin_hll_variant import PUBLISHED
+    gi = 0 if graph == "dbg_sigma20_k2" else 1
+    assert _profile_row(p, graph)[key] == PUBLISHED[p][key][gi]
+
+
+def test_published_accuracy_table_is_complete():
+    assert len(_ROWS) == 12 and {(r["p"], r["graph"]) for r in _ROWS} == {
+        (p, gname) for p in range(8, 14) for gname in _FIXTURE}
+    matched = sum(r[k]["rel_diff"] == 0.0 for r in _ROWS for k in _METRICS)
+    assert matched == 14  # 12 linear-counting numbers of the 400-node graph + rho at p = 8, 9
 
 
 # ---------------------------------------------------------------------------

@@ -619,6 +619,41 @@ def test_layout_switch_and_snapshots(hb, oracle):
             h.set_layout("random")
 
 
+def test_snapshot_rows_follow_the_completed_run(hb, oracle):
+    """hb_set_snapshots after a run changes the next run only: reading returns the rows
+    the completed run recorded, and hb_result_info reports their count."""
+    from sparkx_amd import _lib
+    g = hb.Graph.erdos_renyi(3000, 9000, seed=21)
+    with hb.HyperBall(g, 6, _run=False) as h:
+        _lib.check(h._L.hb_set_snapshots(h._ctx, 2))
+        h.run()
+        _lib.check(h._L.hb_set_snapshots(h._ctx, 16))
+        p, it, rows = h.result_info()
+        assert (p, rows) == (6, 2) and it == h.stats.iterations
+        out = np.full((16, g.size()), -1.0, np.float32)
+        _lib.check(h._L.hb_read_snapshots(h._ctx, _lib.ptr(out, C.c_float)))
+        assert (out[2:] == -1.0).all()
+        ref = oracle.hyperball(g.offsets.astype(np.uint64), g.neighbours, 6, max_iters=1)
+        assert np.array_equal(out[0], oracle.estimates(6, ref.regs_last))
+
+
+def test_surrogate_replaces_the_mirrors_run(hb):
+    """After an anytime ranking the Python mirror describes the surrogate's run: its
+    precision sizes the register reads."""
+    from sparkx_amd.surrogate import Surrogate
+    g = hb.Graph.rmat(12, 8, seed=5)
+    with hb.HyperBall(g, 7) as h:
+        s = Surrogate()
+        try:
+            s.rank(h, p_feat=5)
+        finally:
+            s.close()
+        assert h.p == 5 and h.stats is None
+        p, it, rows = h.result_info()
+        assert (p, rows) == (5, 4) and it <= 4
+        assert h.registers().shape == (g.size(), 32)
+
+
 def test_run_many_matches_oracle(hb, oracle):
     """hb_run_many: graphs of different sizes, densities and p (4..12) on their own
     streams from 4 host threads; each one bit-identical to the oracle."""
