@@ -337,6 +337,18 @@ int hb_eval_ranks(int device, const double *v, uint64_t n, uint32_t *rank);
 int hb_eval_metrics(int device, const double *a, const double *b, uint64_t n, double top_frac,
                     double *out);
 
+/* hb_eval_metrics_noalloc: hb_eval_metrics without any device allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate device memory
+ * (tight serving or evaluation loops, harnesses that forbid hipMalloc).  a and b must
+ * be device-accessible arrays (else HB_EINVAL); out is 4 host doubles, as above, and
+ * the results are identical to hb_eval_metrics's.  Every intermediate buffer lives in
+ * one fixed 32 MiB device array that the library holds on each device it runs on until
+ * the process ends; n up to 2^19 fits, a larger n gives HB_ERANGE.  Runs on the
+ * device's null stream and synchronises it.  Calls that overlap on one device share
+ * that array and race: serialise them. */
+int hb_eval_metrics_noalloc(int device, const double *a, const double *b, uint64_t n, double top_frac,
+                            double *out);
+
 /* ---- graph ingestion on the device ---------------------------------------------
  * hb_stage_mtx parses a MatrixMarket text ("%%MatrixMarket matrix coordinate
  * pattern|integer symmetric|skew-symmetric|general", square) on the device and stages

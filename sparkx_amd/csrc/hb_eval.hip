@@ -15,6 +15,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include <hip/hip_runtime.h>
@@ -307,6 +308,147 @@ extern "C" int hb_eval_metrics(int device, const double *a, const double *b, uin
     out[0] = s_abs / nf;                                                 // utils.rs:57-59
     out[1] = 100.0 * s_rel / nf;                                         // utils.rs:60-74
     out[2] = 1.0 - (6.0 * ssd) / (nf * std::max(nf * nf - 1.0, 1.0));    // utils.rs:87-100
+    out[3] = (double)common / (double)k;
+    hb::set_error("");
+    return HB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// hb_eval_metrics_noalloc: hb_eval_metrics for device arrays, with every
+// intermediate buffer (sort keys and orders, ranks, top-k flags, block partials,
+// the sort's temporary storage) carved out of one fixed device array instead of
+// allocated per call.  The array is loaded with the library; each device holds
+// its own copy at its own address, which host code obtains once per device with
+// hipGetSymbolAddress and uses for every kernel argument and fill.  The array
+// keeps the last call's contents, so everything that must start at zero or at
+// "no NaN" is filled by a kernel at the start of each call; calls that overlap
+// on one device share it and race.
+// ---------------------------------------------------------------------------
+static constexpr size_t kNoAllocScratchBytes = 32ull << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[kMaxDevices] = {};
+
+// device address of `scratch` on the current device (set-up once per device)
+int scratch_ptr(const char *fn, int device, unsigned char **out)
+{
+    if (device >= kMaxDevices) return evfail(HB_EINVAL, "%s: device %d above %d", fn, device, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return evfail(HB_EHIP, "%s: hipGetSymbolAddress(scratch) on device %d: %s", fn, device,
+                          hipGetErrorString(e));
+        g_scratch_ptr[device] = (unsigned char *)p;
+    }
+    *out = g_scratch_ptr[device];
+    return HB_OK;
+}
+
+// a must be memory the device can read (device, managed or mapped host memory)
+int check_device_ptr(const char *fn, const void *p, int arg)
+{
+    hipPointerAttribute_t at;
+    memset(&at, 0, sizeof at);
+    const hipError_t e = hipPointerGetAttributes(&at, p);
+    (void)hipGetLastError();  // an unknown host pointer leaves an error behind
+    if (e != hipSuccess || !at.devicePointer)
+        return evfail(HB_EINVAL, "%s: argument %d is not device-accessible memory (this entry point takes "
+                      "device arrays; hb_eval_metrics also takes host arrays)", fn, arg);
+    return HB_OK;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+}  // namespace
+
+extern "C" int hb_eval_metrics_noalloc(int device, const double *a, const double *b, uint64_t n, double top_frac,
+                                       double *out)
+{
+    using namespace hb::eval;
+    static const char fn[] = "hb_eval_metrics_noalloc";
+    const double *vs[2] = {a, b};
+    if (!out) return evfail(HB_EINVAL, "%s: NULL argument", fn);
+    if (!(top_frac > 0.0 && top_frac <= 1.0)) return evfail(HB_EINVAL, "%s: top_frac not in (0, 1]", fn);
+    if (int rc = check_args(fn, device, n, vs, 2)) return rc;
+    if (int rc = check_device_ptr(fn, a, 1)) return rc;
+    if (int rc = check_device_ptr(fn, b, 2)) return rc;
+    unsigned char *base = nullptr;
+    if (int rc = scratch_ptr(fn, device, &base)) return rc;
+    const hipStream_t s = nullptr;  // the device's null stream; synchronised before returning
+    // carve the fixed array for this n
+    size_t at = 0;
+    auto take = [&](size_t bytes) {
+        unsigned char *p = base + at;
+        at += align256(bytes);
+        return p;
+    };
+    double *kin = (double *)take(n * 8), *kout = (double *)take(n * 8);
+    uint32_t *iin = (uint32_t *)take(n * 4);
+    uint32_t *oa = (uint32_t *)take(n * 4), *ob = (uint32_t *)take(n * 4);
+    uint32_t *rka = (uint32_t *)take(n * 4), *rkb = (uint32_t *)take(n * 4);
+    const uint64_t flag_words = (n + 3) / 4;
+    uint8_t *flag = take(flag_words * 4);
+    unsigned long long *nan = (unsigned long long *)take(2 * 8);
+    Partial *part = (Partial *)take(kSumBlocks * sizeof(Partial));
+    unsigned long long *tpart = (unsigned long long *)take(kSumBlocks * 8);
+    size_t tmp_bytes = 0;
+    HBE_TRY(rocprim::radix_sort_pairs_desc(nullptr, tmp_bytes, kin, kout, iin, oa, (size_t)n, 0, 64, s));
+    void *tmp = take(std::max<size_t>(tmp_bytes, 1));
+    if (at > kNoAllocScratchBytes)
+        return evfail(HB_ERANGE, "%s: n=%llu needs %zu bytes of scratch; the fixed device array holds %zu "
+                      "(n up to 2^19 fits; use hb_eval_metrics for larger inputs)", fn,
+                      (unsigned long long)n, at, kNoAllocScratchBytes);
+    // state left by the previous call: NaN slots to "none", top-k flags to zero
+    HBE_TRY(hb::launch_fill((uint32_t *)nan, 4, 0xffffffffu, s));
+    HBE_TRY(hb::launch_fill((uint32_t *)flag, flag_words, 0u, s));
+    const unsigned grid = (unsigned)std::min<uint64_t>((n + kBlock - 1) / kBlock, 8192);
+    const double *src[2] = {a, b};
+    uint32_t *order[2] = {oa, ob}, *rank[2] = {rka, rkb};
+    for (int j = 0; j < 2; ++j) {
+        k_prep<<<grid, kBlock, 0, s>>>(src[j], n, kin, iin, nan + j);
+        HBE_TRY(hipGetLastError());
+        HBE_TRY(rocprim::radix_sort_pairs_desc(tmp, tmp_bytes, kin, kout, iin, order[j], (size_t)n, 0, 64, s));
+        k_ranks<<<grid, kBlock, 0, s>>>(order[j], n, rank[j]);
+        HBE_TRY(hipGetLastError());
+    }
+    k_terms<<<kSumBlocks, kBlock, 0, s>>>(a, b, rka, rkb, n, part);
+    HBE_TRY(hipGetLastError());
+    const uint64_t k = std::max<uint64_t>(1, (uint64_t)std::ceil(top_frac * (double)n));
+    k_top_mark<<<grid, kBlock, 0, s>>>(oa, k, flag);
+    HBE_TRY(hipGetLastError());
+    k_top_count<<<kSumBlocks, kBlock, 0, s>>>(ob, k, flag, tpart);
+    HBE_TRY(hipGetLastError());
+    // block partials and NaN slots back to the host (fixed-size host arrays)
+    static thread_local Partial hp[kSumBlocks];
+    static thread_local unsigned long long ht[kSumBlocks];
+    unsigned long long nan_at[2];
+    HBE_TRY(hipMemcpyAsync(hp, part, sizeof hp, hipMemcpyDeviceToHost, s));
+    HBE_TRY(hipMemcpyAsync(ht, tpart, sizeof ht, hipMemcpyDeviceToHost, s));
+    HBE_TRY(hipMemcpyAsync(nan_at, nan, sizeof nan_at, hipMemcpyDeviceToHost, s));
+    HBE_TRY(hipStreamSynchronize(s));
+    for (int j = 0; j < 2; ++j)
+        if (nan_at[j] != ~0ull)
+            return evfail(HB_EINVAL, "%s: NaN at index %llu of argument %d (the reference's order is "
+                          "unspecified for it)", fn, nan_at[j], j + 1);
+    double s_abs = 0.0, s_rel = 0.0;
+    unsigned long long lo = 0, hi = 0, common = 0;
+    for (unsigned j = 0; j < kSumBlocks; ++j) {  // second stage, in block order (as hb_eval_metrics)
+        s_abs += hp[j].abs_err;
+        s_rel += hp[j].rel_err;
+        const unsigned long long t = lo + hp[j].ssd_lo;
+        hi += hp[j].ssd_hi + (t < lo);
+        lo = t;
+        common += ht[j];
+    }
+    const double nf = (double)n;
+    const double ssd = u128_to_double(lo, hi);
+    out[0] = s_abs / nf;
+    out[1] = 100.0 * s_rel / nf;
+    out[2] = 1.0 - (6.0 * ssd) / (nf * std::max(nf * nf - 1.0, 1.0));
     out[3] = (double)common / (double)k;
     hb::set_error("");
     return HB_OK;
