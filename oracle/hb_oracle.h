@@ -91,6 +91,9 @@ typedef struct {
     uint64_t increases[1024];/* per-iteration count of nodes whose estimate increased */
     double init_seconds;     /* wall time of the register initialisation */
     double loop_seconds;     /* wall time of the iterations               */
+    uint64_t regime[3];      /* estimates after the union, over all (node, iteration):
+                                [0] linear counting, [1] raw with the HLL++ bias
+                                subtracted (raw <= 5m), [2] raw as is */
 } hbo_stats;
 
 /*

@@ -57,6 +57,7 @@ class Stats(C.Structure):
         ("increases", C.c_uint64 * 1024),
         ("init_seconds", C.c_double),
         ("loop_seconds", C.c_double),
+        ("regime", C.c_uint64 * 3),
     ]
 
 
@@ -206,6 +207,9 @@ class HBResult:
     increases: np.ndarray
     init_seconds: float = 0.0
     loop_seconds: float = 0.0
+    # estimates after the union over all (node, iteration): linear counting, raw with
+    # the HLL++ bias subtracted, raw as is
+    regime: tuple = (0, 0, 0)
 
 
 def hyperball(offsets: np.ndarray, nbrs: np.ndarray, p: int, *, v: Variant | None = None,
@@ -240,7 +244,7 @@ def hyperball(offsets: np.ndarray, nbrs: np.ndarray, p: int, *, v: Variant | Non
     it = st.iterations
     return HBResult(regs_ref, regs_last, dist, inv, it, bool(st.final_parity_latest),
                     np.array(st.increases[:min(it, 1024)], dtype=np.uint64),
-                    st.init_seconds, st.loop_seconds)
+                    st.init_seconds, st.loop_seconds, tuple(int(x) for x in st.regime))
 
 
 CLOSENESS, HARMONIC, LIN = 0, 1, 2
