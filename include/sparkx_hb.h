@@ -333,6 +333,16 @@ int hb_exact_centrality(hb_ctx *ctx, int kind, int normalization, uint64_t src_b
  *
  * 1 <= n < 2^32; a NaN anywhere gives HB_EINVAL (the reference's order for it is
  * unspecified). */
+/* hb_bias_windows: the HLL++ bias-correction windows of precision p (4..16) as the
+ * kernels read them, built on the host; no device work.  No counterpart in the
+ * reference: it exposes the tabulated form of hyperloglog-rs's k = 6 nearest-neighbour
+ * bias rule (estimate_cardinality, called at hyper_ball.rs:806,818) so that it can be
+ * checked against that rule on the CPU.  mean[195]: mean bias of window s (points
+ * s..s+5); breakpoints[195]: breakpoints[s] (s >= 1) is the smallest f32 estimate whose
+ * window starts at s or later (breakpoints[0] = 0); buckets[1024]: window start at the
+ * lower edge of each of 256 buckets per octave over [m/2, 8m). */
+int hb_bias_windows(int p, double *mean, float *breakpoints, uint8_t *buckets);
+
 int hb_eval_ranks(int device, const double *v, uint64_t n, uint32_t *rank);
 int hb_eval_metrics(int device, const double *a, const double *b, uint64_t n, double top_frac,
                     double *out);

@@ -76,6 +76,7 @@ float hbo_estimate_u8(const hbo_variant *v, int p, const uint8_t *regs);
 float hbo_estimate_packed(const hbo_variant *v, int p, const uint32_t *words);
 /* HLL++ bias of the raw estimate e at precision p (mean of the 6 nearest points). */
 double hbo_knn_bias(int p, double e);
+void hbo_knn_bias_many(int p, const float *e, uint64_t n, double *out);
 /* pack / unpack between u8 registers and 6-bit packed words (ceil(m/5) words). */
 void hbo_pack6(int p, const uint8_t *regs, uint32_t *words);
 void hbo_unpack6(int p, const uint32_t *words, uint8_t *regs);

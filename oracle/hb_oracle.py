@@ -115,6 +115,7 @@ def lib():
         L.hbo_estimate_packed.argtypes = [vp, C.c_int, u32p]
         L.hbo_knn_bias.restype = C.c_double
         L.hbo_knn_bias.argtypes = [C.c_int, C.c_double]
+        L.hbo_knn_bias_many.argtypes = [C.c_int, f32p, C.c_uint64, f64p]
         L.hbo_pack6.argtypes = [C.c_int, u8p, u32p]
         L.hbo_unpack6.argtypes = [C.c_int, u32p, u8p]
         L.hbo_hyperball.restype = C.c_int
@@ -165,6 +166,15 @@ def index_rank(p: int, x: int, v: Variant | None = None):
     r = C.c_uint8()
     idx = lib().hbo_index_rank(C.byref(v), p, x, C.byref(r))
     return int(idx), int(r.value)
+
+
+def knn_bias(p: int, e: np.ndarray) -> np.ndarray:
+    """HLL++ EstimateBias (k = 6 nearest points, ties to the lower one) of each f32
+    estimate in e."""
+    e = np.ascontiguousarray(e, dtype=np.float32)
+    out = np.empty(e.size, np.float64)
+    lib().hbo_knn_bias_many(p, _p(e, C.c_float), e.size, _p(out, C.c_double))
+    return out
 
 
 def estimate(p: int, regs: np.ndarray, v: Variant | None = None) -> np.float32:

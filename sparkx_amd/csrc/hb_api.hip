@@ -27,6 +27,81 @@
 #define HB_BIAS_TABLE_QUAL static const
 #include "hll_bias_table.inc"
 
+static_assert(HB_BIAS_K - 5 == HB_BIAS_WINDOWS, "bias windows of 6 consecutive points");
+static_assert(sizeof(HbBiasTable) % 8 == 0, "HbBiasTable is staged as 8-byte words");
+
+namespace hb {
+namespace {
+// Start of the window of 6 consecutive bias points HLL++ EstimateBias averages for the
+// estimate e: the first point above e, then 6 steps each taking the nearer of the
+// points on either side of the window (the lower one on a tie), in f64 as the rule
+// compares them.
+int bias_window_start(const double *raw, double e)
+{
+    int lo = 0, hi = HB_BIAS_K;
+    while (lo < hi) {
+        const int mid = (lo + hi) / 2;
+        if (raw[mid] > e) hi = mid;
+        else lo = mid + 1;
+    }
+    hi = lo;
+    for (int k = 0; k < 6; ++k) {
+        if (lo > 0 && (hi == HB_BIAS_K || e - raw[lo - 1] <= raw[hi] - e)) --lo;
+        else ++hi;
+    }
+    return lo;
+}
+
+void build_bias_table(int p, HbBiasTable *t)
+{
+    const double *raw = HB_BIAS_RAW[p - HB_BIAS_PMIN], *val = HB_BIAS_VAL[p - HB_BIAS_PMIN];
+    memset(t, 0, sizeof *t);
+    for (int s = 0; s < HB_BIAS_WINDOWS; ++s) {  // summed in ascending order, as the rule does
+        double sum = 0.0;
+        for (int i = s; i < s + 6; ++i) sum += val[i];
+        t->mean[s] = sum / 6.0;
+    }
+    // bp[s]: smallest f32 e >= 0 whose window starts at s or later (bisection over the
+    // f32 bit patterns, which order the non-negative floats)
+    for (int s = 1; s < HB_BIAS_WINDOWS; ++s) {
+        uint32_t lo = 0, hi = 0x7f7fffffu;  // start(0) < s <= start(FLT_MAX) = 194
+        while (lo + 1 < hi) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            float f;
+            memcpy(&f, &mid, 4);
+            if (bias_window_start(raw, (double)f) >= s) hi = mid;
+            else lo = mid;
+        }
+        float f;
+        memcpy(&f, &hi, 4);
+        t->bp[s] = f;
+    }
+    t->bucket_base = (uint32_t)(127 + p - 1) << 8;
+    for (uint32_t b = 0; b < HB_BIAS_BUCKETS; ++b) {
+        const uint32_t bits = (t->bucket_base + b) << 15;  // lower edge of bucket b
+        float f;
+        memcpy(&f, &bits, 4);
+        t->bucket[b] = (uint8_t)bias_window_start(raw, (double)f);
+    }
+}
+}  // namespace
+
+// per-precision tables, built once per process
+const HbBiasTable *bias_table(int p)
+{
+    static std::mutex mu;
+    static std::unique_ptr<HbBiasTable> cache[HB_BIAS_PMAX + 1];
+    if (p < HB_BIAS_PMIN || p > HB_BIAS_PMAX) return nullptr;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!cache[p]) {
+        auto t = std::make_unique<HbBiasTable>();
+        build_bias_table(p, t.get());
+        cache[p] = std::move(t);
+    }
+    return cache[p].get();
+}
+}  // namespace hb
+
 namespace {
 
 thread_local std::string g_err;
@@ -103,7 +178,8 @@ struct hb_ctx {
     bool ran = false;
     DevBuf<uint8_t> regs0, regs1, scratch;
     DevBuf<float> est, est_tmp, lc;
-    DevBuf<double> dist, inv, bias, cent;
+    DevBuf<double> dist, inv, cent;
+    DevBuf<HbBiasTable> bias;
     DevBuf<uint32_t> bits0, bits1, sched_buf, chunk_flag;
     DevBuf<uint32_t> active, dyn;  // push iterations: bitmap; per-bin lists + 3 counts
     bool sym = false;              // staged CSR is symmetric (arc multiset fingerprints)
@@ -618,7 +694,7 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
     HB_TRY(c->rec.ensure(cap));
     c->rec_cap = cap;
     HB_TRY(c->lc.ensure(m + 1));
-    HB_TRY(c->bias.ensure(2 * HB_BIAS_K));
+    HB_TRY(c->bias.ensure(1));
     if (c->snap_k) HB_TRY(c->snaps.ensure((size_t)c->snap_k * c->n));
     const bool push_ok = c->sym && p <= 10;
     if (push_ok) HB_TRY(c->active.ensure(words));
@@ -633,10 +709,8 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
     std::vector<float> lc(m + 1, 0.0f);
     for (uint32_t z = 1; z <= m; ++z) lc[z] = (float)((double)m * std::log((double)m / (double)z));
     HB_TRY(hipMemcpyAsync(c->lc.ptr, lc.data(), (m + 1) * 4, hipMemcpyHostToDevice, c->stream));
-    HB_TRY(hipMemcpyAsync(c->bias.ptr, HB_BIAS_RAW[p - HB_BIAS_PMIN], HB_BIAS_K * 8,
-                          hipMemcpyHostToDevice, c->stream));
-    HB_TRY(hipMemcpyAsync(c->bias.ptr + HB_BIAS_K, HB_BIAS_VAL[p - HB_BIAS_PMIN], HB_BIAS_K * 8,
-                          hipMemcpyHostToDevice, c->stream));
+    HB_TRY(hipMemcpyAsync(c->bias.ptr, hb::bias_table(p), sizeof(HbBiasTable), hipMemcpyHostToDevice,
+                          c->stream));
 
     HbDev &d = c->dev;
     d = HbDev{};
@@ -654,8 +728,7 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
     d.st = c->st.ptr;
     d.rec = c->rec.ptr;
     d.lc = c->lc.ptr;
-    d.bias_raw = c->bias.ptr;
-    d.bias_val = c->bias.ptr + HB_BIAS_K;
+    d.bias = c->bias.ptr;
     d.scratch = c->scratch.ptr;
     d.chunk_flag = c->chunk_flag.ptr;
     d.sym = push_ok ? 1 : 0;
@@ -1429,6 +1502,21 @@ extern "C" int hb_run_many(hb_ctx **ctxs, const int *p, int count, int max_iters
     for (auto &th : pool) th.join();
     for (int i = 0; i < count; ++i)
         if (rc[i] != HB_OK) return fail(rc[i], "hb_run_many: context %d: %s", i, msg[i].c_str());
+    g_err.clear();
+    return HB_OK;
+}
+
+// The estimator's bias windows of precision p as the kernels read them (host-built;
+// no device work), for checking against the k-nearest-neighbour rule on the CPU.
+extern "C" int hb_bias_windows(int p, double *mean, float *breakpoints, uint8_t *buckets)
+{
+    if (p < HB_BIAS_PMIN || p > HB_BIAS_PMAX || !mean || !breakpoints || !buckets)
+        return fail(HB_EINVAL, "hb_bias_windows: p=%d not in %d..%d or NULL argument", p, HB_BIAS_PMIN,
+                    HB_BIAS_PMAX);
+    const HbBiasTable *t = hb::bias_table(p);
+    memcpy(mean, t->mean, HB_BIAS_WINDOWS * sizeof(double));
+    memcpy(breakpoints, t->bp, HB_BIAS_WINDOWS * sizeof(float));
+    memcpy(buckets, t->bucket, HB_BIAS_BUCKETS);
     g_err.clear();
     return HB_OK;
 }

@@ -21,8 +21,6 @@
 #include "hb_kernels.h"
 #include "hb_union.h"
 
-#define HB_BIAS_TABLE_QUAL static const
-#include "hll_bias_table.inc"
 
 namespace hb {
 
@@ -58,7 +56,7 @@ struct HbBatchDev {
     HbIterRec *rec;
     uint32_t *queue;        // work-queue head of the current iteration
     const float *lc_all;    // LC tables for p = 4..8 (kLcOff)
-    const double *bias_all; // bias points: [p-4][raw 200 | val 200]
+    const HbBiasTable *bias_all; // bias windows of p = 4..8
     int frontier;
 };
 
@@ -120,10 +118,10 @@ __device__ __forceinline__ void batch_tile_p(const HbBatchDev &b, const HbBatchG
 __global__ __launch_bounds__(kBatchBlock) void k_batch_union(HbBatchDev b)
 {
     __shared__ float s_lc[kLcTotal];
-    __shared__ double s_bias[kBatchPMax - kBatchPMin + 1][400];
+    __shared__ HbBiasTable s_bias[kBatchPMax - kBatchPMin + 1];
     for (int i = threadIdx.x; i < kLcTotal; i += blockDim.x) s_lc[i] = b.lc_all[i];
-    for (int i = threadIdx.x; i < (kBatchPMax - kBatchPMin + 1) * 400; i += blockDim.x)
-        s_bias[i / 400][i % 400] = b.bias_all[i];
+    for (int i = threadIdx.x; i < (int)(sizeof s_bias / 8); i += blockDim.x)
+        ((uint64_t *)s_bias)[i] = ((const uint64_t *)b.bias_all)[i];
     __syncthreads();
     const uint32_t lane = threadIdx.x & (HB_WAVE - 1);
     for (;;) {
@@ -135,8 +133,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_union(HbBatchDev b)
         const HbBatchGraph &g = b.g[tile.graph];
         if (g.st.done) continue;
         const int p = (int)g.p;
-        const HbTables tb = {s_lc + kLcOff[p], s_bias[p - kBatchPMin], s_bias[p - kBatchPMin] + 200,
-                             g.am, g.five_m, g.lc_thr};
+        const HbTables tb = {s_lc + kLcOff[p], &s_bias[p - kBatchPMin], g.am, g.five_m, g.lc_thr};
         HbCounts c;
         switch (p) {
         case 4: batch_tile_p<4>(b, g, tile, tb, c); break;
@@ -244,7 +241,8 @@ struct hb_batch {
     uint32_t *lists = nullptr;
     uint8_t *regs0 = nullptr, *regs1 = nullptr;
     float *est = nullptr, *lc_all = nullptr, *est_tmp = nullptr;
-    double *dist = nullptr, *inv = nullptr, *bias_all = nullptr;
+    double *dist = nullptr, *inv = nullptr;
+    HbBiasTable *bias_all = nullptr;
     float *lc_p[9] = {};                   // per-p LC tables for the single-graph launchers
     hb::HbBatchGraph *g = nullptr;
     hb::HbBatchTile *tiles = nullptr;
@@ -279,8 +277,7 @@ static HbDev graph_dev(hb_batch *b, int gi)
     d.dist = b->dist + g.node_base;
     d.inv = b->inv + g.node_base;
     d.lc = b->lc_p[g.p];
-    d.bias_raw = b->bias_all + (size_t)(g.p - hb::kBatchPMin) * 400;
-    d.bias_val = d.bias_raw + 200;
+    d.bias = b->bias_all + (g.p - hb::kBatchPMin);
     d.n = g.n;
     d.n_linked = g.n_linked;
     d.p = g.p;
@@ -434,9 +431,8 @@ int hb_batch_create(hb_batch **out, int device, const hb_graph_desc *graphs, int
     b->g = dmalloc<hb::HbBatchGraph>(count, &e); HB_BTRY(e);
     b->rec = dmalloc<HbIterRec>(recs, &e); HB_BTRY(e);
     b->lc_all = dmalloc<float>(hb::kLcTotal, &e); HB_BTRY(e);
-    b->bias_all = dmalloc<double>(5 * 400, &e); HB_BTRY(e);
+    b->bias_all = dmalloc<HbBiasTable>(hb::kBatchPMax - hb::kBatchPMin + 1, &e); HB_BTRY(e);
     std::vector<float> lc(hb::kLcTotal, 0.0f);
-    std::vector<double> bias(5 * 400);
     for (int p = hb::kBatchPMin; p <= hb::kBatchPMax; ++p) {
         const uint32_t m = 1u << p;
         std::vector<float> one(m + 1, 0.0f);
@@ -445,11 +441,10 @@ int hb_batch_create(hb_batch **out, int device, const hb_graph_desc *graphs, int
         b->lc_p[p] = dmalloc<float>(m + 1, &e);
         HB_BTRY(e);
         HB_BTRY(hipMemcpy(b->lc_p[p], one.data(), (m + 1) * 4, hipMemcpyHostToDevice));
-        std::copy(HB_BIAS_RAW[p - HB_BIAS_PMIN], HB_BIAS_RAW[p - HB_BIAS_PMIN] + 200, bias.begin() + (p - 4) * 400);
-        std::copy(HB_BIAS_VAL[p - HB_BIAS_PMIN], HB_BIAS_VAL[p - HB_BIAS_PMIN] + 200, bias.begin() + (p - 4) * 400 + 200);
+        HB_BTRY(hipMemcpy(b->bias_all + (p - hb::kBatchPMin), hb::bias_table(p), sizeof(HbBiasTable),
+                          hipMemcpyHostToDevice));
     }
     HB_BTRY(hipMemcpy(b->lc_all, lc.data(), lc.size() * 4, hipMemcpyHostToDevice));
-    HB_BTRY(hipMemcpy(b->bias_all, bias.data(), bias.size() * 8, hipMemcpyHostToDevice));
     // CSR upload (per graph, converted to u32)
     std::vector<uint32_t> tmp;
     for (int i = 0; i < count; ++i) {

@@ -46,6 +46,8 @@ struct HbIterRec {  // per-iteration counters, per kernel slot
     unsigned long long push;             // 1 if iteration t listed its nodes by push
 };
 
+struct HbBiasTable;  // HLL++ bias windows of one precision (below)
+
 struct HbDev {
     const uint32_t *off;   // n+1
     const uint32_t *nbr;   // arcs
@@ -60,7 +62,7 @@ struct HbDev {
     HbIterState *st;
     HbIterRec *rec;        // indexed by t
     const float *lc;       // lc[z] = (float)(m ln(m/z)), z = 1..m
-    const double *bias_raw, *bias_val;  // HLL++ points for this p
+    const HbBiasTable *bias;  // HLL++ bias windows for this p
     float *snaps;          // estimates after iterations 1..snap_k, iteration-major
     uint32_t snap_k;
     uint8_t *scratch;      // partial sketches of hub chunks
@@ -238,34 +240,40 @@ __device__ __forceinline__ uint32_t hb_copy_or(uint32_t v)
     return v;
 }
 
-// HLL++ EstimateBias: mean of the biases of the 6 table points nearest to e
-// (ties to the lower point), summed in ascending order in f64.
-__device__ __forceinline__ double hb_knn_bias(const double *__restrict__ raw,
-                                              const double *__restrict__ val, double e, int K)
+// HLL++ EstimateBias (k = 6 nearest raw-estimate points, ties to the lower point,
+// their biases summed in ascending order in f64 and divided by 6) as a lookup.  The
+// window of 6 consecutive points the rule picks for an estimate e is a monotone step
+// function of e, and e is an f32, so the host tabulates, per precision:
+//   bp[s]   (s >= 1) the smallest f32 e whose window starts at point s or later;
+//   mean[s] the window's mean bias, summed exactly as the rule sums it;
+//   bucket[b] the window start at the lower edge of bucket b, 256 buckets per octave
+//           over [m/2, 8m) (bucket = f32 bits >> 15, less bucket_base).
+// The device reads the bucket's start and steps up while e >= bp[s + 1] (at most one
+// or two steps: buckets are much narrower than the spacing of the points).
+#define HB_BIAS_WINDOWS 195   // 200 points, windows of 6 consecutive points
+#define HB_BIAS_BUCKETS 1024
+
+struct HbBiasTable {
+    double mean[HB_BIAS_WINDOWS + 1];
+    float bp[HB_BIAS_WINDOWS + 1];  // bp[0] unused
+    uint8_t bucket[HB_BIAS_BUCKETS];
+    uint32_t bucket_base;           // (127 + p - 1) << 8: bucket 0 starts at m/2
+    uint32_t pad;
+};
+
+__device__ __forceinline__ double hb_bias_lookup(const HbBiasTable *__restrict__ t, float e)
 {
-    int lo = 0, hi = K;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (raw[mid] > e) hi = mid;
-        else lo = mid + 1;
-    }
-    hi = lo;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        if (lo > 0 && (hi == K || e - raw[lo - 1] <= raw[hi] - e)) --lo;
-        else ++hi;
-    }
-    double s = 0.0;
-    for (int i = lo; i < hi; ++i) s = __dadd_rn(s, val[i]);
-    return s / 6.0;
+    const int32_t b = (int32_t)(__float_as_uint(e) >> 15) - (int32_t)t->bucket_base;
+    uint32_t s = b < 0 ? 0u : t->bucket[b < HB_BIAS_BUCKETS ? b : HB_BIAS_BUCKETS - 1];
+    while (s + 1 < HB_BIAS_WINDOWS && t->bp[s + 1] <= e) ++s;
+    return t->mean[s];
 }
 
 // Lookup tables of the estimator for one p: lc[z] = (float)(m ln(m/z)), z = 1..m,
-// and the HLL++ bias points (raw, bias), 200 each.  Kernels stage them in LDS.
+// and the HLL++ bias windows.  Kernels stage them in LDS.
 struct HbTables {
     const float *lc;
-    const double *raw;
-    const double *val;
+    const HbBiasTable *bias;
     float am;       // alpha_m * m * m, rounded to f32
     float five_m;   // 5m
     double lc_thr;  // HLL++ linear-counting threshold for p
@@ -273,14 +281,12 @@ struct HbTables {
 
 #define HB_TABLES_LDS(P, d, name)                                                      \
     __shared__ float name##_lc[(1 << (P)) + 1];                                        \
-    __shared__ double name##_raw[200], name##_val[200];                                \
+    __shared__ HbBiasTable name##_bias;                                                \
     for (uint32_t i_ = threadIdx.x; i_ <= (1u << (P)); i_ += blockDim.x) name##_lc[i_] = (d).lc[i_]; \
-    for (uint32_t i_ = threadIdx.x; i_ < 200u; i_ += blockDim.x) {                     \
-        name##_raw[i_] = (d).bias_raw[i_];                                             \
-        name##_val[i_] = (d).bias_val[i_];                                             \
-    }                                                                                  \
+    for (uint32_t i_ = threadIdx.x; i_ < sizeof(HbBiasTable) / 8; i_ += blockDim.x)   \
+        ((uint64_t *)&name##_bias)[i_] = ((const uint64_t *)(d).bias)[i_];           \
     __syncthreads();                                                                   \
-    const HbTables name = {name##_lc, name##_raw, name##_val, (d).am, (d).five_m, (d).lc_thr};
+    const HbTables name = {name##_lc, &name##_bias, (d).am, (d).five_m, (d).lc_thr};
 
 // Estimate of the sketch held by a G-lane copy (each lane: 16 registers in acc).
 // All lanes of the copy return the same value.
@@ -290,7 +296,7 @@ struct HbTables {
 //      k = sum_j 2^(mx - M_j); when k < 2^24 every partial sum of the sequential
 //      f32 loop is a multiple of 2^-mx below 2^(24-mx), hence exact, and
 //      S = k * 2^-mx.  Otherwise the sequential loop is replayed.
-//   3. raw <= 5m: subtract the HLL++ bias (k = 6 nearest neighbours).
+//   3. raw <= 5m: subtract the HLL++ bias (k = 6 nearest neighbours, tabulated).
 // f32 products and quotients are formed in f64 and rounded once, which gives the
 // correctly rounded f32 result (53 >= 2*24 + 2).
 template <int G>
@@ -324,6 +330,6 @@ __device__ __forceinline__ float hb_estimate_copy(const HbTables &tb, uint4 acc)
         }
     }
     float est = (float)((double)tb.am / (double)S);
-    if (est <= tb.five_m) est = (float)((double)est - hb_knn_bias(tb.raw, tb.val, (double)est, 200));
+    if (est <= tb.five_m) est = (float)((double)est - hb_bias_lookup(tb.bias, est));
     return est;
 }

@@ -26,6 +26,8 @@ struct HbSchedule {
 
 // thread-local message returned by hb_last_error() (set by every C-ABI entry point)
 void set_error(const char *msg);
+// host-built HLL++ bias windows of precision p (4..16), built once per process
+const HbBiasTable *bias_table(int p);
 
 hipError_t launch_init(const HbDev &d, hipStream_t s);
 hipError_t launch_fill(uint32_t *p, uint64_t count, uint32_t value, hipStream_t s);

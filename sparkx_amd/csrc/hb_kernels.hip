@@ -486,7 +486,7 @@ __device__ float estimate_wide(const HbDev &d, const uint32_t *h, const uint8_t 
         S = __shfl(S, 0);
     }
     float est = (float)((double)d.am / (double)S);
-    if (est <= d.five_m) est = (float)((double)est - hb_knn_bias(d.bias_raw, d.bias_val, (double)est, 200));
+    if (est <= d.five_m) est = (float)((double)est - hb_bias_lookup(d.bias, est));
     return est;
 }
 

@@ -252,3 +252,39 @@ def test_bench_refuses_more_than_one_gpu():
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU only" in r.stderr
+
+
+@pytest.mark.parametrize("p", list(range(4, 17)))
+def test_bias_windows_equal_the_nearest_neighbour_rule(oracle, p):
+    """The estimator's tabulated HLL++ bias (window breakpoints, window means and the
+    per-bucket starting window the kernels read) gives, for every f32 estimate in
+    [m/2, 5m], exactly the bias the k = 6 nearest-neighbour rule gives (the oracle's
+    search, expansion and ascending sum).  Estimates reaching the bias branch lie in
+    that range: the raw estimate is at least alpha m > m/2 (every register term is at
+    most 1) and the branch needs it at most 5m.  The device steps up at most 2 times
+    from a bucket's start."""
+    from sparkx_amd import _lib
+    L = _lib.load()
+    m = 1 << p
+    mean = np.zeros(195)
+    bp = np.zeros(195, np.float32)
+    bk = np.zeros(1024, np.uint8)
+    _lib.check(L.hb_bias_windows(p, _lib.ptr(mean, C.c_double), _lib.ptr(bp, C.c_float),
+                                 _lib.ptr(bk, C.c_uint8)))
+    assert np.all(np.diff(bp[1:]) > 0) and np.all(np.diff(bk.astype(int)) >= 0)
+    lo, hi = int(np.float32(m / 2).view(np.uint32)), int(np.float32(5 * m).view(np.uint32))
+    base = (127 + p - 1) << 8
+    for c0 in range(lo, hi + 1, 1 << 22):
+        bits = np.arange(c0, min(c0 + (1 << 22), hi + 1), dtype=np.uint32)
+        e = bits.view(np.float32)
+        b = (bits >> 15).astype(np.int64) - base
+        s = np.where(b < 0, 0, bk[np.clip(b, 0, 1023)]).astype(np.int64)
+        for _ in range(2):  # the device's walk: while e >= bp[s + 1], s += 1
+            s = s + ((s + 1 < 195) & (bp[np.minimum(s + 1, 194)] <= e))
+        assert not ((s + 1 < 195) & (bp[np.minimum(s + 1, 194)] <= e)).any(), "more than 2 steps"
+        got = mean[s]
+        ref = oracle.knn_bias(p, e)
+        bad = np.flatnonzero(got != ref)
+        assert bad.size == 0, (p, e[bad[:5]], got[bad[:5]], ref[bad[:5]])
+    assert L.hb_bias_windows(3, _lib.ptr(mean, C.c_double), _lib.ptr(bp, C.c_float),
+                             _lib.ptr(bk, C.c_uint8)) == _lib.HB_EINVAL
