@@ -37,6 +37,7 @@ CONFIGS = {
     "ingest": (dict(kind="ingest", scale=22, edge_factor=16, seed=1), 0, 0),
     "many64": (dict(kind="many", count=64, seed=1, threads=8), 0, 0),
     "walks": (dict(kind="walks", scale=24, edge_factor=16, seed=1), 7, 0),
+    "copy": (dict(kind="copy", gib=1), 0, 0),
 }
 
 
@@ -297,6 +298,46 @@ def bench_walks(args, spec):
     print(json.dumps(out), flush=True)
 
 
+def bench_copy(args, spec):
+    """Plain device-to-device copies of 1 GiB (the HIP runtime's copy kernel): the
+    reference point that rocprofv3's FETCH_SIZE / WRITE_SIZE readings of the HyperBall
+    kernels are scaled by (each copy reads and writes exactly `gib` GiB)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = spec["gib"] << 30
+    src, dst = C.c_void_p(), C.c_void_p()
+    ev = [C.c_void_p(), C.c_void_p()]
+    assert hip.hipMalloc(C.byref(src), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(dst), C.c_size_t(nbytes)) == 0
+    try:
+        assert hip.hipMemset(src, 1, C.c_size_t(nbytes)) == 0
+        for e in ev:
+            assert hip.hipEventCreate(C.byref(e)) == 0
+        for _ in range(args.warmup):
+            assert hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) == 0  # device to device
+        assert hip.hipDeviceSynchronize() == 0
+        assert hip.hipEventRecord(ev[0], None) == 0
+        for _ in range(args.steps):
+            assert hip.hipMemcpyAsync(dst, src, C.c_size_t(nbytes), 3, None) == 0
+        assert hip.hipEventRecord(ev[1], None) == 0
+        assert hip.hipEventSynchronize(ev[1]) == 0
+        ms = C.c_float()
+        assert hip.hipEventElapsedTime(C.byref(ms), ev[0], ev[1]) == 0
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
+    per = ms.value / max(args.steps, 1)
+    out = {"metric": "HBM copy (read + write) GB/s", "value": round(2 * nbytes / per / 1e6, 1), "unit": "GB/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(per, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8",
+           "data": "synthetic", "config": {"workload": "copy", "bytes_read_per_copy": nbytes,
+                                           "bytes_written_per_copy": nbytes},
+           "roofline": {"bound": "hbm", "achieved": round(2 * nbytes / per / 1e6, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(2 * nbytes / per / 1e6 / HBM_PEAK_GBS, 4), "traffic": None},
+           "cpu_baseline": None}
+    print(json.dumps(out), flush=True)
+
+
 def bench_many(args, spec):
     """The same 64 graphs (32 ER + 32 R-MAT, n log-uniform in [2^12, 2^18], average degree
     U[4, 32], p uniform in {4..8}) run to convergence together: one engine context and
@@ -494,6 +535,8 @@ def main():
         return bench_ingest(args, spec)
     if spec["kind"] == "many":
         return bench_many(args, spec)
+    if spec["kind"] == "copy":
+        return bench_copy(args, spec)
     if spec["kind"] == "walks":
         return bench_walks(args, spec)
     if spec["kind"] == "surrogate":
