@@ -1,16 +1,33 @@
 // hb_batch.hip -- many independent HyperBall runs (graphs of different sizes and
-// precisions p = 4..8) advanced together: one launch per iteration of a
-// persistent work-queue kernel over (graph, degree-bin) tiles, every graph with
-// its own iteration state, termination rule and buffer parity.  The iteration
-// (begin / union / finalize) is capturable in one hipGraph and replayed until
-// every graph has stopped.  Each graph's registers, estimates and accumulators
-// are bit-identical to its own single-graph run (same per-node union_node).
+// precisions p = 4..8) advanced together.  One iteration of the whole batch is
+//     k_batch_begin -> k_batch_union<4..8> -> k_batch_merge<p> (precisions with hubs)
+//     -> k_batch_finalize
+// captured once in a hipGraph and replayed until every graph has stopped.
+// k_batch_union<P> is a persistent kernel: its wavefronts pop tiles from a work
+// queue over the (graph, degree bin) tiles of every graph of precision P, largest
+// first; a tile is a run of nodes of one bin (team width G, 4G or 64 lanes per node,
+// G = 2^P / 16) or one chunk of a hub's neighbour list, whose partial union goes to
+// scratch and is merged into the hub by k_batch_merge<P>.  Every graph keeps its
+// own iteration state (t, 1/t, parity, frontier mode, stop flag) and applies the
+// reference's stopping rule on its own (hyper_ball.rs:833-849); a graph that has
+// stopped costs nothing.  The per-node work is union_node, the same function the
+// single-graph kernels run, so each graph's registers, estimates, accumulators and
+// per-iteration increase counts are its own hb_run's.
+//
+// Per-iteration counters live in a ring of kRing records per graph; the host copies
+// the new ones out at every check (at most kCheck launches apart, fewer than kRing),
+// so a run has no record cap.  A graph cannot need more than n + 1 iterations (the
+// last one changes nothing), so a graph still running after n + 2 stops the run with
+// an error rather than letting the host loop go on.  With SPARKX_HB_TRACE set in the
+// environment every check prints a progress line to stderr.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -21,25 +38,43 @@
 #include "hb_kernels.h"
 #include "hb_union.h"
 
-
 namespace hb {
 
 static constexpr int kBatchBlock = 256;
-static constexpr int kBatchPMin = 4, kBatchPMax = 8;
-// LC tables of p = 4..8 packed back to back: offsets of each p's table
-static constexpr int kLcOff[kBatchPMax + 1] = {0, 0, 0, 0, 0, 17, 17 + 33, 17 + 33 + 65, 17 + 33 + 65 + 129};
-static constexpr int kLcTotal = 17 + 33 + 65 + 129 + 257;
+static constexpr int kBatchPMin = 4, kBatchPMax = 8, kBatchNP = kBatchPMax - kBatchPMin + 1;
+static constexpr uint32_t kRing = 32;    // per-graph iteration records on the device
+static constexpr uint32_t kCheck = 8;    // launches between host checks (< kRing)
+static constexpr uint32_t kScrRow = 256; // scratch bytes per hub chunk (2^p, p <= 8)
+// work classes of one precision: node bins of team width G, 4G and 64, hub chunks
+static constexpr int kClasses = 4, kClassChunk = 3;
+static constexpr int kQueues = kBatchNP * kClasses;
+
+struct HbBatchRec {  // counters of one iteration of one graph
+    unsigned long long inc, chg, arcs, nodes, gathered;
+};
 
 struct HbBatchGraph {
-    uint64_t off_base, nbr_base, reg_base, node_base, bits_base, rec_base;
-    uint32_t n, p, n_linked, words, rec_cap, overflow;
+    uint64_t off_base, nbr_base, reg_base, node_base, bits_base;
+    uint32_t n, p, n_linked, words;
+    uint32_t ring_base;   // first of the graph's kRing records
+    uint32_t cap;         // iterations after which a running graph is an error (n + 2)
+    uint32_t overflow;    // set when the graph reached cap without stopping
+    uint32_t pad;
     float am, five_m;
     double lc_thr;
     HbIterState st;
 };
 
+// a work-queue item of one precision: `team` lanes per node over lists[first ..
+// first + count) (graph-local ids), or, with team == 0, hub chunk task `first`
 struct HbBatchTile {
     uint32_t graph, first, count, team;
+};
+struct HbBatchTask {  // arcs [beg, end) of node's list (graph-local)
+    uint32_t graph, node, beg, end;
+};
+struct HbBatchHub {   // node's chunks are tasks task0 .. task0 + ntask - 1
+    uint32_t graph, node, task0, ntask;
 };
 
 struct HbBatchDev {
@@ -50,37 +85,56 @@ struct HbBatchDev {
     uint32_t *bits0, *bits1;
     HbBatchGraph *g;
     uint32_t ngraphs;
-    const HbBatchTile *tiles;
-    uint32_t ntiles;
-    const uint32_t *lists;  // graph-local node ids, tiles index into it
-    HbIterRec *rec;
-    uint32_t *queue;        // work-queue head of the current iteration
-    const float *lc_all;    // LC tables for p = 4..8 (kLcOff)
-    const HbBiasTable *bias_all; // bias windows of p = 4..8
+    HbBatchRec *rec;
+    uint32_t *queue;            // [kQueues] work-queue heads, one per (precision, class)
+    const uint32_t *lists;
+    const HbBatchTask *tasks;
+    uint8_t *scratch;           // kScrRow bytes per task
+    uint32_t *chunk_flag;       // 1 if the task was recomputed this iteration
+    const float *lc;            // LC tables of p = 4..8 back to back (lc_off)
+    const HbBiasTable *bias;    // bias windows of p = 4..8
     int frontier;
 };
 
-__global__ __launch_bounds__(kBatchBlock) void k_batch_begin(HbBatchDev b)
+struct HbBatchList {  // the tiles (or hubs) of one (precision, class) and its queue
+    const HbBatchTile *tiles;
+    const HbBatchHub *hubs;
+    uint32_t count;
+    uint32_t queue;
+};
+
+// x as a wave-uniform value (every lane holds lane 0's copy, in scalar registers):
+// values loaded through a per-wave index are uniform, but the compiler cannot prove
+// it and would keep every field of a graph's state in vector registers
+template <class T>
+__device__ __forceinline__ T wave_uniform(const T &x)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) *b.queue = 0u;
-    for (uint32_t gi = blockIdx.x; gi < b.ngraphs; gi += gridDim.x) {
-        const HbBatchGraph &g = b.g[gi];
-        if (g.st.done) continue;
-        uint32_t *bw = ((g.st.iter & 1u) ? b.bits1 : b.bits0) + g.bits_base;
-        for (uint32_t i = threadIdx.x; i < g.words; i += blockDim.x) bw[i] = 0u;
-    }
+    static_assert(sizeof(T) % 4 == 0, "whole 32-bit words");
+    uint32_t w[sizeof(T) / 4];
+    memcpy(w, &x, sizeof(T));
+#pragma unroll
+    for (uint32_t k = 0; k < sizeof(T) / 4; ++k) w[k] = __builtin_amdgcn_readfirstlane(w[k]);
+    T y;
+    memcpy(&y, w, sizeof(T));
+    return y;
 }
 
-template <int P, int T>
-__device__ __forceinline__ void batch_tile(const HbBatchDev &b, const HbBatchGraph &g, const HbBatchTile &tile,
-                                           const HbTables &tb, HbCounts &c)
+__host__ __device__ constexpr uint32_t lc_off(int p) { return p <= 4 ? 0u : lc_off(p - 1) + (1u << (p - 1)) + 1u; }
+static constexpr uint32_t kLcTotal = lc_off(kBatchPMax + 1);
+
+__device__ __forceinline__ HbBatchRec *ring_rec(const HbBatchDev &b, const HbBatchGraph &g, uint32_t it)
 {
-    const uint32_t lane = threadIdx.x & (HB_WAVE - 1);
+    return b.rec + g.ring_base + it % kRing;
+}
+
+// the iteration's view of graph g (as the single-graph kernels build it)
+__device__ __forceinline__ HbView batch_view(const HbBatchDev &b, const HbBatchGraph &g, uint32_t *s_first)
+{
     const uint32_t it = g.st.iter;
     HbView v;
     v.off = b.off + g.off_base;
     v.nbr = b.nbr + g.nbr_base;
-    v.orig = v.nbr;
+    v.orig = v.nbr;          // ids are hashed as given (no renumbering in a batch)
     v.node_id = nullptr;
     v.in = (g.st.parity ? b.regs1 : b.regs0) + g.reg_base;
     v.out = (g.st.parity ? b.regs0 : b.regs1) + g.reg_base;
@@ -92,70 +146,199 @@ __device__ __forceinline__ void batch_tile(const HbBatchDev &b, const HbBatchGra
     v.t = g.st.t;
     v.inv_t = g.st.inv_t;
     v.use_frontier = b.frontier && it > 1 && !g.st.skip_scan;
-    v.first = false;
-    v.gather_bits = nullptr;
-    v.scr = nullptr;
-    const uint32_t *list = b.lists + tile.first;
-    for (uint32_t i = lane / T; i < tile.count; i += HB_WAVE / T) union_node<P, T>(v, tb, list[i], lane % T, c);
+    v.gather_bits = v.use_frontier ? v.bitsR : nullptr;
+    v.first = it == 1;       // p <= 8: iteration 1 hashes neighbour ids instead of gathering
+    v.scr = s_first;
+    return v;
 }
 
-template <int P>
-__device__ __forceinline__ void batch_tile_p(const HbBatchDev &b, const HbBatchGraph &g, const HbBatchTile &tile,
-                                             const HbTables &tb, HbCounts &c)
+template <int P, int T>
+__device__ __forceinline__ void batch_nodes(const HbView &v, const HbTables &tb, const uint32_t *list,
+                                            uint32_t count, HbCounts &c)
 {
-    constexpr int G = (1 << P) / 16;
-    switch (tile.team) {
-    case 1: if constexpr (G <= 1) batch_tile<P, 1>(b, g, tile, tb, c); break;
-    case 2: if constexpr (G <= 2) batch_tile<P, 2>(b, g, tile, tb, c); break;
-    case 4: if constexpr (G <= 4) batch_tile<P, 4>(b, g, tile, tb, c); break;
-    case 8: if constexpr (G <= 8) batch_tile<P, 8>(b, g, tile, tb, c); break;
-    case 16: batch_tile<P, 16>(b, g, tile, tb, c); break;
-    case 32: batch_tile<P, 32>(b, g, tile, tb, c); break;
-    default: batch_tile<P, 64>(b, g, tile, tb, c); break;
+    const uint32_t lane = threadIdx.x & (HB_WAVE - 1);
+    if (v.gather_bits) {
+        for (uint32_t i = lane / T; i < count; i += HB_WAVE / T) union_node<P, T, true>(v, tb, list[i], lane % T, c);
+    } else {
+        for (uint32_t i = lane / T; i < count; i += HB_WAVE / T) union_node<P, T, false>(v, tb, list[i], lane % T, c);
     }
 }
 
-__global__ __launch_bounds__(kBatchBlock) void k_batch_union(HbBatchDev b)
+// one hub chunk: partial union of its neighbours into scratch (as k_chunk)
+template <int P>
+__device__ __forceinline__ void batch_chunk(const HbBatchDev &b, const HbView &v, uint32_t task_id,
+                                            const HbBatchTask &task, HbCounts &c)
 {
-    __shared__ float s_lc[kLcTotal];
-    __shared__ HbBiasTable s_bias[kBatchPMax - kBatchPMin + 1];
-    for (int i = threadIdx.x; i < kLcTotal; i += blockDim.x) s_lc[i] = b.lc_all[i];
-    for (int i = threadIdx.x; i < (int)(sizeof s_bias / 8); i += blockDim.x)
-        ((uint64_t *)s_bias)[i] = ((const uint64_t *)b.bias_all)[i];
-    __syncthreads();
-    const uint32_t lane = threadIdx.x & (HB_WAVE - 1);
-    for (;;) {
-        uint32_t ti = 0;
-        if (lane == 0) ti = atomicAdd(b.queue, 1u);
-        ti = __shfl(ti, 0);
-        if (ti >= b.ntiles) break;
-        const HbBatchTile tile = b.tiles[ti];
-        const HbBatchGraph &g = b.g[tile.graph];
-        if (g.st.done) continue;
-        const int p = (int)g.p;
-        const HbTables tb = {s_lc + kLcOff[p], &s_bias[p - kBatchPMin], g.am, g.five_m, g.lc_thr};
-        HbCounts c;
-        switch (p) {
-        case 4: batch_tile_p<4>(b, g, tile, tb, c); break;
-        case 5: batch_tile_p<5>(b, g, tile, tb, c); break;
-        case 6: batch_tile_p<6>(b, g, tile, tb, c); break;
-        case 7: batch_tile_p<7>(b, g, tile, tb, c); break;
-        default: batch_tile_p<8>(b, g, tile, tb, c); break;
-        }
+    constexpr uint32_t G = (1u << P) / 16, T = HB_WAVE, S = T / G;
+    const uint32_t tl = threadIdx.x & (HB_WAVE - 1), cl = tl % G, slot = tl / G;
+    const uint32_t *__restrict__ nb = v.nbr + task.beg;
+    const uint32_t len = task.end - task.beg;
+    bool active = true;
+    if (v.use_frontier) {
+        uint32_t any = 0;
+        for (uint32_t j = tl; j < len; j += T) any |= bit_test(v.bitsR, nb[j]);
+        active = hb_copy_or<T>(any) != 0;
+    }
+    if (tl == 0) b.chunk_flag[task_id] = active ? 1u : 0u;
+    if (!active) return;
+    uint32_t gathered = 0;
+    uint4 acc = v.use_frontier
+                    ? hb_gather_union<P, S, true>(v.in, v.first, v.scr, v.bitsR, nb, nb, len, slot, cl,
+                                                  make_uint4(0, 0, 0, 0), gathered)
+                    : hb_gather_union<P, S, false>(v.in, v.first, v.scr, nullptr, nb, nb, len, slot, cl,
+                                                   make_uint4(0, 0, 0, 0), gathered);
+    if (cl == 0) c.gathered += gathered;
 #pragma unroll
-        for (int s = 1; s < HB_WAVE; s <<= 1) {
-            c.inc += __shfl_xor(c.inc, s);
-            c.arcs += __shfl_xor(c.arcs, s);
-            c.nodes += __shfl_xor(c.nodes, s);
-            c.chg += __shfl_xor(c.chg, s);
+    for (uint32_t s = G; s < T; s <<= 1) acc = hb_bmax4(acc, hb_shfl_xor4<T>(acc, s));
+    if (slot == 0) *(uint4 *)(b.scratch + (uint64_t)task_id * kScrRow + cl * 16) = acc;
+    if (tl == 0) c.arcs += len;
+}
+
+// the LDS estimator tables of precision P from the batch's per-precision arrays
+#define HB_BATCH_TABLES(P, b, name)                                                        \
+    __shared__ float name##_lc[(1 << (P)) + 1];                                            \
+    __shared__ HbBiasTable name##_bias;                                                    \
+    for (uint32_t i_ = threadIdx.x; i_ <= (1u << (P)); i_ += blockDim.x)                   \
+        name##_lc[i_] = (b).lc[lc_off(P) + i_];                                            \
+    for (uint32_t i_ = threadIdx.x; i_ < sizeof(HbBiasTable) / 8; i_ += blockDim.x)       \
+        ((uint64_t *)&name##_bias)[i_] = ((const uint64_t *)((b).bias + ((P) - kBatchPMin)))[i_]; \
+    __syncthreads();
+
+__device__ __forceinline__ void wave_sum_counts(HbCounts &c)
+{
+#pragma unroll
+    for (int s = 1; s < HB_WAVE; s <<= 1) {
+        c.inc += __shfl_xor(c.inc, s);
+        c.chg += __shfl_xor(c.chg, s);
+        c.arcs += __shfl_xor(c.arcs, s);
+        c.nodes += __shfl_xor(c.nodes, s);
+        c.gathered += __shfl_xor(c.gathered, s);
+    }
+}
+
+__device__ __forceinline__ void add_counts(HbBatchRec *r, const HbCounts &c)
+{
+    if (c.inc) atomicAdd(&r->inc, c.inc);
+    if (c.chg) atomicAdd(&r->chg, c.chg);
+    if (c.arcs) atomicAdd(&r->arcs, c.arcs);
+    if (c.nodes) atomicAdd(&r->nodes, c.nodes);
+    if (c.gathered) atomicAdd(&r->gathered, c.gathered);
+}
+
+// Next tile of a work queue for the whole wavefront (wave-uniform), or `count`.
+__device__ __forceinline__ uint32_t next_tile(const HbBatchDev &b, const HbBatchList &l)
+{
+    uint32_t ti = 0;
+    if ((threadIdx.x & (HB_WAVE - 1)) == 0) ti = atomicAdd(b.queue + l.queue, 1u);
+    ti = __builtin_amdgcn_readfirstlane(__shfl(ti, 0));
+    return ti < l.count ? ti : l.count;
+}
+
+// Persistent union over the node tiles of one (precision P, team width T) class.
+// Every loop below has a trip count fixed by its tile (at most 4096 nodes), and the
+// queue head only grows, so every wavefront leaves once the head passes the count.
+template <int P, int T>
+__global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(P >= 7 ? 7 : 8))) void k_batch_nodes(
+    HbBatchDev b, HbBatchList l)
+{
+    HB_BATCH_TABLES(P, b, tbl)
+    __shared__ uint32_t s_first[kBatchBlock * 16];
+    const uint32_t lane = threadIdx.x & (HB_WAVE - 1);
+    for (uint32_t ti = next_tile(b, l); ti < l.count; ti = next_tile(b, l)) {
+        const HbBatchTile tile = wave_uniform(l.tiles[ti]);
+        const HbBatchGraph g = wave_uniform(b.g[tile.graph]);
+        if (g.st.done) continue;
+        const HbTables tb = {tbl_lc, &tbl_bias, g.am, g.five_m, g.lc_thr};
+        const HbView v = batch_view(b, g, s_first);
+        HbCounts c;
+        batch_nodes<P, T>(v, tb, b.lists + tile.first, tile.count, c);
+        wave_sum_counts(c);
+        if (lane == 0) add_counts(ring_rec(b, g, g.st.iter), c);
+    }
+}
+
+// Persistent partial unions of the hub chunks of precision P, one wavefront per chunk.
+template <int P>
+__global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(P >= 7 ? 7 : 8))) void k_batch_chunk(
+    HbBatchDev b, HbBatchList l)
+{
+    __shared__ uint32_t s_first[kBatchBlock * 16];
+    const uint32_t lane = threadIdx.x & (HB_WAVE - 1);
+    for (uint32_t ti = next_tile(b, l); ti < l.count; ti = next_tile(b, l)) {
+        const HbBatchTile tile = wave_uniform(l.tiles[ti]);
+        const HbBatchGraph g = wave_uniform(b.g[tile.graph]);
+        if (g.st.done) continue;
+        const HbView v = batch_view(b, g, s_first);
+        HbCounts c;
+        batch_chunk<P>(b, v, tile.first, wave_uniform(b.tasks[tile.first]), c);
+        wave_sum_counts(c);
+        if (lane == 0) add_counts(ring_rec(b, g, g.st.iter), c);
+    }
+}
+
+// hubs of precision P: one wavefront per hub merges its own registers with the
+// partials of its recomputed chunks (as k_merge)
+template <int P>
+__global__ __launch_bounds__(kBatchBlock) void k_batch_merge(HbBatchDev b, HbBatchList l)
+{
+    constexpr uint32_t G = (1u << P) / 16, T = HB_WAVE, S = T / G;
+    HB_BATCH_TABLES(P, b, tbl)
+    const uint32_t tl = threadIdx.x & (HB_WAVE - 1), cl = tl % G, slot = tl / G;
+    const uint32_t waves = gridDim.x * (kBatchBlock / HB_WAVE);
+    const uint32_t w0 = __builtin_amdgcn_readfirstlane(blockIdx.x * (kBatchBlock / HB_WAVE) + threadIdx.x / HB_WAVE);
+    for (uint32_t h = w0; h < l.count; h += waves) {
+        const HbBatchHub hub = wave_uniform(l.hubs[h]);
+        const HbBatchGraph g = wave_uniform(b.g[hub.graph]);
+        if (g.st.done) continue;
+        const HbTables tb = {tbl_lc, &tbl_bias, g.am, g.five_m, g.lc_thr};
+        const HbView v = batch_view(b, g, nullptr);
+        const uint32_t u = hub.node;
+        HbCounts c;
+        uint32_t any = 0;
+        for (uint32_t j = tl; j < hub.ntask; j += T) any |= b.chunk_flag[hub.task0 + j];
+        if (hb_copy_or<T>(any) == 0) {
+            // no chunk saw a changed neighbour: c_t(u) = c_{t-1}(u)
+            if (slot == 0 && bit_test(v.bitsR, u))
+                *(uint4 *)(v.out + ((uint64_t)u << P) + cl * 16) = *(const uint4 *)(v.in + ((uint64_t)u << P) + cl * 16);
+            continue;
         }
-        if (lane == 0) {
-            HbIterRec *r = b.rec + g.rec_base + g.st.iter;
-            if (c.inc) atomicAdd(&r->inc[0], c.inc);
-            if (c.arcs) atomicAdd(&r->arcs[0], c.arcs);
-            if (c.nodes) atomicAdd(&r->nodes[0], c.nodes);
-            if (c.chg) atomicAdd(&r->chg[0], c.chg);
+        const uint4 own = v.first ? hb_own_from_word(hb_insertion<P>(u), cl)
+                                  : *(const uint4 *)(v.in + ((uint64_t)u << P) + cl * 16);
+        uint4 acc = own;
+        for (uint32_t j = slot; j < hub.ntask; j += S)
+            if (b.chunk_flag[hub.task0 + j])
+                acc = hb_bmax4(acc, *(const uint4 *)(b.scratch + (uint64_t)(hub.task0 + j) * kScrRow + cl * 16));
+#pragma unroll
+        for (uint32_t s = G; s < T; s <<= 1) acc = hb_bmax4(acc, hb_shfl_xor4<T>(acc, s));
+        const uint32_t diffw = (acc.x != own.x) | (acc.y != own.y) | (acc.z != own.z) | (acc.w != own.w);
+        const bool changed = hb_copy_or<G>(diffw) != 0;
+        const float curr = hb_estimate_copy<G>(tb, acc);
+        if (slot == 0) *(uint4 *)(v.out + ((uint64_t)u << P) + cl * 16) = acc;
+        if (tl == 0) {
+            const float prev = v.est[u];
+            if (accumulate(v, u, prev, curr)) c.inc += 1;
+            v.est[u] = curr;
+            if (changed) {
+                atomicOr(v.bitsW + (u >> 5), 1u << (u & 31u));
+                c.chg += 1;
+            }
+            c.nodes += 1;
+            add_counts(ring_rec(b, g, g.st.iter), c);
         }
+    }
+}
+
+// start of an iteration: empty queues; for every running graph clear the bitmap
+// this iteration writes and its ring record
+__global__ __launch_bounds__(kBatchBlock) void k_batch_begin(HbBatchDev b)
+{
+    if (blockIdx.x == 0 && threadIdx.x < kQueues) b.queue[threadIdx.x] = 0u;
+    for (uint32_t gi = blockIdx.x; gi < b.ngraphs; gi += gridDim.x) {
+        const HbBatchGraph &g = b.g[gi];
+        if (g.st.done) continue;
+        uint32_t *bw = ((g.st.iter & 1u) ? b.bits1 : b.bits0) + g.bits_base;
+        for (uint32_t i = threadIdx.x; i < g.words; i += blockDim.x) bw[i] = 0u;
+        if (threadIdx.x == 0) *ring_rec(b, g, g.st.iter) = HbBatchRec{0, 0, 0, 0, 0};
     }
 }
 
@@ -168,20 +351,20 @@ __global__ void k_batch_finalize(HbBatchDev b)
     HbIterState &st = g.st;
     if (st.done) return;
     const uint32_t it = st.iter;
-    const HbIterRec &r = b.rec[g.rec_base + it];
-    const unsigned long long inc = r.inc[0], chg = r.chg[0];
-    if (inc == 0 || (st.max_iters && it >= st.max_iters) || it + 2 >= g.rec_cap) {
+    const HbBatchRec &r = *ring_rec(b, g, it);
+    const bool truncated = st.max_iters && it >= st.max_iters;
+    if (r.inc == 0 || truncated || it >= g.cap) {
         st.done = 1;
         st.executed = it;
-        st.converged = inc == 0;
-        g.overflow = inc != 0 && !(st.max_iters && it >= st.max_iters);
+        st.converged = r.inc == 0;
+        g.overflow = r.inc != 0 && !truncated;
         return;
     }
     st.iter = it + 1;
     st.parity ^= 1u;
     st.t = st.t + 1.0;
     st.inv_t = __ddiv_rn(1.0, st.t);
-    st.skip_scan = 2 * chg > (unsigned long long)g.n_linked;
+    st.skip_scan = 2 * r.chg > (unsigned long long)g.n_linked;
 }
 
 }  // namespace hb
@@ -221,11 +404,32 @@ float alpha_f32(uint32_t m)
 }
 
 template <class T>
-T *dmalloc(size_t count, hipError_t *err)
+struct DevArr {
+    T *ptr = nullptr;
+    size_t count = 0;
+    hipError_t alloc(size_t n)
+    {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        count = n;
+        return hipMalloc(&ptr, std::max<size_t>(n, 1) * sizeof(T));
+    }
+    hipError_t put(const std::vector<T> &h, hipStream_t s)
+    {
+        const hipError_t e = alloc(h.size());
+        if (e != hipSuccess || h.empty()) return e;
+        return hipMemcpyAsync(ptr, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s);
+    }
+    ~DevArr()
+    {
+        if (ptr) (void)hipFree(ptr);
+    }
+};
+
+bool trace_on()
 {
-    T *p = nullptr;
-    *err = hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
-    return *err == hipSuccess ? p : nullptr;
+    const char *e = std::getenv("SPARKX_HB_TRACE");
+    return e && *e && strcmp(e, "0") != 0;
 }
 
 }  // namespace
@@ -233,34 +437,30 @@ T *dmalloc(size_t count, hipError_t *err)
 struct hb_batch {
     int device = 0;
     hipStream_t stream = nullptr;
-    std::vector<hb::HbBatchGraph> hg;       // host copy (bases, constants)
+    std::vector<hb::HbBatchGraph> hg;  // host copy (bases, constants)
     std::vector<std::vector<uint32_t>> h_off;
     std::vector<int> max_iters;
-    uint64_t total_nodes = 0, total_arcs = 0, total_regs = 0, total_words = 0, total_rec = 0;
-    uint32_t *off = nullptr, *nbr = nullptr, *bits0 = nullptr, *bits1 = nullptr, *queue = nullptr;
-    uint32_t *lists = nullptr;
-    uint8_t *regs0 = nullptr, *regs1 = nullptr;
-    float *est = nullptr, *lc_all = nullptr, *est_tmp = nullptr;
-    double *dist = nullptr, *inv = nullptr;
-    HbBiasTable *bias_all = nullptr;
-    float *lc_p[9] = {};                   // per-p LC tables for the single-graph launchers
-    hb::HbBatchGraph *g = nullptr;
-    hb::HbBatchTile *tiles = nullptr;
-    uint32_t ntiles = 0;
-    HbIterRec *rec = nullptr;
-    uint32_t small_max = 8, mid_max = 64, tile_arcs = 16384;
+    uint64_t total_nodes = 0, total_arcs = 0, total_regs = 0, total_words = 0;
+    DevArr<uint32_t> off, nbr, bits0, bits1, queue, lists, chunk_flag;
+    DevArr<uint8_t> regs0, regs1, scratch;
+    DevArr<float> est, lc_all, est_tmp;
+    DevArr<double> dist, inv;
+    DevArr<HbBiasTable> bias_all;
+    DevArr<hb::HbBatchGraph> g;
+    DevArr<hb::HbBatchRec> rec;
+    DevArr<hb::HbBatchTile> tiles[hb::kQueues];  // per (precision, class)
+    DevArr<hb::HbBatchHub> hubs[hb::kBatchNP];
+    DevArr<hb::HbBatchTask> tasks;
+    float *lc_p[hb::kBatchPMax + 1] = {};  // device pointers into lc_all, per p
+    uint32_t ntiles[hb::kQueues] = {}, nhubs[hb::kBatchNP] = {}, ntasks = 0;
+    unsigned grid[hb::kQueues] = {};
+    uint32_t small_max = 0, mid_max = 0, tile_arcs = 16384;  // 0: per-precision defaults
     bool sched_ok = false, ran = false;
     std::vector<hb::HbBatchGraph> h_final;
-    std::vector<HbIterRec> h_rec;
+    std::vector<std::vector<hb::HbBatchRec>> history;  // per graph, iteration t at [t - 1]
 
     ~hb_batch()
     {
-        void *ptrs[] = {off, nbr, bits0, bits1, queue, lists, regs0, regs1, est, lc_all, est_tmp,
-                        dist, inv, bias_all, g, tiles, rec};
-        for (void *p : ptrs)
-            if (p) (void)hipFree(p);
-        for (float *p : lc_p)
-            if (p) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -269,15 +469,15 @@ static HbDev graph_dev(hb_batch *b, int gi)
 {
     const hb::HbBatchGraph &g = b->hg[gi];
     HbDev d{};
-    d.off = b->off + g.off_base;
-    d.nbr = b->nbr + g.nbr_base;
-    d.regs0 = b->regs0 + g.reg_base;
-    d.regs1 = b->regs1 + g.reg_base;
-    d.est = b->est + g.node_base;
-    d.dist = b->dist + g.node_base;
-    d.inv = b->inv + g.node_base;
+    d.off = b->off.ptr + g.off_base;
+    d.nbr = b->nbr.ptr + g.nbr_base;
+    d.regs0 = b->regs0.ptr + g.reg_base;
+    d.regs1 = b->regs1.ptr + g.reg_base;
+    d.est = b->est.ptr + g.node_base;
+    d.dist = b->dist.ptr + g.node_base;
+    d.inv = b->inv.ptr + g.node_base;
     d.lc = b->lc_p[g.p];
-    d.bias = b->bias_all + (g.p - hb::kBatchPMin);
+    d.bias = b->bias_all.ptr + (g.p - hb::kBatchPMin);
     d.n = g.n;
     d.n_linked = g.n_linked;
     d.p = g.p;
@@ -287,63 +487,100 @@ static HbDev graph_dev(hb_batch *b, int gi)
     return d;
 }
 
+// degree bounds of one graph's bins (the single-graph engine's defaults unless set)
+static void bin_bounds(const hb_batch *b, uint32_t p, uint32_t *small_max, uint32_t *mid_max, uint32_t *wave_max,
+                       uint32_t *chunk)
+{
+    *chunk = 2048;
+    *small_max = b->small_max ? b->small_max : (p <= 5 ? 64u : 256u);
+    *wave_max = p <= 5 ? 16384u : 2048u;
+    *wave_max = std::max(*wave_max, *small_max);
+    *mid_max = b->mid_max ? std::max(b->mid_max, *small_max) : std::max(*small_max, std::min(*wave_max, 8 * *small_max));
+    *mid_max = std::min(*mid_max, *wave_max);
+}
+
 static int build_tiles(hb_batch *b)
 {
     if (b->sched_ok) return HB_OK;
-    std::vector<uint32_t> lists;
-    struct T {
+    struct Item {
         hb::HbBatchTile t;
         uint64_t work;
     };
-    std::vector<T> tiles;
+    std::vector<uint32_t> lists;
+    std::vector<hb::HbBatchTask> tasks;
+    std::vector<Item> items[hb::kQueues];
+    std::vector<hb::HbBatchHub> hubs[hb::kBatchNP];
     for (size_t gi = 0; gi < b->hg.size(); ++gi) {
+        const hb::HbBatchGraph &hgi = b->hg[gi];
         const auto &off = b->h_off[gi];
-        const uint32_t p = b->hg[gi].p, G = (1u << p) / 16u;
+        const uint32_t p = hgi.p, G = (1u << p) / 16u, pi = p - hb::kBatchPMin;
         const uint32_t teams[3] = {G, std::min(64u, 4 * G), 64u};
+        uint32_t small_max, mid_max, wave_max, chunk;
+        bin_bounds(b, p, &small_max, &mid_max, &wave_max, &chunk);
         std::vector<uint32_t> bins[3];
-        for (uint32_t u = 0; u < b->hg[gi].n; ++u) {
+        for (uint32_t u = 0; u < hgi.n; ++u) {
             const uint32_t d = off[u + 1] - off[u];
             if (d == 0) continue;
-            bins[d <= b->small_max ? 0 : (d <= b->mid_max ? 1 : 2)].push_back(u);
+            if (d <= small_max) bins[0].push_back(u);
+            else if (d <= mid_max) bins[1].push_back(u);
+            else if (d <= wave_max) bins[2].push_back(u);
+            else {  // hub: chunks of `chunk` arcs, each one tile of its own
+                hb::HbBatchHub h{(uint32_t)gi, u, (uint32_t)tasks.size(), 0};
+                for (uint32_t e = off[u]; e < off[u + 1]; e += chunk, ++h.ntask) {
+                    const uint32_t end = std::min(e + chunk, off[u + 1]);
+                    items[pi * hb::kClasses + hb::kClassChunk].push_back(
+                        {{(uint32_t)gi, (uint32_t)tasks.size(), 1, 0}, (uint64_t)(end - e + 8) << p});
+                    tasks.push_back({(uint32_t)gi, u, e, end});
+                }
+                hubs[pi].push_back(h);
+            }
         }
         for (int k = 0; k < 3; ++k) {
-            size_t i = 0;
             const uint32_t per_wave = 64u / teams[k];
+            size_t i = 0;
             while (i < bins[k].size()) {
-                T tt{};
-                tt.t.graph = (uint32_t)gi;
-                tt.t.first = (uint32_t)lists.size();
-                tt.t.team = teams[k];
+                Item it{};
+                it.t.graph = (uint32_t)gi;
+                it.t.first = (uint32_t)lists.size();
+                it.t.team = teams[k];
                 uint64_t arcs = 0;
                 uint32_t cnt = 0;
+                // a tile holds whole waves' worth of nodes until it has tile_arcs arcs
                 while (i < bins[k].size() && (arcs < b->tile_arcs || cnt % per_wave) && cnt < 4096) {
                     const uint32_t u = bins[k][i++];
                     lists.push_back(u);
                     arcs += off[u + 1] - off[u] + 8;
                     ++cnt;
                 }
-                tt.t.count = cnt;
-                tt.work = arcs << p;
-                tiles.push_back(tt);
+                it.t.count = cnt;
+                it.work = arcs << p;
+                items[pi * hb::kClasses + k].push_back(it);
             }
         }
     }
-    // longest tiles first: the dynamic queue then balances the tail
-    std::stable_sort(tiles.begin(), tiles.end(), [](const T &a, const T &c) { return a.work > c.work; });
-    std::vector<hb::HbBatchTile> ht(tiles.size());
-    for (size_t i = 0; i < tiles.size(); ++i) ht[i] = tiles[i].t;
-    if (b->lists) (void)hipFree(b->lists);
-    if (b->tiles) (void)hipFree(b->tiles);
-    b->lists = nullptr;
-    b->tiles = nullptr;
-    hipError_t e;
-    b->lists = dmalloc<uint32_t>(lists.size(), &e);
-    HB_BTRY(e);
-    b->tiles = dmalloc<hb::HbBatchTile>(ht.size(), &e);
-    HB_BTRY(e);
-    if (!lists.empty()) HB_BTRY(hipMemcpy(b->lists, lists.data(), lists.size() * 4, hipMemcpyHostToDevice));
-    if (!ht.empty()) HB_BTRY(hipMemcpy(b->tiles, ht.data(), ht.size() * sizeof(hb::HbBatchTile), hipMemcpyHostToDevice));
-    b->ntiles = (uint32_t)ht.size();
+    HB_BTRY(b->lists.put(lists, b->stream));
+    HB_BTRY(b->tasks.put(tasks, b->stream));
+    b->ntasks = (uint32_t)tasks.size();
+    HB_BTRY(b->scratch.alloc((size_t)b->ntasks * hb::kScrRow));
+    HB_BTRY(b->chunk_flag.alloc(b->ntasks));
+    int dev_cus = 0;
+    HB_BTRY(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, b->device));
+    for (int q = 0; q < hb::kQueues; ++q) {
+        // largest first: the dynamic queue then balances the tail
+        std::stable_sort(items[q].begin(), items[q].end(), [](const Item &x, const Item &y) { return x.work > y.work; });
+        std::vector<hb::HbBatchTile> t(items[q].size());
+        for (size_t i = 0; i < t.size(); ++i) t[i] = items[q][i].t;
+        HB_BTRY(b->tiles[q].put(t, b->stream));
+        b->ntiles[q] = (uint32_t)t.size();
+        // persistent grid: as many blocks as fit at once (7 of 4 wavefronts per CU at
+        // 7 wavefronts per SIMD), no more than there are tiles
+        b->grid[q] = (unsigned)std::min<uint64_t>(std::max<uint32_t>(b->ntiles[q], 1), 7ull * (unsigned)std::max(dev_cus, 1));
+    }
+    for (int pi = 0; pi < hb::kBatchNP; ++pi) {
+        HB_BTRY(b->hubs[pi].put(hubs[pi], b->stream));
+        b->nhubs[pi] = (uint32_t)hubs[pi].size();
+    }
+    HB_BTRY(hipStreamSynchronize(b->stream));
     b->sched_ok = true;
     return HB_OK;
 }
@@ -364,7 +601,7 @@ int hb_batch_create(hb_batch **out, int device, const hb_graph_desc *graphs, int
     b->hg.resize(count);
     b->h_off.resize(count);
     b->max_iters.resize(count);
-    uint64_t nodes = 0, arcs = 0, regs = 0, words = 0, recs = 0;
+    uint64_t nodes = 0, arcs = 0, regs = 0, words = 0;
     for (int i = 0; i < count; ++i) {
         const hb_graph_desc &gd = graphs[i];
         if (gd.p < hb::kBatchPMin || gd.p > hb::kBatchPMax)
@@ -394,11 +631,11 @@ int hb_batch_create(hb_batch **out, int device, const hb_graph_desc *graphs, int
         g.reg_base = regs;
         g.node_base = nodes;
         g.bits_base = words;
-        g.rec_base = recs;
+        g.ring_base = (uint32_t)i * hb::kRing;
         g.n = (uint32_t)gd.n;
         g.p = (uint32_t)gd.p;
         g.words = (uint32_t)((gd.n + 31) / 32);
-        g.rec_cap = gd.max_iters > 0 ? (uint32_t)gd.max_iters + 2 : (uint32_t)std::min<uint64_t>(gd.n + 3, 1u << 16);
+        g.cap = (uint32_t)gd.n + 2;
         const uint32_t m = 1u << gd.p;
         g.am = alpha_f32(m) * ((float)m * (float)m);
         g.five_m = 5.0f * (float)m;
@@ -409,52 +646,47 @@ int hb_batch_create(hb_batch **out, int device, const hb_graph_desc *graphs, int
         arcs += gd.arcs;
         regs += gd.n << gd.p;
         words += g.words;
-        recs += g.rec_cap;
     }
     b->total_nodes = nodes;
     b->total_arcs = arcs;
     b->total_regs = regs;
     b->total_words = words;
-    b->total_rec = recs;
-    hipError_t e;
-    b->off = dmalloc<uint32_t>(nodes + count, &e); HB_BTRY(e);
-    b->nbr = dmalloc<uint32_t>(arcs, &e); HB_BTRY(e);
-    b->regs0 = dmalloc<uint8_t>(regs, &e); HB_BTRY(e);
-    b->regs1 = dmalloc<uint8_t>(regs, &e); HB_BTRY(e);
-    b->est = dmalloc<float>(nodes, &e); HB_BTRY(e);
-    b->est_tmp = dmalloc<float>(nodes, &e); HB_BTRY(e);
-    b->dist = dmalloc<double>(nodes, &e); HB_BTRY(e);
-    b->inv = dmalloc<double>(nodes, &e); HB_BTRY(e);
-    b->bits0 = dmalloc<uint32_t>(words, &e); HB_BTRY(e);
-    b->bits1 = dmalloc<uint32_t>(words, &e); HB_BTRY(e);
-    b->queue = dmalloc<uint32_t>(1, &e); HB_BTRY(e);
-    b->g = dmalloc<hb::HbBatchGraph>(count, &e); HB_BTRY(e);
-    b->rec = dmalloc<HbIterRec>(recs, &e); HB_BTRY(e);
-    b->lc_all = dmalloc<float>(hb::kLcTotal, &e); HB_BTRY(e);
-    b->bias_all = dmalloc<HbBiasTable>(hb::kBatchPMax - hb::kBatchPMin + 1, &e); HB_BTRY(e);
+    HB_BTRY(b->off.alloc(nodes + count));
+    HB_BTRY(b->nbr.alloc(arcs));
+    HB_BTRY(b->regs0.alloc(regs));
+    HB_BTRY(b->regs1.alloc(regs));
+    HB_BTRY(b->est.alloc(nodes));
+    HB_BTRY(b->est_tmp.alloc(nodes));
+    HB_BTRY(b->dist.alloc(nodes));
+    HB_BTRY(b->inv.alloc(nodes));
+    HB_BTRY(b->bits0.alloc(words));
+    HB_BTRY(b->bits1.alloc(words));
+    HB_BTRY(b->queue.alloc(hb::kQueues));
+    HB_BTRY(b->g.alloc(count));
+    HB_BTRY(b->rec.alloc((size_t)count * hb::kRing));
+    // estimator tables of p = 4..8: LC values (f64 then f32, as m ln(m/z)) and bias windows
     std::vector<float> lc(hb::kLcTotal, 0.0f);
     for (int p = hb::kBatchPMin; p <= hb::kBatchPMax; ++p) {
         const uint32_t m = 1u << p;
-        std::vector<float> one(m + 1, 0.0f);
-        for (uint32_t z = 1; z <= m; ++z) one[z] = (float)((double)m * std::log((double)m / (double)z));
-        std::copy(one.begin(), one.end(), lc.begin() + hb::kLcOff[p]);
-        b->lc_p[p] = dmalloc<float>(m + 1, &e);
-        HB_BTRY(e);
-        HB_BTRY(hipMemcpy(b->lc_p[p], one.data(), (m + 1) * 4, hipMemcpyHostToDevice));
-        HB_BTRY(hipMemcpy(b->bias_all + (p - hb::kBatchPMin), hb::bias_table(p), sizeof(HbBiasTable),
-                          hipMemcpyHostToDevice));
+        for (uint32_t z = 1; z <= m; ++z) lc[hb::lc_off(p) + z] = (float)((double)m * std::log((double)m / (double)z));
     }
-    HB_BTRY(hipMemcpy(b->lc_all, lc.data(), lc.size() * 4, hipMemcpyHostToDevice));
+    HB_BTRY(b->lc_all.alloc(hb::kLcTotal));
+    HB_BTRY(hipMemcpy(b->lc_all.ptr, lc.data(), lc.size() * 4, hipMemcpyHostToDevice));
+    for (int p = hb::kBatchPMin; p <= hb::kBatchPMax; ++p) b->lc_p[p] = b->lc_all.ptr + hb::lc_off(p);
+    HB_BTRY(b->bias_all.alloc(hb::kBatchNP));
+    for (int p = hb::kBatchPMin; p <= hb::kBatchPMax; ++p)
+        HB_BTRY(hipMemcpy(b->bias_all.ptr + (p - hb::kBatchPMin), hb::bias_table(p), sizeof(HbBiasTable),
+                          hipMemcpyHostToDevice));
     // CSR upload (per graph, converted to u32)
     std::vector<uint32_t> tmp;
     for (int i = 0; i < count; ++i) {
         const hb_graph_desc &gd = graphs[i];
         const hb::HbBatchGraph &g = b->hg[i];
-        HB_BTRY(hipMemcpy(b->off + g.off_base, b->h_off[i].data(), (gd.n + 1) * 4, hipMemcpyHostToDevice));
+        HB_BTRY(hipMemcpy(b->off.ptr + g.off_base, b->h_off[i].data(), (gd.n + 1) * 4, hipMemcpyHostToDevice));
         tmp.resize(gd.arcs);
         for (uint64_t k = 0; k < gd.arcs; ++k)
             tmp[k] = (uint32_t)(gd.nbr_bytes == 8 ? ((const uint64_t *)gd.nbrs)[k] : ((const uint32_t *)gd.nbrs)[k]);
-        if (gd.arcs) HB_BTRY(hipMemcpy(b->nbr + g.nbr_base, tmp.data(), gd.arcs * 4, hipMemcpyHostToDevice));
+        if (gd.arcs) HB_BTRY(hipMemcpy(b->nbr.ptr + g.nbr_base, tmp.data(), gd.arcs * 4, hipMemcpyHostToDevice));
     }
     *out = b.release();
     hb::set_error("");
@@ -473,25 +705,15 @@ int hb_batch_set_schedule(hb_batch *b, uint32_t small_max_deg, uint32_t mid_max_
 {
     if (!b) return bfail(HB_EINVAL, "hb_batch_set_schedule: NULL batch");
     if (small_max_deg) b->small_max = small_max_deg;
-    if (mid_max_deg) b->mid_max = std::max(mid_max_deg, b->small_max);
+    if (mid_max_deg) b->mid_max = mid_max_deg;
     if (tile_arcs) b->tile_arcs = tile_arcs;
     b->sched_ok = false;
     return HB_OK;
 }
 
-// The batched engine is switched off: its first GPU run (64 graphs of 4K-262K nodes,
-// p = 4..8) stalled without finishing and the cause has not been found by reading the
-// code, so it stays off until one is shown.  hb_batch_create and the readers still
-// validate their arguments; hb_batch_run (and the autotuner that calls it) return
-// HB_ESTATE.
-static constexpr bool kBatchEngineEnabled = false;
-
 int hb_batch_run(hb_batch *b, uint32_t flags, hb_batch_stats *stats)
 {
     if (!b) return bfail(HB_EINVAL, "hb_batch_run: NULL batch");
-    if (!kBatchEngineEnabled)
-        return bfail(HB_ESTATE, "hb_batch_run: the batched engine is switched off in this build "
-                                "(an unexplained stall of its first GPU run is under investigation)");
     HB_BTRY(hipSetDevice(b->device));
     {
         const int rc = build_tiles(b);
@@ -500,6 +722,7 @@ int hb_batch_run(hb_batch *b, uint32_t flags, hb_batch_stats *stats)
     b->ran = false;
     const int ng = (int)b->hg.size();
     std::vector<hb::HbBatchGraph> init = b->hg;
+    uint32_t cap_max = 0;
     for (int i = 0; i < ng; ++i) {
         HbIterState &st = init[i].st;
         st = HbIterState{};
@@ -508,27 +731,32 @@ int hb_batch_run(hb_batch *b, uint32_t flags, hb_batch_stats *stats)
         st.t = 1.0;
         st.inv_t = 1.0;
         init[i].overflow = 0;
+        cap_max = std::max(cap_max, st.max_iters ? std::min(st.max_iters, init[i].cap) : init[i].cap);
     }
     hb::HbBatchDev d{};
-    d.off = b->off;
-    d.nbr = b->nbr;
-    d.regs0 = b->regs0;
-    d.regs1 = b->regs1;
-    d.est = b->est;
-    d.dist = b->dist;
-    d.inv = b->inv;
-    d.bits0 = b->bits0;
-    d.bits1 = b->bits1;
-    d.g = b->g;
+    d.off = b->off.ptr;
+    d.nbr = b->nbr.ptr;
+    d.regs0 = b->regs0.ptr;
+    d.regs1 = b->regs1.ptr;
+    d.est = b->est.ptr;
+    d.dist = b->dist.ptr;
+    d.inv = b->inv.ptr;
+    d.bits0 = b->bits0.ptr;
+    d.bits1 = b->bits1.ptr;
+    d.g = b->g.ptr;
     d.ngraphs = (uint32_t)ng;
-    d.tiles = b->tiles;
-    d.ntiles = b->ntiles;
-    d.lists = b->lists;
-    d.rec = b->rec;
-    d.queue = b->queue;
-    d.lc_all = b->lc_all;
-    d.bias_all = b->bias_all;
+    d.rec = b->rec.ptr;
+    d.queue = b->queue.ptr;
+    d.lists = b->lists.ptr;
+    d.tasks = b->tasks.ptr;
+    d.scratch = b->scratch.ptr;
+    d.chunk_flag = b->chunk_flag.ptr;
+    d.lc = b->lc_all.ptr;
+    d.bias = b->bias_all.ptr;
     d.frontier = (flags & HB_FLAG_DENSE) ? 0 : 1;
+    hb::HbBatchList lists[hb::kQueues], hub_lists[hb::kBatchNP];
+    for (int q = 0; q < hb::kQueues; ++q) lists[q] = {b->tiles[q].ptr, nullptr, b->ntiles[q], (uint32_t)q};
+    for (int pi = 0; pi < hb::kBatchNP; ++pi) hub_lists[pi] = {nullptr, b->hubs[pi].ptr, b->nhubs[pi], 0};
 
     hipEvent_t e0, e1, e2;
     HB_BTRY(hipEventCreate(&e0));
@@ -538,18 +766,36 @@ int hb_batch_run(hb_batch *b, uint32_t flags, hb_batch_stats *stats)
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     uint32_t launches = 0;
-    const unsigned grid = (unsigned)std::min<uint32_t>(std::max<uint32_t>(b->ntiles, 1), 4096);
+    const bool trace = trace_on();
     auto launch_iter = [&]() -> hipError_t {
         hb::k_batch_begin<<<(unsigned)std::min(ng, 1024), hb::kBatchBlock, 0, b->stream>>>(d);
-        hb::k_batch_union<<<grid, hb::kBatchBlock, 0, b->stream>>>(d);
+        // per precision: the three node bins (team widths G, 4G capped at 64, and 64),
+        // the hub chunks, then the hubs
+#define HB_BATCH_LAUNCH(q, kernel)                                                                     \
+        if (lists[q].count) kernel<<<b->grid[q], hb::kBatchBlock, 0, b->stream>>>(d, lists[q]);
+#define HB_BATCH_P(P)                                                                                  \
+        {                                                                                              \
+            constexpr int G_ = (1 << P) / 16, q0 = (P - hb::kBatchPMin) * hb::kClasses;               \
+            HB_BATCH_LAUNCH(q0, (hb::k_batch_nodes<P, G_>))                                            \
+            HB_BATCH_LAUNCH(q0 + 1, (hb::k_batch_nodes<P, (4 * G_ < 64 ? 4 * G_ : 64)>))              \
+            HB_BATCH_LAUNCH(q0 + 2, (hb::k_batch_nodes<P, 64>))                                        \
+            HB_BATCH_LAUNCH(q0 + hb::kClassChunk, hb::k_batch_chunk<P>)                                \
+            if (hub_lists[P - hb::kBatchPMin].count)                                                   \
+                hb::k_batch_merge<P><<<(hub_lists[P - hb::kBatchPMin].count + 3) / 4, hb::kBatchBlock, 0, b->stream>>>( \
+                    d, hub_lists[P - hb::kBatchPMin]);                                                 \
+        }
+        HB_BATCH_P(4) HB_BATCH_P(5) HB_BATCH_P(6) HB_BATCH_P(7) HB_BATCH_P(8)
+#undef HB_BATCH_P
+#undef HB_BATCH_LAUNCH
         hb::k_batch_finalize<<<(ng + 63) / 64, 64, 0, b->stream>>>(d);
         return hipGetLastError();
     };
+    b->history.assign(ng, {});
+    const auto t_start = std::chrono::steady_clock::now();
     do {
-        if (hipMemcpyAsync(b->g, init.data(), ng * sizeof(hb::HbBatchGraph), hipMemcpyHostToDevice, b->stream) != hipSuccess ||
-            hipMemsetAsync(b->rec, 0, b->total_rec * sizeof(HbIterRec), b->stream) != hipSuccess ||
-            hb::launch_fill(b->bits0, b->total_words, 0u, b->stream) != hipSuccess ||
-            hb::launch_fill(b->bits1, b->total_words, 0u, b->stream) != hipSuccess ||
+        if (hipMemcpyAsync(b->g.ptr, init.data(), ng * sizeof(hb::HbBatchGraph), hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+            hb::launch_fill(b->bits0.ptr, b->total_words, 0u, b->stream) != hipSuccess ||
+            hb::launch_fill(b->bits1.ptr, b->total_words, 0u, b->stream) != hipSuccess ||
             hipEventRecord(e0, b->stream) != hipSuccess) {
             rc = bfail(HB_EHIP, "hb_batch_run: set-up failed");
             break;
@@ -571,21 +817,47 @@ int hb_batch_run(hb_batch *b, uint32_t flags, hb_batch_stats *stats)
             }
         }
         std::vector<hb::HbBatchGraph> hs(ng);
+        std::vector<hb::HbBatchRec> ring((size_t)ng * hb::kRing);
+        std::vector<uint32_t> harvested(ng, 0);  // iterations copied into history
         for (;;) {
-            const uint32_t k = launches < 8 ? 4u : 16u;
+            // no graph can need more than cap_max iterations; launches past that only
+            // happen if a graph failed to stop, which finalize reports as overflow
+            const uint32_t k = launches < 8 ? 2u : hb::kCheck;
+            if (launches >= cap_max + 2 * hb::kCheck) {
+                rc = bfail(HB_ESTATE, "hb_batch_run: %u launches without every graph stopping", launches);
+                break;
+            }
             for (uint32_t i = 0; i < k && rc == HB_OK; ++i) {
                 const hipError_t el = gexec ? hipGraphLaunch(gexec, b->stream) : launch_iter();
                 if (el != hipSuccess) rc = bfail(HB_EHIP, "iteration launch: %s", hipGetErrorString(el));
             }
             launches += k;
             if (rc != HB_OK) break;
-            if (hipMemcpyAsync(hs.data(), b->g, ng * sizeof(hb::HbBatchGraph), hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
+            if (hipMemcpyAsync(hs.data(), b->g.ptr, ng * sizeof(hb::HbBatchGraph), hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
+                hipMemcpyAsync(ring.data(), b->rec.ptr, ring.size() * sizeof(hb::HbBatchRec), hipMemcpyDeviceToHost,
+                               b->stream) != hipSuccess ||
                 hipStreamSynchronize(b->stream) != hipSuccess) {
                 rc = bfail(HB_EHIP, "iteration failed: %s", hipGetErrorString(hipGetLastError()));
                 break;
             }
             bool all = true;
-            for (const auto &g : hs) all = all && g.st.done;
+            uint32_t running = 0, it_max = 0;
+            for (int i = 0; i < ng; ++i) {
+                const hb::HbBatchGraph &g = hs[i];
+                const uint32_t complete = g.st.done ? g.st.executed : g.st.iter - 1;  // iterations finished
+                for (uint32_t t = harvested[i] + 1; t <= complete; ++t)
+                    b->history[i].push_back(ring[(size_t)i * hb::kRing + t % hb::kRing]);
+                harvested[i] = complete;
+                all = all && g.st.done;
+                running += !g.st.done;
+                it_max = std::max(it_max, complete);
+            }
+            if (trace) {
+                const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+                fprintf(stderr, "[hb_batch] %.3fs launches %u: %u of %d graphs running, longest at %u iterations\n", s,
+                        launches, running, ng, it_max);
+                fflush(stderr);
+            }
             if (all) break;
         }
         if (rc != HB_OK) break;
@@ -593,7 +865,8 @@ int hb_batch_run(hb_batch *b, uint32_t flags, hb_batch_stats *stats)
         (void)hipEventSynchronize(e2);
         b->h_final = hs;
         for (int i = 0; i < ng; ++i)
-            if (hs[i].overflow) rc = bfail(HB_ERANGE, "hb_batch_run: graph %d needs more than %u iterations", i, hs[i].rec_cap - 2);
+            if (hs[i].overflow)
+                rc = bfail(HB_ESTATE, "hb_batch_run: graph %d still changing after %u iterations (n + 2)", i, hs[i].cap);
     } while (0);
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -605,26 +878,26 @@ int hb_batch_run(hb_batch *b, uint32_t flags, hb_batch_stats *stats)
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipEventDestroy(e2);
-    if (rc != HB_OK) return rc;
-    b->h_rec.resize(b->total_rec);
-    HB_BTRY(hipMemcpy(b->h_rec.data(), b->rec, b->total_rec * sizeof(HbIterRec), hipMemcpyDeviceToHost));
+    if (rc != HB_OK) {
+        (void)hipStreamSynchronize(b->stream);
+        return rc;
+    }
     b->ran = true;
     if (stats) {
         memset(stats, 0, sizeof *stats);
         stats->launches = (int32_t)launches;
         stats->ms_total = ms_loop;
         stats->ms_init = ms_init;
-        stats->tiles = b->ntiles;
+        for (int q = 0; q < hb::kQueues; ++q) stats->tiles += b->ntiles[q];
         for (int i = 0; i < ng; ++i) {
             const auto &g = b->h_final[i];
             stats->iterations_total += g.st.executed;
             stats->iterations_max = std::max<int32_t>(stats->iterations_max, (int32_t)g.st.executed);
             const uint64_t R = 1ull << g.p;
-            for (uint32_t t = 1; t <= g.st.executed; ++t) {
-                const HbIterRec &r = b->h_rec[g.rec_base + t];
-                stats->arcs_gathered += r.arcs[0];
-                stats->nodes_recomputed += r.nodes[0];
-                stats->alg_bytes += r.nodes[0] * (8 + 2 * R + 40) + r.arcs[0] * (4 + R);
+            for (const hb::HbBatchRec &r : b->history[i]) {
+                stats->arcs_gathered += r.arcs;
+                stats->nodes_recomputed += r.nodes;
+                stats->alg_bytes += r.nodes * (8 + 2 * R + 40) + r.arcs * 4 + r.gathered * R;
             }
         }
     }
@@ -640,16 +913,18 @@ int hb_batch_graph_stats(hb_batch *b, int graph, int32_t *iterations, int32_t *c
     const auto &g = b->h_final[graph];
     if (iterations) *iterations = (int32_t)g.st.executed;
     if (converged) *converged = (int32_t)g.st.converged;
-    const int k = std::min<int>(cap, (int)g.st.executed);
-    for (int t = 1; t <= k; ++t) increases[t - 1] = b->h_rec[g.rec_base + t].inc[0];
-    return k;
+    const int k = std::min<int>(cap, (int)b->history[graph].size());
+    if (increases)
+        for (int t = 0; t < k; ++t) increases[t] = b->history[graph][t].inc;
+    return increases ? k : 0;
 }
 
 static const uint8_t *batch_regs(hb_batch *b, int graph, int which)
 {
     const auto &g = b->h_final[graph];
-    if (which == HB_REGS_REFERENCE) return b->regs0 + g.reg_base;
-    return ((g.st.parity ^ 1u) ? b->regs1 : b->regs0) + g.reg_base;
+    if (which == HB_REGS_REFERENCE) return b->regs0.ptr + g.reg_base;
+    // the last iteration read regs[parity] and wrote the other buffer
+    return (g.st.parity ? b->regs0.ptr : b->regs1.ptr) + g.reg_base;
 }
 
 int hb_batch_read_registers(hb_batch *b, int graph, int which, uint8_t *dst)
@@ -670,8 +945,8 @@ int hb_batch_read_estimates(hb_batch *b, int graph, int which, float *dst)
     if (!b->ran) return bfail(HB_ESTATE, "hb_batch_read_estimates: no completed run");
     HB_BTRY(hipSetDevice(b->device));
     const auto &g = b->hg[graph];
-    HB_BTRY(hb::launch_estimates(graph_dev(b, graph), batch_regs(b, graph, which), b->est_tmp, b->stream));
-    HB_BTRY(hipMemcpyAsync(dst, b->est_tmp, (size_t)g.n * 4, hipMemcpyDeviceToHost, b->stream));
+    HB_BTRY(hb::launch_estimates(graph_dev(b, graph), batch_regs(b, graph, which), b->est_tmp.ptr, b->stream));
+    HB_BTRY(hipMemcpyAsync(dst, b->est_tmp.ptr, (size_t)g.n * 4, hipMemcpyDeviceToHost, b->stream));
     HB_BTRY(hipStreamSynchronize(b->stream));
     return HB_OK;
 }
@@ -682,8 +957,8 @@ int hb_batch_read_accumulators(hb_batch *b, int graph, double *dist, double *inv
     if (!b->ran) return bfail(HB_ESTATE, "hb_batch_read_accumulators: no completed run");
     HB_BTRY(hipSetDevice(b->device));
     const auto &g = b->hg[graph];
-    if (dist) HB_BTRY(hipMemcpy(dist, b->dist + g.node_base, (size_t)g.n * 8, hipMemcpyDeviceToHost));
-    if (inv) HB_BTRY(hipMemcpy(inv, b->inv + g.node_base, (size_t)g.n * 8, hipMemcpyDeviceToHost));
+    if (dist) HB_BTRY(hipMemcpy(dist, b->dist.ptr + g.node_base, (size_t)g.n * 8, hipMemcpyDeviceToHost));
+    if (inv) HB_BTRY(hipMemcpy(inv, b->inv.ptr + g.node_base, (size_t)g.n * 8, hipMemcpyDeviceToHost));
     return HB_OK;
 }
 
@@ -696,18 +971,21 @@ extern "C" int hb_batch_autotune(hb_batch *b, uint32_t *small_max_deg, uint32_t 
     if (!b) return bfail(HB_EINVAL, "hb_batch_autotune: NULL batch");
     hb_batch_stats st;
     auto trial = [&](uint32_t s, uint32_t m, uint32_t t, double *best) -> int {
-        int rc = hb_batch_set_schedule(b, s, m, t);
-        if (rc != HB_OK) return rc;
+        b->small_max = s;
+        b->mid_max = m;
+        b->tile_arcs = t;
+        b->sched_ok = false;
         *best = 1e300;
         for (int r = 0; r < 2; ++r) {
-            rc = hb_batch_run(b, HB_FLAG_GRAPH, &st);
+            const int rc = hb_batch_run(b, HB_FLAG_GRAPH, &st);
             if (rc != HB_OK) return rc;
             *best = std::min(*best, st.ms_total);
         }
         return HB_OK;
     };
+    // 0 = the per-precision defaults of the single-graph engine
     uint32_t knob[3] = {b->small_max, b->mid_max, b->tile_arcs};
-    const uint32_t cand[3][4] = {{2, 4, 8, 16}, {16, 32, 64, 256}, {2048, 8192, 16384, 65536}};
+    const uint32_t cand[3][4] = {{0, 16, 64, 256}, {0, 256, 1024, 4096}, {4096, 8192, 16384, 65536}};
     double best = 0.0;
     int rc = trial(knob[0], knob[1], knob[2], &best);
     if (rc != HB_OK) return rc;
@@ -716,7 +994,6 @@ extern "C" int hb_batch_autotune(hb_batch *b, uint32_t *small_max_deg, uint32_t 
             if (v == knob[k]) continue;
             uint32_t t[3] = {knob[0], knob[1], knob[2]};
             t[k] = v;
-            if (t[1] < t[0]) continue;
             double ms = 0.0;
             rc = trial(t[0], t[1], t[2], &ms);
             if (rc != HB_OK) return rc;
@@ -726,8 +1003,10 @@ extern "C" int hb_batch_autotune(hb_batch *b, uint32_t *small_max_deg, uint32_t 
             }
         }
     }
-    rc = hb_batch_set_schedule(b, knob[0], knob[1], knob[2]);
-    if (rc != HB_OK) return rc;
+    b->small_max = knob[0];
+    b->mid_max = knob[1];
+    b->tile_arcs = knob[2];
+    b->sched_ok = false;
     if (small_max_deg) *small_max_deg = knob[0];
     if (mid_max_deg) *mid_max_deg = knob[1];
     if (tile_arcs) *tile_arcs = knob[2];

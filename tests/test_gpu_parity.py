@@ -325,8 +325,10 @@ def test_cpp_mirror_matches_python_mirror(hb, oracle, tmp_path):
         assert (float(d), float(cc), float(ll)) == (ref.dist[u], c[u], lin[u])
 
 
-BATCH_OFF = pytest.mark.skip(reason="batched engine switched off: its first GPU run (64 graphs) stalled "
-                                     "and the cause is not yet found; hb_batch_run returns HB_ESTATE")
+@pytest.fixture
+def batch_trace(monkeypatch):
+    """Progress lines from the batched engine's host loop on stderr (SPARKX_HB_TRACE)."""
+    monkeypatch.setenv("SPARKX_HB_TRACE", "1")
 
 
 def _compare_batch_graph(bt, i, h, ctx):
@@ -342,29 +344,32 @@ def _compare_batch_graph(bt, i, h, ctx):
     assert np.array_equal(bt.estimates(i), h.estimates()), ctx
 
 
-@BATCH_OFF
-def test_batch_64_graphs_match_single_graph_runs(hb):
-    """Batched configuration: 64 seeded graphs (32 ER + 32 R-MAT, n in [2^12, 2^18],
-    mixed p = 4..8) in one work-queue launch per iteration, hipGraph-replayed; every
-    graph bit-identical to its own single-graph run."""
-    from sparkx_amd.batch import HyperBallBatch, batched_workload
-    items = batched_workload(64, seed=1)
-    say("batch64", sum(g.size() for g, _ in items), sum(g.width() for g, _ in items))
-    with HyperBallBatch(items) as bt:
-        st = bt.run(graph_capture=True)
-        assert st.tiles > 0 and st.iterations_max > 0
-        for i, (g, p) in enumerate(items):
-            with hb.HyperBall(g, p) as h:
-                _compare_batch_graph(bt, i, h, f"graph {i} p={p}")
+def _hub_graph(hb, n, hub_deg, extra, seed):
+    """Symmetric CSR: node 0 joined to nodes 1..hub_deg, plus `extra` random edges
+    among the other nodes (duplicates and loops dropped): a hub whose list is cut
+    into chunks."""
+    rng = np.random.default_rng(seed)
+    src = [np.zeros(hub_deg, np.int64), rng.integers(1, n, extra)]
+    dst = [np.arange(1, hub_deg + 1), rng.integers(1, n, extra)]
+    a, b = np.concatenate(src), np.concatenate(dst)
+    keep = a != b
+    a, b = a[keep], b[keep]
+    pairs = np.unique(np.stack([np.concatenate([a, b]), np.concatenate([b, a])], 1), axis=0)
+    off = np.zeros(n + 1, np.uint32)
+    np.add.at(off, pairs[:, 0] + 1, 1)
+    return hb.Graph(np.cumsum(off).astype(np.uint32), pairs[:, 1].astype(np.uint32), f"hub{n}_{hub_deg}")
 
 
-@BATCH_OFF
-def test_batch_small_graphs_against_oracle_and_schedules(hb, oracle):
+def test_batch_small_graphs_against_oracle_and_schedules(hb, oracle, batch_trace):
     from sparkx_amd.batch import HyperBallBatch
     items = []
     for i, p in enumerate((4, 5, 6, 7, 8, 6)):
         g = hb.Graph.erdos_renyi(700 + 100 * i, 2000 + 500 * i, seed=50 + i) if i % 2 else hb.Graph.rmat(10, 8, seed=60 + i)
         items.append((g, p))
+    # hubs above the wavefront bound (2048 arcs at p >= 6, 16384 at p <= 5): chunked lists
+    items.append((_hub_graph(hb, 9000, 6000, 20000, 71), 6))
+    items.append((_hub_graph(hb, 9000, 5000, 30000, 72), 8))
+    items.append((_hub_graph(hb, 20000, 17000, 10000, 73), 4))
     refs = [_oracle_run(oracle, g, p) for g, p in items]
     variants = [dict(graph_capture=True), dict(graph_capture=False), dict(dense=True),
                 dict(schedule=(2, 16, 512)), dict(schedule=(16, 256, 65536))]
@@ -390,8 +395,7 @@ def test_batch_small_graphs_against_oracle_and_schedules(hb, oracle):
             assert np.array_equal(bt.registers(i, _lib.HB_REGS_REFERENCE).ravel(), ref.regs_ref), i
 
 
-@BATCH_OFF
-def test_batch_truncation_and_errors(hb, oracle):
+def test_batch_truncation_and_errors(hb, oracle, batch_trace):
     from sparkx_amd import HyperBallError, _lib
     from sparkx_amd.batch import HyperBallBatch
     g = hb.Graph.erdos_renyi(2000, 6000, seed=77)
@@ -404,6 +408,21 @@ def test_batch_truncation_and_errors(hb, oracle):
     with pytest.raises(HyperBallError) as ei:
         HyperBallBatch([(g, 9)])
     assert ei.value.code == _lib.HB_ERANGE
+
+
+def test_batch_64_graphs_match_single_graph_runs(hb, batch_trace):
+    """Batched configuration: 64 seeded graphs (32 ER + 32 R-MAT, n in [2^12, 2^18],
+    mixed p = 4..8) in one work-queue launch per iteration, hipGraph-replayed; every
+    graph bit-identical to its own single-graph run."""
+    from sparkx_amd.batch import HyperBallBatch, batched_workload
+    items = batched_workload(64, seed=1)
+    say("batch64", sum(g.size() for g, _ in items), sum(g.width() for g, _ in items))
+    with HyperBallBatch(items) as bt:
+        st = bt.run(graph_capture=True)
+        assert st.tiles > 0 and st.iterations_max > 0
+        for i, (g, p) in enumerate(items):
+            with hb.HyperBall(g, p) as h:
+                _compare_batch_graph(bt, i, h, f"graph {i} p={p}")
 
 
 def test_autotune_keeps_results(hb, oracle):
