@@ -3,11 +3,14 @@
 //     k_batch_begin -> k_batch_union<4..8> -> k_batch_merge<p> (precisions with hubs)
 //     -> k_batch_finalize
 // captured once in a hipGraph and replayed until every graph has stopped.
-// k_batch_union<P> is a persistent kernel: its wavefronts pop tiles from a work
-// queue over the (graph, degree bin) tiles of every graph of precision P, largest
-// first; a tile is a run of nodes of one bin (team width G, 4G or 64 lanes per node,
-// G = 2^P / 16) or one chunk of a hub's neighbour list, whose partial union goes to
-// scratch and is merged into the hub by k_batch_merge<P>.  Every graph keeps its
+// The work of one precision P comes in four classes: node tiles of the three degree
+// bins (team width G, 4G or 64 lanes per node, G = 2^P / 16) and chunks of hub
+// neighbour lists, whose partial unions go to scratch and are merged into their hubs
+// by k_batch_merge<P>.  k_batch_compact lists, at the start of every iteration, the
+// tiles of each class whose graph is still running (largest first); k_batch_nodes<P,T>
+// and k_batch_chunk<P> then walk their class's list, wavefront w taking tiles w,
+// w + W, ... (W wavefronts in the grid), and return at once when it is empty, so
+// graphs that have stopped cost nothing.  Every graph keeps its
 // own iteration state (t, 1/t, parity, frontier mode, stop flag) and applies the
 // reference's stopping rule on its own (hyper_ball.rs:833-849); a graph that has
 // stopped costs nothing.  The per-node work is union_node, the same function the
@@ -65,7 +68,7 @@ struct HbBatchGraph {
     HbIterState st;
 };
 
-// a work-queue item of one precision: `team` lanes per node over lists[first ..
+// a tile of one precision: `team` lanes per node over lists[first ..
 // first + count) (graph-local ids), or, with team == 0, hub chunk task `first`
 struct HbBatchTile {
     uint32_t graph, first, count, team;
@@ -78,7 +81,9 @@ struct HbBatchHub {   // node's chunks are tasks task0 .. task0 + ntask - 1
 };
 
 struct HbBatchDev {
-    const uint32_t *off, *nbr;
+    const uint32_t *off, *nbr;  // every graph's CSR in its degree order (local ids)
+    const uint32_t *orig;       // caller's id of each arc's head (hashed in iteration 1)
+    const uint32_t *node_id;    // caller's id of each node (hashed at init)
     uint8_t *regs0, *regs1;
     float *est;
     double *dist, *inv;
@@ -86,7 +91,8 @@ struct HbBatchDev {
     HbBatchGraph *g;
     uint32_t ngraphs;
     HbBatchRec *rec;
-    uint32_t *queue;            // [kQueues] work-queue heads, one per (precision, class)
+    const uint32_t *active;     // per class: tiles of running graphs, at class_base[q]
+    const uint32_t *active_count;  // [kQueues]
     const uint32_t *lists;
     const HbBatchTask *tasks;
     uint8_t *scratch;           // kScrRow bytes per task
@@ -96,11 +102,16 @@ struct HbBatchDev {
     int frontier;
 };
 
-struct HbBatchList {  // the tiles (or hubs) of one (precision, class) and its queue
+struct HbBatchList {  // the tiles (or hubs) of one (precision, class)
     const HbBatchTile *tiles;
     const HbBatchHub *hubs;
     uint32_t count;
-    uint32_t queue;
+    uint32_t queue;      // class index
+    uint32_t base;       // first slot of the class in the active list
+};
+
+struct HbBatchClasses {  // every class, for the compaction
+    HbBatchList q[kQueues];
 };
 
 // x as a wave-uniform value (every lane holds lane 0's copy, in scalar registers):
@@ -134,8 +145,8 @@ __device__ __forceinline__ HbView batch_view(const HbBatchDev &b, const HbBatchG
     HbView v;
     v.off = b.off + g.off_base;
     v.nbr = b.nbr + g.nbr_base;
-    v.orig = v.nbr;          // ids are hashed as given (no renumbering in a batch)
-    v.node_id = nullptr;
+    v.orig = b.orig + g.nbr_base;        // nodes are renumbered by degree per graph; the
+    v.node_id = b.node_id + g.node_base; // caller's ids are what every sketch inserts
     v.in = (g.st.parity ? b.regs1 : b.regs0) + g.reg_base;
     v.out = (g.st.parity ? b.regs0 : b.regs1) + g.reg_base;
     v.est = b.est + g.node_base;
@@ -172,6 +183,7 @@ __device__ __forceinline__ void batch_chunk(const HbBatchDev &b, const HbView &v
     constexpr uint32_t G = (1u << P) / 16, T = HB_WAVE, S = T / G;
     const uint32_t tl = threadIdx.x & (HB_WAVE - 1), cl = tl % G, slot = tl / G;
     const uint32_t *__restrict__ nb = v.nbr + task.beg;
+    const uint32_t *__restrict__ nbh = v.orig + task.beg;  // caller's ids, hashed in iteration 1
     const uint32_t len = task.end - task.beg;
     bool active = true;
     if (v.use_frontier) {
@@ -183,9 +195,9 @@ __device__ __forceinline__ void batch_chunk(const HbBatchDev &b, const HbView &v
     if (!active) return;
     uint32_t gathered = 0;
     uint4 acc = v.use_frontier
-                    ? hb_gather_union<P, S, true>(v.in, v.first, v.scr, v.bitsR, nb, nb, len, slot, cl,
+                    ? hb_gather_union<P, S, true>(v.in, v.first, v.scr, v.bitsR, nb, nbh, len, slot, cl,
                                                   make_uint4(0, 0, 0, 0), gathered)
-                    : hb_gather_union<P, S, false>(v.in, v.first, v.scr, nullptr, nb, nb, len, slot, cl,
+                    : hb_gather_union<P, S, false>(v.in, v.first, v.scr, nullptr, nb, nbh, len, slot, cl,
                                                    make_uint4(0, 0, 0, 0), gathered);
     if (cl == 0) c.gathered += gathered;
 #pragma unroll
@@ -225,29 +237,24 @@ __device__ __forceinline__ void add_counts(HbBatchRec *r, const HbCounts &c)
     if (c.gathered) atomicAdd(&r->gathered, c.gathered);
 }
 
-// Next tile of a work queue for the whole wavefront (wave-uniform), or `count`.
-__device__ __forceinline__ uint32_t next_tile(const HbBatchDev &b, const HbBatchList &l)
-{
-    uint32_t ti = 0;
-    if ((threadIdx.x & (HB_WAVE - 1)) == 0) ti = atomicAdd(b.queue + l.queue, 1u);
-    ti = __builtin_amdgcn_readfirstlane(__shfl(ti, 0));
-    return ti < l.count ? ti : l.count;
-}
+// Global wavefront index and count of the grid (wave-uniform).
+__device__ __forceinline__ uint32_t wave_id() { return __builtin_amdgcn_readfirstlane(blockIdx.x * (kBatchBlock / HB_WAVE) + threadIdx.x / HB_WAVE); }
+__device__ __forceinline__ uint32_t wave_count() { return gridDim.x * (kBatchBlock / HB_WAVE); }
 
-// Persistent union over the node tiles of one (precision P, team width T) class.
-// Every loop below has a trip count fixed by its tile (at most 4096 nodes), and the
-// queue head only grows, so every wavefront leaves once the head passes the count.
+// Union over the active node tiles of one (precision P, team width T) class.  Every
+// loop has a trip count fixed by the active list and its tiles (at most 4096 nodes).
 template <int P, int T>
 __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(P >= 7 ? 7 : 8))) void k_batch_nodes(
     HbBatchDev b, HbBatchList l)
 {
+    const uint32_t count = b.active_count[l.queue];
+    if (count == 0) return;  // the whole grid leaves: no graph of this class is running
     HB_BATCH_TABLES(P, b, tbl)
     __shared__ uint32_t s_first[kBatchBlock * 16];
     const uint32_t lane = threadIdx.x & (HB_WAVE - 1);
-    for (uint32_t ti = next_tile(b, l); ti < l.count; ti = next_tile(b, l)) {
-        const HbBatchTile tile = wave_uniform(l.tiles[ti]);
+    for (uint32_t w = wave_id(); w < count; w += wave_count()) {
+        const HbBatchTile tile = wave_uniform(l.tiles[b.active[l.base + w]]);
         const HbBatchGraph g = wave_uniform(b.g[tile.graph]);
-        if (g.st.done) continue;
         const HbTables tb = {tbl_lc, &tbl_bias, g.am, g.five_m, g.lc_thr};
         const HbView v = batch_view(b, g, s_first);
         HbCounts c;
@@ -257,23 +264,55 @@ __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(P >
     }
 }
 
-// Persistent partial unions of the hub chunks of precision P, one wavefront per chunk.
+// Partial unions of the active hub chunks of precision P, one wavefront per chunk.
 template <int P>
 __global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(P >= 7 ? 7 : 8))) void k_batch_chunk(
     HbBatchDev b, HbBatchList l)
 {
+    const uint32_t count = b.active_count[l.queue];
+    if (count == 0) return;
     __shared__ uint32_t s_first[kBatchBlock * 16];
     const uint32_t lane = threadIdx.x & (HB_WAVE - 1);
-    for (uint32_t ti = next_tile(b, l); ti < l.count; ti = next_tile(b, l)) {
-        const HbBatchTile tile = wave_uniform(l.tiles[ti]);
+    for (uint32_t w = wave_id(); w < count; w += wave_count()) {
+        const HbBatchTile tile = wave_uniform(l.tiles[b.active[l.base + w]]);
         const HbBatchGraph g = wave_uniform(b.g[tile.graph]);
-        if (g.st.done) continue;
         const HbView v = batch_view(b, g, s_first);
         HbCounts c;
         batch_chunk<P>(b, v, tile.first, wave_uniform(b.tasks[tile.first]), c);
         wave_sum_counts(c);
         if (lane == 0) add_counts(ring_rec(b, g, g.st.iter), c);
     }
+}
+
+// Start of an iteration, one block per class: the tiles of graphs still running, in
+// list order (largest first), into the class's active list.
+__global__ __launch_bounds__(1024) void k_batch_compact(HbBatchDev b, HbBatchClasses cl)
+{
+    __shared__ uint32_t s_wave[1024 / HB_WAVE];
+    __shared__ uint32_t s_total;
+    const HbBatchList l = cl.q[blockIdx.x];
+    uint32_t *out = (uint32_t *)b.active + l.base;
+    const uint32_t lane = threadIdx.x & (HB_WAVE - 1), wv = threadIdx.x / HB_WAVE;
+    if (threadIdx.x == 0) s_total = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < l.count; base += 1024) {  // uniform trip count
+        const uint32_t i = base + threadIdx.x;
+        const bool keep = i < l.count && !b.g[l.tiles[i].graph].st.done;
+        const unsigned long long m = __ballot(keep);
+        if (lane == 0) s_wave[wv] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t before = s_total;
+        for (uint32_t k = 0; k < wv; ++k) before += s_wave[k];
+        if (keep) out[before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = i;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t t = 0;
+            for (uint32_t k = 0; k < 1024 / HB_WAVE; ++k) t += s_wave[k];
+            s_total += t;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) ((uint32_t *)b.active_count)[blockIdx.x] = s_total;
 }
 
 // hubs of precision P: one wavefront per hub merges its own registers with the
@@ -284,9 +323,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_merge(HbBatchDev b, HbBat
     constexpr uint32_t G = (1u << P) / 16, T = HB_WAVE, S = T / G;
     HB_BATCH_TABLES(P, b, tbl)
     const uint32_t tl = threadIdx.x & (HB_WAVE - 1), cl = tl % G, slot = tl / G;
-    const uint32_t waves = gridDim.x * (kBatchBlock / HB_WAVE);
-    const uint32_t w0 = __builtin_amdgcn_readfirstlane(blockIdx.x * (kBatchBlock / HB_WAVE) + threadIdx.x / HB_WAVE);
-    for (uint32_t h = w0; h < l.count; h += waves) {
+    for (uint32_t h = wave_id(); h < l.count; h += wave_count()) {
         const HbBatchHub hub = wave_uniform(l.hubs[h]);
         const HbBatchGraph g = wave_uniform(b.g[hub.graph]);
         if (g.st.done) continue;
@@ -302,7 +339,7 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_merge(HbBatchDev b, HbBat
                 *(uint4 *)(v.out + ((uint64_t)u << P) + cl * 16) = *(const uint4 *)(v.in + ((uint64_t)u << P) + cl * 16);
             continue;
         }
-        const uint4 own = v.first ? hb_own_from_word(hb_insertion<P>(u), cl)
+        const uint4 own = v.first ? hb_own_from_word(hb_insertion<P>(v.node_id[u]), cl)
                                   : *(const uint4 *)(v.in + ((uint64_t)u << P) + cl * 16);
         uint4 acc = own;
         for (uint32_t j = slot; j < hub.ntask; j += S)
@@ -328,11 +365,10 @@ __global__ __launch_bounds__(kBatchBlock) void k_batch_merge(HbBatchDev b, HbBat
     }
 }
 
-// start of an iteration: empty queues; for every running graph clear the bitmap
-// this iteration writes and its ring record
+// start of an iteration: for every running graph clear the bitmap this iteration
+// writes and its ring record
 __global__ __launch_bounds__(kBatchBlock) void k_batch_begin(HbBatchDev b)
 {
-    if (blockIdx.x == 0 && threadIdx.x < kQueues) b.queue[threadIdx.x] = 0u;
     for (uint32_t gi = blockIdx.x; gi < b.ngraphs; gi += gridDim.x) {
         const HbBatchGraph &g = b.g[gi];
         if (g.st.done) continue;
@@ -441,8 +477,11 @@ struct hb_batch {
     std::vector<std::vector<uint32_t>> h_off;
     std::vector<int> max_iters;
     uint64_t total_nodes = 0, total_arcs = 0, total_regs = 0, total_words = 0;
-    DevArr<uint32_t> off, nbr, bits0, bits1, queue, lists, chunk_flag;
-    DevArr<uint8_t> regs0, regs1, scratch;
+    // CSRs in each graph's degree order, the caller's ids (orig per arc, perm per node)
+    // and the inverse numbering (sigma: caller's node -> device node)
+    DevArr<uint32_t> off, nbr, orig, perm, sigma, bits0, bits1, lists, chunk_flag;
+    DevArr<uint32_t> active, active_count;  // per class: tiles of running graphs
+    DevArr<uint8_t> regs0, regs1, scratch, readback;
     DevArr<float> est, lc_all, est_tmp;
     DevArr<double> dist, inv;
     DevArr<HbBiasTable> bias_all;
@@ -452,9 +491,12 @@ struct hb_batch {
     DevArr<hb::HbBatchHub> hubs[hb::kBatchNP];
     DevArr<hb::HbBatchTask> tasks;
     float *lc_p[hb::kBatchPMax + 1] = {};  // device pointers into lc_all, per p
-    uint32_t ntiles[hb::kQueues] = {}, nhubs[hb::kBatchNP] = {}, ntasks = 0;
+    uint32_t ntiles[hb::kQueues] = {}, tile_base[hb::kQueues] = {}, nhubs[hb::kBatchNP] = {}, ntasks = 0;
     unsigned grid[hb::kQueues] = {};
-    uint32_t small_max = 0, mid_max = 0, tile_arcs = 16384;  // 0: per-precision defaults
+    // 0: per-precision defaults.  A tile should be a few rounds of one wavefront: a
+    // class with fewer tiles than resident wavefronts (7,168) leaves most of them idle
+    // while a few walk long tiles node by node.
+    uint32_t small_max = 0, mid_max = 0, tile_arcs = 1024;
     bool sched_ok = false, ran = false;
     std::vector<hb::HbBatchGraph> h_final;
     std::vector<std::vector<hb::HbBatchRec>> history;  // per graph, iteration t at [t - 1]
@@ -471,6 +513,8 @@ static HbDev graph_dev(hb_batch *b, int gi)
     HbDev d{};
     d.off = b->off.ptr + g.off_base;
     d.nbr = b->nbr.ptr + g.nbr_base;
+    d.orig = b->orig.ptr + g.nbr_base;
+    d.node_id = b->perm.ptr + g.node_base;
     d.regs0 = b->regs0.ptr + g.reg_base;
     d.regs1 = b->regs1.ptr + g.reg_base;
     d.est = b->est.ptr + g.node_base;
@@ -566,7 +610,8 @@ static int build_tiles(hb_batch *b)
     int dev_cus = 0;
     HB_BTRY(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, b->device));
     for (int q = 0; q < hb::kQueues; ++q) {
-        // largest first: the dynamic queue then balances the tail
+        // largest first: wavefront w takes active tiles w, w + W, ..., so the longest
+        // ones start first and the short ones fill in behind them
         std::stable_sort(items[q].begin(), items[q].end(), [](const Item &x, const Item &y) { return x.work > y.work; });
         std::vector<hb::HbBatchTile> t(items[q].size());
         for (size_t i = 0; i < t.size(); ++i) t[i] = items[q][i].t;
@@ -580,6 +625,12 @@ static int build_tiles(hb_batch *b)
         HB_BTRY(b->hubs[pi].put(hubs[pi], b->stream));
         b->nhubs[pi] = (uint32_t)hubs[pi].size();
     }
+    uint32_t all = 0;
+    for (int q = 0; q < hb::kQueues; ++q) {
+        b->tile_base[q] = all;
+        all += b->ntiles[q];
+    }
+    HB_BTRY(b->active.alloc(all));
     HB_BTRY(hipStreamSynchronize(b->stream));
     b->sched_ok = true;
     return HB_OK;
@@ -661,7 +712,7 @@ int hb_batch_create(hb_batch **out, int device, const hb_graph_desc *graphs, int
     HB_BTRY(b->inv.alloc(nodes));
     HB_BTRY(b->bits0.alloc(words));
     HB_BTRY(b->bits1.alloc(words));
-    HB_BTRY(b->queue.alloc(hb::kQueues));
+    HB_BTRY(b->active_count.alloc(hb::kQueues));
     HB_BTRY(b->g.alloc(count));
     HB_BTRY(b->rec.alloc((size_t)count * hb::kRing));
     // estimator tables of p = 4..8: LC values (f64 then f32, as m ln(m/z)) and bias windows
@@ -687,6 +738,27 @@ int hb_batch_create(hb_batch **out, int device, const hb_graph_desc *graphs, int
         for (uint64_t k = 0; k < gd.arcs; ++k)
             tmp[k] = (uint32_t)(gd.nbr_bytes == 8 ? ((const uint64_t *)gd.nbrs)[k] : ((const uint32_t *)gd.nbrs)[k]);
         if (gd.arcs) HB_BTRY(hipMemcpy(b->nbr.ptr + g.nbr_base, tmp.data(), gd.arcs * 4, hipMemcpyHostToDevice));
+    }
+    // every graph renumbered by descending degree on the device (as hb_run's default
+    // layout): its most gathered register rows sit together; the tiles are built from
+    // the renumbered offsets
+    {
+        DevArr<uint32_t> roff, rnbr;
+        HB_BTRY(roff.alloc(nodes + count));
+        HB_BTRY(rnbr.alloc(arcs));
+        HB_BTRY(b->orig.alloc(arcs));
+        HB_BTRY(b->perm.alloc(nodes));
+        HB_BTRY(b->sigma.alloc(nodes));
+        for (int i = 0; i < count; ++i) {
+            const hb::HbBatchGraph &g = b->hg[i];
+            HB_BTRY(hb::relabel_by_degree(b->off.ptr + g.off_base, b->nbr.ptr + g.nbr_base, g.n,
+                                          (uint32_t)graphs[i].arcs, b->perm.ptr + g.node_base,
+                                          b->sigma.ptr + g.node_base, roff.ptr + g.off_base, rnbr.ptr + g.nbr_base,
+                                          b->orig.ptr + g.nbr_base, b->stream));
+            HB_BTRY(hipMemcpy(b->h_off[i].data(), roff.ptr + g.off_base, ((size_t)g.n + 1) * 4, hipMemcpyDeviceToHost));
+        }
+        std::swap(b->off.ptr, roff.ptr);
+        std::swap(b->nbr.ptr, rnbr.ptr);
     }
     *out = b.release();
     hb::set_error("");
@@ -736,6 +808,8 @@ int hb_batch_run(hb_batch *b, uint32_t flags, hb_batch_stats *stats)
     hb::HbBatchDev d{};
     d.off = b->off.ptr;
     d.nbr = b->nbr.ptr;
+    d.orig = b->orig.ptr;
+    d.node_id = b->perm.ptr;
     d.regs0 = b->regs0.ptr;
     d.regs1 = b->regs1.ptr;
     d.est = b->est.ptr;
@@ -746,7 +820,8 @@ int hb_batch_run(hb_batch *b, uint32_t flags, hb_batch_stats *stats)
     d.g = b->g.ptr;
     d.ngraphs = (uint32_t)ng;
     d.rec = b->rec.ptr;
-    d.queue = b->queue.ptr;
+    d.active = b->active.ptr;
+    d.active_count = b->active_count.ptr;
     d.lists = b->lists.ptr;
     d.tasks = b->tasks.ptr;
     d.scratch = b->scratch.ptr;
@@ -755,8 +830,10 @@ int hb_batch_run(hb_batch *b, uint32_t flags, hb_batch_stats *stats)
     d.bias = b->bias_all.ptr;
     d.frontier = (flags & HB_FLAG_DENSE) ? 0 : 1;
     hb::HbBatchList lists[hb::kQueues], hub_lists[hb::kBatchNP];
-    for (int q = 0; q < hb::kQueues; ++q) lists[q] = {b->tiles[q].ptr, nullptr, b->ntiles[q], (uint32_t)q};
-    for (int pi = 0; pi < hb::kBatchNP; ++pi) hub_lists[pi] = {nullptr, b->hubs[pi].ptr, b->nhubs[pi], 0};
+    hb::HbBatchClasses classes;
+    for (int q = 0; q < hb::kQueues; ++q)
+        classes.q[q] = lists[q] = {b->tiles[q].ptr, nullptr, b->ntiles[q], (uint32_t)q, b->tile_base[q]};
+    for (int pi = 0; pi < hb::kBatchNP; ++pi) hub_lists[pi] = {nullptr, b->hubs[pi].ptr, b->nhubs[pi], 0, 0};
 
     hipEvent_t e0, e1, e2;
     HB_BTRY(hipEventCreate(&e0));
@@ -769,6 +846,7 @@ int hb_batch_run(hb_batch *b, uint32_t flags, hb_batch_stats *stats)
     const bool trace = trace_on();
     auto launch_iter = [&]() -> hipError_t {
         hb::k_batch_begin<<<(unsigned)std::min(ng, 1024), hb::kBatchBlock, 0, b->stream>>>(d);
+        hb::k_batch_compact<<<hb::kQueues, 1024, 0, b->stream>>>(d, classes);
         // per precision: the three node bins (team widths G, 4G capped at 64, and 64),
         // the hub chunks, then the hubs
 #define HB_BATCH_LAUNCH(q, kernel)                                                                     \
@@ -927,15 +1005,26 @@ static const uint8_t *batch_regs(hb_batch *b, int graph, int which)
     return (g.st.parity ? b->regs0.ptr : b->regs1.ptr) + g.reg_base;
 }
 
+// n rows of row_bytes of graph `graph` from device src (its device order) to host dst
+// in the caller's node order
+static int batch_copy_out(hb_batch *b, int graph, const void *src, void *dst, size_t row_bytes)
+{
+    const auto &g = b->hg[graph];
+    const size_t bytes = (size_t)g.n * row_bytes;
+    if (b->readback.count < bytes) HB_BTRY(b->readback.alloc(bytes));
+    HB_BTRY(hb::unpermute_rows(src, b->readback.ptr, b->sigma.ptr + g.node_base, g.n, row_bytes, b->stream));
+    HB_BTRY(hipMemcpyAsync(dst, b->readback.ptr, bytes, hipMemcpyDeviceToHost, b->stream));
+    HB_BTRY(hipStreamSynchronize(b->stream));
+    return HB_OK;
+}
+
 int hb_batch_read_registers(hb_batch *b, int graph, int which, uint8_t *dst)
 {
     if (!b || !dst || graph < 0 || graph >= (int)b->hg.size() || (which != 0 && which != 1))
         return bfail(HB_EINVAL, "hb_batch_read_registers: bad arguments");
     if (!b->ran) return bfail(HB_ESTATE, "hb_batch_read_registers: no completed run");
     HB_BTRY(hipSetDevice(b->device));
-    const auto &g = b->hg[graph];
-    HB_BTRY(hipMemcpy(dst, batch_regs(b, graph, which), (size_t)g.n << g.p, hipMemcpyDeviceToHost));
-    return HB_OK;
+    return batch_copy_out(b, graph, batch_regs(b, graph, which), dst, (size_t)1 << b->hg[graph].p);
 }
 
 int hb_batch_read_estimates(hb_batch *b, int graph, int which, float *dst)
@@ -944,11 +1033,8 @@ int hb_batch_read_estimates(hb_batch *b, int graph, int which, float *dst)
         return bfail(HB_EINVAL, "hb_batch_read_estimates: bad arguments");
     if (!b->ran) return bfail(HB_ESTATE, "hb_batch_read_estimates: no completed run");
     HB_BTRY(hipSetDevice(b->device));
-    const auto &g = b->hg[graph];
     HB_BTRY(hb::launch_estimates(graph_dev(b, graph), batch_regs(b, graph, which), b->est_tmp.ptr, b->stream));
-    HB_BTRY(hipMemcpyAsync(dst, b->est_tmp.ptr, (size_t)g.n * 4, hipMemcpyDeviceToHost, b->stream));
-    HB_BTRY(hipStreamSynchronize(b->stream));
-    return HB_OK;
+    return batch_copy_out(b, graph, b->est_tmp.ptr, dst, 4);
 }
 
 int hb_batch_read_accumulators(hb_batch *b, int graph, double *dist, double *inv)
@@ -957,8 +1043,14 @@ int hb_batch_read_accumulators(hb_batch *b, int graph, double *dist, double *inv
     if (!b->ran) return bfail(HB_ESTATE, "hb_batch_read_accumulators: no completed run");
     HB_BTRY(hipSetDevice(b->device));
     const auto &g = b->hg[graph];
-    if (dist) HB_BTRY(hipMemcpy(dist, b->dist.ptr + g.node_base, (size_t)g.n * 8, hipMemcpyDeviceToHost));
-    if (inv) HB_BTRY(hipMemcpy(inv, b->inv.ptr + g.node_base, (size_t)g.n * 8, hipMemcpyDeviceToHost));
+    if (dist) {
+        const int rc = batch_copy_out(b, graph, b->dist.ptr + g.node_base, dist, 8);
+        if (rc != HB_OK) return rc;
+    }
+    if (inv) {
+        const int rc = batch_copy_out(b, graph, b->inv.ptr + g.node_base, inv, 8);
+        if (rc != HB_OK) return rc;
+    }
     return HB_OK;
 }
 
@@ -985,7 +1077,7 @@ extern "C" int hb_batch_autotune(hb_batch *b, uint32_t *small_max_deg, uint32_t 
     };
     // 0 = the per-precision defaults of the single-graph engine
     uint32_t knob[3] = {b->small_max, b->mid_max, b->tile_arcs};
-    const uint32_t cand[3][4] = {{0, 16, 64, 256}, {0, 256, 1024, 4096}, {4096, 8192, 16384, 65536}};
+    const uint32_t cand[3][4] = {{0, 16, 64, 256}, {0, 256, 1024, 4096}, {256, 512, 1024, 4096}};
     double best = 0.0;
     int rc = trial(knob[0], knob[1], knob[2], &best);
     if (rc != HB_OK) return rc;
