@@ -174,10 +174,11 @@ int hb_set_schedule(hb_ctx *ctx, uint32_t small_max_deg, uint32_t wave_max_deg,
 
 /* ---- batches: many independent graphs advanced together ------------------
  * No reference counterpart (the reference runs one graph per HyperBallInner).
- * Graphs of mixed size and precision (p = 4..8) share one launch per iteration
- * of a work-queue kernel over (graph, degree-bin) tiles; each graph keeps its own
- * iteration count, termination rule and buffer parity, and its results are
- * bit-identical to hb_run on that graph alone. */
+ * Graphs of mixed size and precision (p = 4..8) are advanced together: one hipGraph
+ * replay per iteration of persistent kernels over the (graph, degree-bin) tiles of the
+ * graphs still running; each graph keeps its own iteration count, termination rule and
+ * buffer parity, and its results are bit-identical to hb_run on that graph alone.
+ * SPARKX_HB_TRACE=1 in the environment prints one progress line per host check. */
 typedef struct {
     const void *offsets;
     int off_bytes;      /* 4 or 8 */
