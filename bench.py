@@ -535,7 +535,7 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true",
                     help="leave out the surrogate field of the R-MAT-24 line (trained after the timed steps)")
     ap.add_argument("--graph-cache", default=None, help="path prefix to save/load the generated CSR (.npy)")
-    ap.add_argument("--layout", default="degree", choices=["degree", "input"],
+    ap.add_argument("--layout", default="auto", choices=["auto", "degree", "chain", "input"],
                     help="device node numbering (results are identical)")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -45,10 +45,16 @@ typedef enum {
 
 /* hb_set_layout: node numbering on the device */
 #define HB_LAYOUT_INPUT 0     /* the caller's numbering */
-#define HB_LAYOUT_DEGREE 1    /* default: nodes by descending degree (ties by id), each
-                                 neighbour list ascending; built on the device when the
-                                 graph is staged.  Results are identical and every read
-                                 returns the caller's order. */
+#define HB_LAYOUT_DEGREE 1    /* nodes by descending degree (ties by id), each neighbour
+                                 list ascending; built on the device when the graph is
+                                 staged.  Results are identical and every read returns
+                                 the caller's order. */
+#define HB_LAYOUT_CHAIN 2     /* nodes by (nearest seed, level) of a multi-source
+                                 breadth-first search from about one node in 64: along
+                                 long low-degree paths (de Bruijn graphs of genomes)
+                                 neighbours get nearby numbers and share cache lines */
+#define HB_LAYOUT_AUTO 3      /* default: CHAIN when every degree is <= 64 and there are
+                                 at most 4 arcs per node on average, DEGREE otherwise */
 
 /* hb_centrality kinds */
 #define HB_CLOSENESS 0
@@ -93,7 +99,7 @@ typedef struct {
        neighbour ids scanned; iteration 1 hashes ids instead of reading blocks, and
        outside dense iterations only neighbours that changed are read */
     uint64_t kernel_gathered[HB_NKERNELS];
-    int32_t layout;              /* HB_LAYOUT_* the run used */
+    int32_t layout;              /* HB_LAYOUT_* the run used (INPUT, DEGREE or CHAIN) */
     float ms_layout;             /* device time it took to build the degree layout of the
                                     staged graph (at hb_stage_csr / hb_set_layout; not
                                     part of ms_init or ms_total); 0 for HB_LAYOUT_INPUT */

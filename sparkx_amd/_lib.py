@@ -13,7 +13,7 @@ LIB_PATH = os.environ.get("SPARKX_HB_LIB") or os.path.join(HERE, "libsparkx_hb.s
 HB_OK, HB_EINVAL, HB_ENOMEM, HB_EHIP, HB_ESTATE, HB_ERANGE = 0, -1, -2, -3, -4, -5
 HB_FLAG_DENSE, HB_FLAG_GRAPH, HB_FLAG_NF, HB_FLAG_TIMING, HB_FLAG_KTIMING = 0x1, 0x2, 0x4, 0x8, 0x10
 HB_REGS_REFERENCE, HB_REGS_LATEST = 0, 1
-HB_LAYOUT_INPUT, HB_LAYOUT_DEGREE = 0, 1
+HB_LAYOUT_INPUT, HB_LAYOUT_DEGREE, HB_LAYOUT_CHAIN, HB_LAYOUT_AUTO = 0, 1, 2, 3
 HB_CLOSENESS, HB_HARMONIC, HB_LIN = 0, 1, 2
 HB_STATS_MAX_ITERS = 4096
 HB_NKERNELS = 5

@@ -49,7 +49,7 @@ class HyperBallMany:
     """Runs HyperBall on every (graph, p) of `items` concurrently; `hbs[i]` is the
     HyperBall of item i (its accessors, epilogues and caches as usual)."""
 
-    def __init__(self, items, device: int = 0, threads: int = 0, layout: str = "degree"):
+    def __init__(self, items, device: int = 0, threads: int = 0, layout: str = "auto"):
         from .hyper_ball import HyperBall
         self.items = list(items)
         self.threads = threads

@@ -199,16 +199,19 @@ struct hb_ctx {
     DevBuf<uint32_t> walk_hub;  // hub chunk table of the walk passes, then the u32 degrees
     uint32_t walk_chunks = 0, walk_hubs = 0;  // of the table in walk_hub (valid while allocated)
 
-    // node order on the device: HB_LAYOUT_DEGREE renumbers the nodes by descending
-    // degree at the first run after staging; reads return the caller's order
-    int layout = HB_LAYOUT_DEGREE;
+    // node order on the device (HB_LAYOUT_*), built when the graph is staged; reads
+    // return the caller's order.  `layout` is what was asked for, `layout_used` what
+    // the staged graph got (HB_LAYOUT_AUTO resolves to CHAIN or DEGREE)
+    int layout = HB_LAYOUT_AUTO;
+    int layout_used = HB_LAYOUT_INPUT;
+    uint32_t chain_levels = 0;  // breadth-first levels the chain layout searched
     bool laid_out = false;  // perm .. rorig hold that numbering of the staged graph
     DevBuf<uint32_t> perm, sigma, roff, rnbr, rorig;
     std::vector<uint32_t> h_roff;
     float ms_layout = 0.0f;  // device time of building it
     DevBuf<uint8_t> readback;  // per-node results moved back to the caller's order
 
-    bool relabelled() const { return layout == HB_LAYOUT_DEGREE && laid_out; }
+    bool relabelled() const { return layout_used != HB_LAYOUT_INPUT && laid_out; }
     const uint32_t *run_off() const { return relabelled() ? roff.ptr : off.ptr; }
     const uint32_t *run_nbr() const { return relabelled() ? rnbr.ptr : nbr.ptr; }
     const uint32_t *run_orig() const { return relabelled() ? rorig.ptr : nbr.ptr; }
@@ -224,6 +227,8 @@ struct hb_ctx {
         rorig.release();
         h_roff.clear();
         laid_out = false;
+        layout_used = HB_LAYOUT_INPUT;
+        chain_levels = 0;
         ms_layout = 0.0f;
     }
 
@@ -348,7 +353,8 @@ int hb_set_schedule(hb_ctx *c, uint32_t small_max_deg, uint32_t wave_max_deg, ui
 int hb_set_layout(hb_ctx *c, int layout)
 {
     if (!c) return fail(HB_EINVAL, "hb_set_layout: ctx is NULL");
-    if (layout != HB_LAYOUT_INPUT && layout != HB_LAYOUT_DEGREE)
+    if (layout != HB_LAYOUT_INPUT && layout != HB_LAYOUT_DEGREE && layout != HB_LAYOUT_CHAIN &&
+        layout != HB_LAYOUT_AUTO)
         return fail(HB_EINVAL, "hb_set_layout: bad layout %d", layout);
     if (layout != c->layout) {
         if (!set_device(c)) return fail(HB_EHIP, "hipSetDevice failed");
@@ -597,11 +603,24 @@ static float alpha_f32(uint32_t m)
     return 0.7213f / (1.0f + 1.079f / (float)m);
 }
 
-// Builds the degree-ordered numbering of the staged graph once (device work, timed
-// with events into ms_layout); a no-op for HB_LAYOUT_INPUT or when already built.
+// HB_LAYOUT_AUTO: the chain layout for graphs made of long low-degree paths (every
+// degree <= 64 and at most 4 arcs per node on average, as de Bruijn graphs of
+// genomes), where it puts a node's neighbours in its own cache lines; the degree
+// layout otherwise, where the gathers go to the high-degree nodes' rows.
+static int resolve_layout(const hb_ctx *c)
+{
+    if (c->layout != HB_LAYOUT_AUTO) return c->layout;
+    uint32_t max_deg = 0;
+    for (size_t u = 0; u + 1 < c->h_off.size(); ++u) max_deg = std::max(max_deg, c->h_off[u + 1] - c->h_off[u]);
+    return max_deg <= 64 && c->arcs <= 4 * (uint64_t)c->n ? HB_LAYOUT_CHAIN : HB_LAYOUT_DEGREE;
+}
+
+// Builds the device numbering of the staged graph once (device work, timed with
+// events into ms_layout); a no-op for HB_LAYOUT_INPUT or when already built.
 static int ensure_layout(hb_ctx *c)
 {
-    if (c->layout != HB_LAYOUT_DEGREE || c->laid_out || c->n == 0) return HB_OK;
+    if (c->layout == HB_LAYOUT_INPUT || c->laid_out || c->n == 0) return HB_OK;
+    const int kind = resolve_layout(c);
     const uint32_t n = (uint32_t)c->n, arcs = (uint32_t)c->arcs;
     HB_TRY(c->perm.ensure(n));
     HB_TRY(c->sigma.ensure(n));
@@ -612,8 +631,12 @@ static int ensure_layout(hb_ctx *c)
     HB_TRY(hipEventCreate(&e0));
     HB_TRY(hipEventCreate(&e1));
     (void)hipEventRecord(e0, c->stream);
-    hipError_t e = hb::relabel_by_degree(c->off.ptr, c->nbr.ptr, n, arcs, c->perm.ptr, c->sigma.ptr, c->roff.ptr,
-                                         c->rnbr.ptr, c->rorig.ptr, c->stream);
+    uint32_t levels = 0;
+    hipError_t e = kind == HB_LAYOUT_CHAIN
+                       ? hb::relabel_by_chain(c->off.ptr, c->nbr.ptr, n, arcs, c->perm.ptr, c->sigma.ptr, c->roff.ptr,
+                                              c->rnbr.ptr, c->rorig.ptr, 4096, &levels, c->stream)
+                       : hb::relabel_by_degree(c->off.ptr, c->nbr.ptr, n, arcs, c->perm.ptr, c->sigma.ptr,
+                                               c->roff.ptr, c->rnbr.ptr, c->rorig.ptr, c->stream);
     if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
     std::vector<uint32_t> h((size_t)n + 1);
     if (e == hipSuccess) e = hipMemcpyAsync(h.data(), c->roff.ptr, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost,
@@ -625,11 +648,13 @@ static int ensure_layout(hb_ctx *c)
     (void)hipEventDestroy(e1);
     if (e != hipSuccess) {
         c->release_layout();
-        return fail(e == hipErrorOutOfMemory ? HB_ENOMEM : HB_EHIP, "degree layout failed: %s",
-                    hipGetErrorString(e));
+        return fail(e == hipErrorOutOfMemory ? HB_ENOMEM : HB_EHIP, "%s layout failed: %s",
+                    kind == HB_LAYOUT_CHAIN ? "chain" : "degree", hipGetErrorString(e));
     }
     c->h_roff.swap(h);
     c->ms_layout = ms;
+    c->layout_used = kind;
+    c->chain_levels = levels;
     c->laid_out = true;
     return HB_OK;
 }
@@ -915,7 +940,7 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
                 if (sl < HB_SLOT_CHUNK) stats->kernel_first_nodes[sl] = c->h_rec[1].nodes[sl];
             }
         stats->symmetric = c->sym ? 1 : 0;
-        stats->layout = c->relabelled() ? HB_LAYOUT_DEGREE : HB_LAYOUT_INPUT;
+        stats->layout = c->relabelled() ? c->layout_used : HB_LAYOUT_INPUT;
         stats->ms_layout = c->relabelled() ? c->ms_layout : 0.0f;
         for (uint32_t t = 1; t <= executed; ++t) {
             const HbIterRec &r = c->h_rec[t];

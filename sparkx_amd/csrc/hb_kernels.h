@@ -51,6 +51,13 @@ hipError_t launch_centrality(const HbDev &d, const float *est, int kind, int nor
 // each list ascending), rorig[e] the caller's id of rnbr[e].  Synchronises s.
 hipError_t relabel_by_degree(const uint32_t *off, const uint32_t *nbr, uint32_t n, uint32_t arcs, uint32_t *perm,
                              uint32_t *sigma, uint32_t *roff, uint32_t *rnbr, uint32_t *rorig, hipStream_t s);
+// Chain layout: nodes ordered by (nearest seed, level) of a multi-source breadth-first
+// search from about one linked node in 64 (pseudo-random, fixed), then unreached linked
+// nodes and isolated nodes by id; same outputs as relabel_by_degree.  *levels receives
+// the number of levels searched (at most max_levels).  Synchronises s.
+hipError_t relabel_by_chain(const uint32_t *off, const uint32_t *nbr, uint32_t n, uint32_t arcs, uint32_t *perm,
+                            uint32_t *sigma, uint32_t *roff, uint32_t *rnbr, uint32_t *rorig, uint32_t max_levels,
+                            uint32_t *levels, hipStream_t s);
 // dst row u (caller's order) = src row sigma[u]; row_bytes a multiple of 4
 hipError_t unpermute_rows(const void *src, void *dst, const uint32_t *sigma, uint32_t n, size_t row_bytes,
                           hipStream_t s);

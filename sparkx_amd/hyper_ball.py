@@ -31,7 +31,7 @@ class HyperBall:
 
     def __init__(self, g: Graph, p: int, *, max_iters: int = 0, dense: bool = False,
                  graph_capture: bool = False, timing: bool = False, nf: bool = False,
-                 ktiming: bool = False, layout: str = "degree",
+                 ktiming: bool = False, layout: str = "auto",
                  device: int = 0, schedule: Optional[tuple] = None, _run: bool = True, _stage_fn=None):
         if not 4 <= p <= 16:
             raise ValueError(f"precision p={p} outside 4..16")
@@ -79,12 +79,16 @@ class HyperBall:
         return cls(None, cls.PRECISION if p is None else p, _stage_fn=stage_mtx_fn(text), **kw)
 
     # ---- engine ---------------------------------------------------------------
-    LAYOUTS = {"input": _lib.HB_LAYOUT_INPUT, "degree": _lib.HB_LAYOUT_DEGREE}
+    LAYOUTS = {"input": _lib.HB_LAYOUT_INPUT, "degree": _lib.HB_LAYOUT_DEGREE, "chain": _lib.HB_LAYOUT_CHAIN,
+               "auto": _lib.HB_LAYOUT_AUTO}
 
     def set_layout(self, layout: str) -> None:
-        """Node numbering on the device: "degree" (default; nodes by descending degree,
-        built on the GPU at the first run) or "input".  Results are the same either way
-        and always come back in the graph's own node order."""
+        """Node numbering on the device, built on the GPU when the graph is staged:
+        "degree" (nodes by descending degree), "chain" (nodes along the paths of a
+        multi-source breadth-first search, for long low-degree paths such as de Bruijn
+        graphs), "auto" (default: "chain" when every degree is <= 64 and the graph has at
+        most 4 arcs per node, "degree" otherwise) or "input".  Results are the same
+        either way and always come back in the graph's own node order."""
         if layout not in self.LAYOUTS:
             raise ValueError(f"layout {layout!r} not in {sorted(self.LAYOUTS)}")
         _lib.check(self._L.hb_set_layout(self._ctx, self.LAYOUTS[layout]))

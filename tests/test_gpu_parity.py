@@ -586,21 +586,25 @@ def _layout_graph(hb, name):
 
 @pytest.mark.parametrize("name,p", [("rmat13", 4), ("rmat13", 7), ("rmat13", 10), ("rmat13", 12),
                                     ("directed", 7), ("multi", 6), ("multi", 11), ("kmer", 6)])
-def test_degree_layout_matches_input_layout_and_oracle(hb, oracle, name, p):
-    """The degree-ordered device numbering changes nothing a caller can read: both
-    layouts match the oracle bit for bit, with the same neighbourhood function and
-    centralities, and the layout is built once per staged graph."""
+def test_device_layouts_match_input_layout_and_oracle(hb, oracle, name, p):
+    """The device numberings (degree order, chain order) change nothing a caller can
+    read: every layout matches the oracle bit for bit, with the same neighbourhood
+    function and centralities, and the layout is built once per staged graph; "auto"
+    takes the chain order for low-degree sparse graphs (the k-mer and the random
+    directed graph) and the degree order for the others."""
     g = _layout_graph(hb, name)
     say("layout", name, p, g.size(), g.width())
     ref = _oracle_run(oracle, g, p)
     out = {}
     # the "multi" hub (degree 918) goes through the chunk / merge kernels
     sched = (4, 256, 128) if name == "multi" else None
-    for layout in ("input", "degree"):
+    chain_auto = int(g.degrees().max()) <= 64 and g.width() <= 4 * g.size()  # the auto rule
+    codes = {"input": 0, "degree": 1, "chain": 2, "auto": 2 if chain_auto else 1}
+    for layout in ("input", "degree", "chain", "auto"):
         with hb.HyperBall(g, p, layout=layout, nf=True, schedule=sched) as h:
             st = h.stats
-            assert st.layout == (1 if layout == "degree" else 0)
-            assert (st.ms_layout > 0) == (layout == "degree")
+            assert st.layout == codes[layout], layout
+            assert (st.ms_layout > 0) == (layout != "input")
             _assert_same(h, ref, oracle, p, f"{name} p={p} {layout}")
             nf = list(st.nf[:st.iterations])
             cent = [h.compute_closeness_centrality(True), h.compute_harmonic_centrality(False),
@@ -608,9 +612,10 @@ def test_degree_layout_matches_input_layout_and_oracle(hb, oracle, name, p):
             st2 = h.run(nf=True)  # same staged graph: the layout is reused, not rebuilt
             assert st2.ms_layout == st.ms_layout and st2.iterations == ref.iterations
             out[layout] = (nf, cent)
-    assert out["input"][0] == out["degree"][0]
-    for a, b in zip(out["input"][1], out["degree"][1]):
-        assert np.array_equal(a, b)
+    for layout in ("degree", "chain", "auto"):
+        assert out["input"][0] == out[layout][0]
+        for a, b in zip(out["input"][1], out[layout][1]):
+            assert np.array_equal(a, b)
 
 
 def test_layout_switch_and_snapshots(hb, oracle):
@@ -632,6 +637,9 @@ def test_layout_switch_and_snapshots(hb, oracle):
         h.set_layout("degree")
         with pytest.raises(hb.HyperBallError):
             h.registers(0)
+        h.run()
+        assert np.array_equal(snapshots(h), snaps_in)
+        h.set_layout("chain")
         h.run()
         assert np.array_equal(snapshots(h), snaps_in)
         with pytest.raises(ValueError):

@@ -56,7 +56,7 @@ def _graphs(hb):
     yield "isolated", hb.Graph(off, nb, "iso")
 
 
-@pytest.mark.parametrize("layout", ["degree", "input"])
+@pytest.mark.parametrize("layout", ["degree", "chain", "input"])
 @pytest.mark.parametrize("passes", [2, 3])
 def test_walk_features_match_oracle(hb, oracle, layout, passes):
     from sparkx_amd.surrogate import walk_features
