@@ -115,7 +115,7 @@ def cpu_baseline(g, p, threads, iters):
     return r.iterations / r.loop_seconds, r.loop_seconds, r.iterations, r.init_seconds
 
 
-def bench_surrogate(args, spec, eval_graphs=None):
+def bench_surrogate(args, spec, eval_graphs=None, graphs=None):
     """Learned surrogate: MLP trained on GPU (Adam, lr 1e-3, MSE) on seeded R-MAT
     scale-20 graphs labelled by full p=7 HyperBall (inputs and target log(1+H)
     standardised per graph over linked nodes);
@@ -156,7 +156,7 @@ def bench_surrogate(args, spec, eval_graphs=None):
     results = {}
     evals = eval_graphs or [("rmat24", CONFIGS["rmat24"][0], 7, 0), ("kmer", CONFIGS["kmer"][0], 6, 64)]
     for name, gspec, p_full, mi in evals:
-        g = make_graph(gspec, args.graph_cache if name == "rmat24" else None)
+        g = (graphs or {}).get(name) or make_graph(gspec, args.graph_cache if name == "rmat24" else None)
         with HyperBall(g, p_full, max_iters=mi) as full:
             full.run(max_iters=mi)  # warm
             st = full.run(max_iters=mi)
@@ -171,6 +171,7 @@ def bench_surrogate(args, spec, eval_graphs=None):
                 net.rank_walks(h, passes)  # warm
                 walk_out[passes] = net.rank_walks(h, passes)
         linked = g.degrees() > 0
+        arcs = g.width()
         t0 = time.perf_counter()
         m_all = metrics.evaluate(pred, truth, 0.01)  # on the GPU, reference tie semantics
         m_lnk = metrics.evaluate(pred[linked], truth[linked], 0.01)
@@ -186,8 +187,9 @@ def bench_surrogate(args, spec, eval_graphs=None):
                 "spearman_all_nodes": round(w_all["spearman"], 5),
                 "spearman_linked_nodes": round(w_lnk["spearman"], 5),
                 "top1pct_overlap": round(w_all["top_overlap"], 5)}
+        del g
         results[name] = {
-            "nodes": g.size(), "arcs": g.width(), "full_p": p_full, "full_iterations": int(st.iterations),
+            "nodes": int(linked.size), "arcs": int(arcs), "full_p": p_full, "full_iterations": int(st.iterations),
             "full_ms": round(full_ms, 3), "surrogate_ms": round(sur_ms, 3),
             "time_ratio": round(sur_ms / full_ms, 4),
             "spearman_all_nodes": round(m_all["spearman"], 5),
@@ -659,50 +661,48 @@ def main():
         "cpu_baseline": cpu,
         "speedup_vs_cpu": round(value / cpu["value"], 2) if cpu else None,
     }
-    if args.config == "rmat24" and not args.no_surrogate:
-        out["surrogate"] = surrogate_on_run(h, g)
     h.close()
+    if args.config == "rmat24" and not args.no_surrogate:
+        out["surrogate"] = surrogate_on_run(args, g)
     print(json.dumps(out), flush=True)
 
 
-def surrogate_on_run(h, g, train_graphs=16, passes=2):
-    """The second half of the headline metric, after the timed steps: a walk-count
-    surrogate (2 passes) trained on seeded R-MAT-20 graphs labelled by full p=7 runs
-    ranks the benchmark graph; its device time against the full run's (register init +
-    iterations) and its Spearman rho against the full run's harmonic accumulator, the
-    reference's tie semantics, on the GPU (--config surrogate: the longer study)."""
-    import numpy as np
-
-    from sparkx_amd import Graph, metrics
-    from sparkx_amd.surrogate import Surrogate, labelled_walks
+def surrogate_on_run(args, g):
+    """The second half of the headline metric, after the timed steps: the learned
+    surrogate as configured -- an MLP trained on 32 seeded R-MAT-20 graphs labelled by
+    full p=7 runs -- with each input set (HyperBall snapshots N_1..N_4 with degree; walk
+    counts from 2 or 3 gather passes), ranking the benchmark graph and the k-mer graph
+    (t <= 64): Spearman rho (reference tie semantics, on the GPU), top-1% overlap and
+    device time against the full run's.  `best` names the input set with the highest
+    rho on the benchmark graph among those within 1/5 of the full run's time."""
+    spec = CONFIGS["surrogate"][0]
     t0 = time.perf_counter()
-    xs, ys = [], []
-    for i in range(train_graphs):
-        tg = Graph.rmat(20, 16, seed=1000 + i)
-        x, y, _, _ = labelled_walks(tg, passes)
-        linked = tg.degrees() > 0
-        xs.append(x[linked])
-        ys.append(y[linked])
-    net = Surrogate(seed=1)
-    net.train(np.concatenate(xs), np.concatenate(ys), epochs=4, batch=8192, lr=1e-3, seed=1)
-    del xs, ys
-    st = h.run()  # the denominator: one plain run (no per-kernel events) of the same graph
-    full_ms = st.ms_total + st.ms_init
-    truth = h.harmonic_geodesic_distances().copy()
-    net.rank_walks(h, passes)  # warm
-    ms = [net.rank_walks(h, passes)[2] for _ in range(3)]
-    pred, _, _ = net.rank_walks(h, passes)
-    net.close()
-    linked = g.degrees() > 0
-    m = metrics.evaluate(pred[linked], truth[linked], 0.01)
-    m_all = metrics.evaluate(pred, truth, 0.01)
-    sur_ms = sum(ms) / len(ms)
-    log(f"surrogate: {sur_ms:.3f} ms ({sur_ms / full_ms:.3f} of the run), rho {m['spearman']:.4f} "
-        f"(trained and evaluated in {time.perf_counter() - t0:.1f}s)")
-    return {"inputs": f"walk counts, {passes} passes", "train": f"{train_graphs} R-MAT-20 graphs, 4 epochs",
-            "ms": round(sur_ms, 3), "full_run_ms": round(full_ms, 3), "time_ratio": round(sur_ms / full_ms, 4),
-            "spearman_linked_nodes": round(m["spearman"], 5), "spearman_all_nodes": round(m_all["spearman"], 5),
-            "top1pct_overlap": round(m_all["top_overlap"], 5)}
+    study = bench_surrogate(args, spec, graphs={"rmat24": g})
+    rows = {}
+    for name, r in study["eval"].items():
+        sets = {"snapshots_N1_N4": {k: r[k] for k in ("surrogate_ms", "time_ratio", "spearman_linked_nodes",
+                                                       "spearman_all_nodes", "top1pct_overlap")}}
+        for k, w in r["walks"].items():
+            sets[f"walks_{k}"] = dict(w)
+        for v in sets.values():
+            v["meets_target"] = bool(v["spearman_linked_nodes"] >= 0.95 and v["time_ratio"] <= 0.2)
+        rows[name] = {"full_ms": r["full_ms"], "full_iterations": r["full_iterations"], "inputs": sets,
+                      "no_network_4_iterations": {"spearman_linked_nodes": r["baseline_4iter_spearman_linked"],
+                                                  "top1pct_overlap": r["baseline_4iter_top1pct_overlap"]}}
+    fast = {k: v for k, v in rows["rmat24"]["inputs"].items() if v["time_ratio"] <= 0.2}
+    best = max(fast, key=lambda k: fast[k]["spearman_linked_nodes"]) if fast else None
+    log(f"surrogate study: best within 1/5 of the run on rmat24: {best} "
+        f"({time.perf_counter() - t0:.1f}s with training)")
+    out = {"train": f"{spec['train_graphs']} R-MAT-{spec['train_scale']} graphs labelled by full p=7 runs, "
+                    "Adam lr 1e-3, 4 epochs, MSE on log(1 + H) standardised per graph",
+           "target": "Spearman >= 0.95 in <= 1/5 of the full run's GPU time",
+           "best": best, "eval": rows}
+    if best:
+        b = rows["rmat24"]["inputs"][best]
+        out.update({"inputs": best, "ms": b["surrogate_ms"], "full_run_ms": rows["rmat24"]["full_ms"],
+                    "time_ratio": b["time_ratio"], "spearman_linked_nodes": b["spearman_linked_nodes"],
+                    "spearman_all_nodes": b["spearman_all_nodes"], "top1pct_overlap": b["top1pct_overlap"]})
+    return out
 
 
 if __name__ == "__main__":
