@@ -95,6 +95,7 @@ typedef struct {
     uint64_t regime[3];      /* estimates after the union, over all (node, iteration):
                                 [0] linear counting, [1] raw with the HLL++ bias
                                 subtracted (raw <= 5m), [2] raw as is */
+    double iter_seconds[1024]; /* wall time of each iteration (all threads) */
 } hbo_stats;
 
 /*
@@ -113,6 +114,12 @@ int hbo_hyperball(const uint64_t *offsets, const uint32_t *nbrs, uint64_t n, int
                   const hbo_variant *v, int threads, int packed, int max_iters,
                   uint8_t *regs_ref, uint8_t *regs_last, double *dist, double *inv,
                   hbo_stats *stats, hbo_iter_cb cb, void *user);
+
+/* Single-thread timing of the first iteration's node work on a pseudo-random sample
+ * of about one node in `stride` (registers initialised first, not timed);
+ * seconds[reps] per repetition, *nodes / *arcs the sample's size. */
+int hbo_node_sample(const uint64_t *offsets, const uint32_t *nbrs, uint64_t n, int p, const hbo_variant *v,
+                    int packed, uint64_t stride, int reps, double *seconds, uint64_t *nodes, uint64_t *arcs);
 
 /* estimates of every node's sketch (u8 registers) */
 void hbo_estimates(const hbo_variant *v, int p, uint64_t n, const uint8_t *regs, float *out);

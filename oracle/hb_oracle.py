@@ -58,6 +58,7 @@ class Stats(C.Structure):
         ("init_seconds", C.c_double),
         ("loop_seconds", C.c_double),
         ("regime", C.c_uint64 * 3),
+        ("iter_seconds", C.c_double * 1024),
     ]
 
 
@@ -116,6 +117,9 @@ def lib():
         L.hbo_knn_bias.restype = C.c_double
         L.hbo_knn_bias.argtypes = [C.c_int, C.c_double]
         L.hbo_knn_bias_many.argtypes = [C.c_int, f32p, C.c_uint64, f64p]
+        L.hbo_node_sample.restype = C.c_int
+        L.hbo_node_sample.argtypes = [u64p, u32p, C.c_uint64, C.c_int, vp, C.c_int, C.c_uint64, C.c_int, f64p,
+                                      C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
         L.hbo_pack6.argtypes = [C.c_int, u8p, u32p]
         L.hbo_unpack6.argtypes = [C.c_int, u32p, u8p]
         L.hbo_hyperball.restype = C.c_int
@@ -166,6 +170,21 @@ def index_rank(p: int, x: int, v: Variant | None = None):
     r = C.c_uint8()
     idx = lib().hbo_index_rank(C.byref(v), p, x, C.byref(r))
     return int(idx), int(r.value)
+
+
+def node_sample(offsets, nbrs, p: int, stride: int, reps: int = 3, packed: bool = True):
+    """(seconds per repetition, sampled nodes, their arcs): one thread doing the first
+    iteration's work for a pseudo-random sample of about one node in `stride`."""
+    off = np.ascontiguousarray(offsets, dtype=np.uint64)
+    nb = np.ascontiguousarray(nbrs, dtype=np.uint32)
+    secs = np.zeros(reps, np.float64)
+    nodes, arcs = C.c_uint64(), C.c_uint64()
+    v = pinned()
+    rc = lib().hbo_node_sample(_p(off, C.c_uint64), _p(nb, C.c_uint32), off.size - 1, p, C.byref(v), int(packed),
+                               stride, reps, _p(secs, C.c_double), C.byref(nodes), C.byref(arcs))
+    if rc != 0:
+        raise RuntimeError(f"hbo_node_sample failed: {rc}")
+    return secs, nodes.value, arcs.value
 
 
 def knn_bias(p: int, e: np.ndarray) -> np.ndarray:
@@ -220,6 +239,7 @@ class HBResult:
     # estimates after the union over all (node, iteration): linear counting, raw with
     # the HLL++ bias subtracted, raw as is
     regime: tuple = (0, 0, 0)
+    iter_seconds: tuple = ()  # wall time of each iteration
 
 
 def hyperball(offsets: np.ndarray, nbrs: np.ndarray, p: int, *, v: Variant | None = None,
@@ -254,7 +274,8 @@ def hyperball(offsets: np.ndarray, nbrs: np.ndarray, p: int, *, v: Variant | Non
     it = st.iterations
     return HBResult(regs_ref, regs_last, dist, inv, it, bool(st.final_parity_latest),
                     np.array(st.increases[:min(it, 1024)], dtype=np.uint64),
-                    st.init_seconds, st.loop_seconds, tuple(int(x) for x in st.regime))
+                    st.init_seconds, st.loop_seconds, tuple(int(x) for x in st.regime),
+                    tuple(float(x) for x in st.iter_seconds[:min(it, 1024)]))
 
 
 CLOSENESS, HARMONIC, LIN = 0, 1, 2
