@@ -135,6 +135,14 @@ int hb_stage_csr(hb_ctx *ctx, const void *offsets, int off_bytes, uint64_t n,
  * (src/centralities/hyper_ball.rs:124-157, :702-736, :754-874). `stats` may be NULL. */
 int hb_run(hb_ctx *ctx, int p, int max_iters, uint32_t flags, hb_stats *stats);
 
+/* Rebuilds a completed run of precision p on the staged graph from saved results:
+ * regs (n * 2^p u8 registers, node-major, the buffer HB_REGS_REFERENCE reads) and the
+ * f64 accumulators dist / inv, all in the caller's node order.  Every reader and
+ * epilogue then works as after hb_run; the restored run reports 0 iterations.
+ * Replaces: HyperBallInner::get_or_compute reopening the counters and accumulator
+ * files (src/centralities/hyper_ball.rs:165-201; its restored struct has iters = 0). */
+int hb_restore(hb_ctx *ctx, int p, const uint8_t *regs, const double *dist, const double *inv);
+
 /* Chooses the device numbering of the nodes (HB_LAYOUT_*) for the following runs
  * (builds it now if a graph is staged); discards the results of an earlier run if it
  * changes.  No reference counterpart: the reference walks the caller's CSR as given. */

@@ -8,6 +8,7 @@
 #pragma once
 
 #include <cstdint>
+#include <cstdio>
 #include <memory>
 #include <optional>
 #include <stdexcept>
@@ -54,19 +55,55 @@ public:
     {
         return HyperBall(g, 0, 0u, device);
     }
-    // extension: truncation and scheduling flags (HB_FLAG_*)
-    HyperBall(const CsrView &g, int max_iters, uint32_t flags, int device = 0) : g_(g)
+    // HyperBallInner::get_or_compute (:165-201): reopens the registers
+    // (hyperballregisters_<P>_<id>.mmap, this build's own file: the reference's
+    // counters file holds hyperloglog-rs structs whose layout is not available) and the
+    // accumulators (hyperballdistances_<id>.mmap, hyperballinvdistances_<id>.mmap) from
+    // cache_dir, then searches the centrality caches; without them it runs and saves
+    // them.  The object keeps the cache: compute_* write their files there.  A
+    // reopened run reports 0 iterations, as the reference's does.
+    static HyperBall get_or_compute(const CsrView &g, const std::string &cache_dir, const std::string &gid,
+                                    int device = 0)
     {
-        check(hb_create(&ctx_, device));
-        try {
-            check(hb_stage_csr(ctx_, g.offsets, g.offset_bytes, g.nodes, g.neighbours,
-                               g.neighbour_bytes, g.arcs));
-            stats_ = std::make_unique<hb_stats>();
-            check(hb_run(ctx_, P, max_iters, flags, stats_.get()));
-        } catch (...) {
-            hb_destroy(ctx_);
-            throw;
+        const size_t n = (size_t)g.nodes;
+        std::vector<uint8_t> regs;
+        std::vector<double> dist, inv;
+        if (read_file(file(cache_dir, "hyperballregisters", gid, P), regs, n << P) &&
+            read_file(file(cache_dir, "hyperballdistances", gid, -1), dist, n) &&
+            read_file(file(cache_dir, "hyperballinvdistances", gid, -1), inv, n)) {
+            HyperBall h(g, device);
+            check(hb_restore(h.ctx_, P, regs.data(), dist.data(), inv.data()));
+            h.stats_ = std::make_unique<hb_stats>();  // zeros: 0 iterations
+            h.stats_->p = P;
+            h.cache_dir_ = cache_dir;
+            h.gid_ = gid;
+            // every cached centrality that exists (the reference ignores the others)
+            const int kinds[5][2] = {{HB_CLOSENESS, 0}, {HB_HARMONIC, 0}, {HB_CLOSENESS, 1}, {HB_HARMONIC, 1},
+                                     {HB_LIN, 0}};
+            for (const auto &k : kinds) {
+                std::vector<double> &slot = k[0] == HB_LIN ? h.lin_ : k[0] == HB_CLOSENESS ? h.closeness_[k[1]]
+                                                                                            : h.harmonic_[k[1]];
+                try {
+                    (void)h.search(k[0], k[1] != 0, slot);
+                } catch (const HyperBallError &) {
+                }
+            }
+            return h;
         }
+        HyperBall h(g, 0, 0u, device);
+        h.cache_dir_ = cache_dir;
+        h.gid_ = gid;
+        write_file(file(cache_dir, "hyperballregisters", gid, P), h.counters());
+        write_file(file(cache_dir, "hyperballdistances", gid, -1), h.geodesic_distances());
+        write_file(file(cache_dir, "hyperballinvdistances", gid, -1), h.harmonic_geodesic_distances());
+        return h;
+    }
+
+    // extension: truncation and scheduling flags (HB_FLAG_*)
+    HyperBall(const CsrView &g, int max_iters, uint32_t flags, int device = 0) : HyperBall(g, device)
+    {
+        stats_ = std::make_unique<hb_stats>();
+        check(hb_run(ctx_, P, max_iters, flags, stats_.get()));
     }
     HyperBall(const HyperBall &) = delete;
     HyperBall &operator=(const HyperBall &) = delete;
@@ -79,6 +116,8 @@ public:
         std::swap(closeness_, o.closeness_);
         std::swap(harmonic_, o.harmonic_);
         std::swap(lin_, o.lin_);
+        std::swap(cache_dir_, o.cache_dir_);
+        std::swap(gid_, o.gid_);
         return *this;
     }
     ~HyperBall()
@@ -115,16 +154,30 @@ public:
         return harmonic_geodesic_distances()[node_id];
     }
 
-    // :434-477, :487-529, :532-555
+    // :434-477, :487-529, :532-555 (written to the cache when the object has one)
     const std::vector<double> &compute_closeness_centrality(bool normalize)
     {
-        return closeness_[normalize] = centrality(HB_CLOSENESS, normalize);
+        return saved(HB_CLOSENESS, normalize, closeness_[normalize] = centrality(HB_CLOSENESS, normalize));
     }
     const std::vector<double> &compute_harmonic_centrality(bool normalize)
     {
-        return harmonic_[normalize] = centrality(HB_HARMONIC, normalize);
+        return saved(HB_HARMONIC, normalize, harmonic_[normalize] = centrality(HB_HARMONIC, normalize));
     }
-    const std::vector<double> &compute_lins_centrality() { return lin_ = centrality(HB_LIN, false); }
+    const std::vector<double> &compute_lins_centrality()
+    {
+        return saved(HB_LIN, false, lin_ = centrality(HB_LIN, false));
+    }
+
+    // :302-317, :359-374, :405-415: the cached file into its slot, else Err
+    const std::vector<double> &search_cache_closeness_centrality(bool normalization)
+    {
+        return search(HB_CLOSENESS, normalization, closeness_[normalization]);
+    }
+    const std::vector<double> &search_cache_harmonic_centrality(bool normalization)
+    {
+        return search(HB_HARMONIC, normalization, harmonic_[normalization]);
+    }
+    const std::vector<double> &search_cache_lins_centrality() { return search(HB_LIN, false, lin_); }
 
     // :290-299, :347-356, :397-402: Err("no such cached centrality")
     const std::vector<double> &get_closeness_centrality(bool normalization) const
@@ -172,6 +225,64 @@ public:
     }
 
 private:
+    // a context with the graph staged, no run yet
+    HyperBall(const CsrView &g, int device) : g_(g)
+    {
+        check(hb_create(&ctx_, device));
+        const int rc = hb_stage_csr(ctx_, g.offsets, g.offset_bytes, g.nodes, g.neighbours, g.neighbour_bytes, g.arcs);
+        if (rc < 0) {
+            const HyperBallError e(rc, hb_last_error());
+            hb_destroy(ctx_);
+            ctx_ = nullptr;
+            throw e;
+        }
+    }
+
+    // <suffix>_<id>.mmap, or <suffix>_<seq>_<id>.mmap (graph/cache/utils.rs:318-341)
+    static std::string file(const std::string &dir, const char *suffix, const std::string &gid, int seq)
+    {
+        return dir + "/" + suffix + (seq >= 0 ? "_" + std::to_string(seq) : std::string()) + "_" + gid + ".mmap";
+    }
+    // closeness / harmonic: seq 1 without normalisation, 2 with; Lin 0 (hyper_ball.rs:203-246)
+    std::string centrality_file(int kind, bool normalize) const
+    {
+        const char *suffix = kind == HB_CLOSENESS ? "hyperballcloseness" : kind == HB_HARMONIC ? "hyperballharmonic"
+                                                                                              : "hyperballlin";
+        return file(cache_dir_, suffix, gid_, kind == HB_LIN ? 0 : (normalize ? 2 : 1));
+    }
+    template <class T>
+    static bool read_file(const std::string &path, std::vector<T> &out, size_t count)
+    {
+        FILE *f = std::fopen(path.c_str(), "rb");
+        if (!f) return false;
+        out.resize(count);
+        const size_t got = count ? std::fread(out.data(), sizeof(T), count, f) : 0;
+        const bool at_end = std::fgetc(f) == EOF;
+        std::fclose(f);
+        return got == count && at_end;
+    }
+    template <class T>
+    static void write_file(const std::string &path, const std::vector<T> &v)
+    {
+        FILE *f = std::fopen(path.c_str(), "wb");
+        if (!f) throw HyperBallError(HB_EINVAL, "cannot write " + path);
+        const size_t put = v.empty() ? 0 : std::fwrite(v.data(), sizeof(T), v.size(), f);
+        std::fclose(f);
+        if (put != v.size()) throw HyperBallError(HB_EINVAL, "short write to " + path);
+    }
+    const std::vector<double> &saved(int kind, bool normalize, const std::vector<double> &v) const
+    {
+        if (!cache_dir_.empty()) write_file(centrality_file(kind, normalize), v);
+        return v;
+    }
+    const std::vector<double> &search(int kind, bool normalize, std::vector<double> &slot)
+    {
+        std::vector<double> v;
+        if (cache_dir_.empty() || !read_file(centrality_file(kind, normalize), v, (size_t)g_.nodes))
+            throw HyperBallError(HB_ESTATE, "error no such cached centrality");
+        return slot = std::move(v);
+    }
+
     std::vector<double> centrality(int kind, bool normalize) const
     {
         std::vector<double> c(g_.nodes);
@@ -188,6 +299,7 @@ private:
     CsrView g_{};
     std::unique_ptr<hb_stats> stats_;
     std::vector<double> closeness_[2], harmonic_[2], lin_;
+    std::string cache_dir_, gid_;  // set when opened through get_or_compute
 };
 
 using HyperBall4 = HyperBall<4>;

@@ -33,7 +33,7 @@ EXPORTS = (
     "hb_mlp_get_params", "hb_mlp_gradient", "hb_mlp_train", "hb_mlp_predict", "hb_surrogate",
     "hb_walk_features", "hb_surrogate_walks",
     "hb_exact_bfs", "hb_exact_centrality", "hb_eval_ranks", "hb_eval_metrics", "hb_eval_metrics_noalloc", "hb_bias_windows", "hb_set_layout",
-    "hb_stage_mtx", "hb_stage_coo", "hb_read_csr", "hb_run_many", "hb_result_info",
+    "hb_stage_mtx", "hb_stage_coo", "hb_read_csr", "hb_run_many", "hb_result_info", "hb_restore",
 )
 
 
@@ -177,6 +177,7 @@ def load() -> C.CDLL:
         "hb_eval_metrics": (C.c_int, [C.c_int, f64p, f64p, C.c_uint64, C.c_double, f64p]),
         "hb_eval_metrics_noalloc": (C.c_int, [C.c_int, f64p, f64p, C.c_uint64, C.c_double, f64p]),
         "hb_bias_windows": (C.c_int, [C.c_int, f64p, C.POINTER(C.c_float), C.POINTER(C.c_uint8)]),
+        "hb_restore": (C.c_int, [vp, C.c_int, u8p, f64p, f64p]),
         "hb_set_layout": (C.c_int, [vp, C.c_int]),
         "hb_stage_mtx": (C.c_int, [vp, C.c_char_p, C.c_uint64, C.POINTER(HbMtxInfo)]),
         "hb_stage_coo": (C.c_int, [vp, C.c_uint64, u32p, u32p, C.c_uint64, C.c_uint32]),

@@ -10,7 +10,11 @@ centralities `hyperballcloseness_<seq>_<id>.mmap`, `hyperballharmonic_<seq>_<id>
 (seq 1: normalize=false, seq 2: normalize=true) and `hyperballlin_0_<id>.mmap`
 (src/centralities/hyper_ball.rs:203-246).  The counters file `hyperball_<id>.mmap`
 holds hyperloglog-rs structs whose byte layout is not available offline, so it is
-not written; the reference's search_cache_* and the accumulator files interoperate.
+neither written nor read: the registers go to a file of this build's own,
+`hyperballregisters_<p>_<id>.mmap` (u8, node-major, the buffer the reference's
+counters file would hold), which the reference never opens.  The accumulator and
+centrality files interoperate; the reference's own get_or_compute (hyper_ball.rs:
+165-201) finds no counters file and recomputes.
 
 MTX input follows the reference's reader (src/graph/cache/parser_mtx/mod.rs:34-162,
 producers.rs:228-234, impl_sort.rs:10-51): 1-based entries, `symmetric` headers
@@ -28,7 +32,7 @@ from .graph import Graph
 SUFFIX = {
     "edges": "edges", "index": "index", "distances": "hyperballdistances",
     "inv_distances": "hyperballinvdistances", "closeness": "hyperballcloseness",
-    "harmonic": "hyperballharmonic", "lin": "hyperballlin",
+    "harmonic": "hyperballharmonic", "lin": "hyperballlin", "registers": "hyperballregisters",
 }
 
 
@@ -62,12 +66,25 @@ def save_graph_cache(g: Graph, cache_dir: str, gid: str) -> None:
     np.asarray(g.neighbours, dtype="<u8").tofile(cache_file(cache_dir, "edges", gid))
 
 
+def write_centrality(cache_dir: str, kind: str, gid: str, normalize: bool, values) -> str:
+    """One centrality file under the reference's name (as its compute_* create it)."""
+    os.makedirs(cache_dir, exist_ok=True)
+    path = cache_file(cache_dir, kind, gid, centrality_seq(kind, normalize))
+    np.asarray(values, "<f8").tofile(path)
+    return path
+
+
 def write_results(hb, cache_dir: str, gid: str) -> list:
     """Writes the accumulators and every centrality `hb` has computed under the
-    reference's file names; returns the paths written."""
+    reference's file names, and the registers under this build's own name; returns
+    the paths written."""
     os.makedirs(cache_dir, exist_ok=True)
     dist, inv = hb._accumulators()
     out = []
+    p = hb.result_info()[0]
+    path = cache_file(cache_dir, "registers", gid, p)
+    hb.registers().tofile(path)
+    out.append(path)
     for kind, arr in (("distances", dist), ("inv_distances", inv)):
         path = cache_file(cache_dir, kind, gid)
         np.asarray(arr, "<f8").tofile(path)
@@ -87,12 +104,13 @@ def write_results(hb, cache_dir: str, gid: str) -> list:
 
 def drop_cache(hb, cache_dir: str, gid: str) -> None:
     """HyperBallInner::drop_cache (hyper_ball.rs:564-593) over the files write_results
-    wrote: the accumulators, then the closeness files -- the reference names seq 1
-    for both normalisation slots (:575-579), so with both cached the second removal
-    fails -- then the harmonic files, whose removal is conditioned on the *closeness*
-    slots (:582-586), then Lin's.  A missing file raises FileNotFoundError, as the
-    reference's `?` returns Err.  The counters file is not written here (its
-    hyperloglog-rs layout is unavailable), so it is not removed."""
+    wrote: the registers (in place of the reference's counters file), the
+    accumulators, then the closeness files -- the reference names seq 1 for both
+    normalisation slots (:575-579), so with both cached the second removal fails --
+    then the harmonic files, whose removal is conditioned on the *closeness* slots
+    (:582-586), then Lin's.  A missing file raises FileNotFoundError, as the
+    reference's `?` returns Err."""
+    os.remove(cache_file(cache_dir, "registers", gid, hb.result_info()[0]))
     for kind in ("distances", "inv_distances"):
         os.remove(cache_file(cache_dir, kind, gid))
     for i in (0, 1):

@@ -1167,6 +1167,77 @@ extern "C" int hb_result_info(const hb_ctx *c, int *p, int *iterations, int *sna
     return HB_OK;
 }
 
+// A completed run rebuilt from saved results (the reference's get_or_compute reopens
+// its counters and accumulator files, hyper_ball.rs:165-201): the registers go to both
+// buffers (the reference's counters file is the one it reads), the accumulators as
+// they are, each permuted into the device's node order; the estimator tables are
+// staged as hb_run stages them.  The restored run reports 0 iterations, as the
+// reference's restored struct does (iters: 0).
+extern "C" int hb_restore(hb_ctx *c, int p, const uint8_t *regs, const double *dist, const double *inv)
+{
+    if (!c || (!regs && c->n) || (!dist && c->n) || (!inv && c->n)) return fail(HB_EINVAL, "hb_restore: NULL argument");
+    if (p < 4 || p > 16) return fail(HB_EINVAL, "hb_restore: precision p=%d outside 4..16", p);
+    if (!c->off.ptr) return fail(HB_ESTATE, "hb_restore: no graph staged");
+    if (!set_device(c)) return fail(HB_EHIP, "hipSetDevice failed");
+    c->ran = false;
+    const uint32_t n = (uint32_t)c->n, m = 1u << p;
+    if (int rc = ensure_layout(c)) return rc;
+    const size_t rbytes = (size_t)n << p;
+    HB_TRY(c->regs0.ensure(std::max<size_t>(rbytes, 1)));
+    HB_TRY(c->regs1.ensure(std::max<size_t>(rbytes, 1)));
+    HB_TRY(c->dist.ensure(std::max<uint32_t>(n, 1)));
+    HB_TRY(c->inv.ensure(std::max<uint32_t>(n, 1)));
+    HB_TRY(c->cent.ensure(std::max<uint32_t>(n, 1)));
+    HB_TRY(c->lc.ensure(m + 1));
+    HB_TRY(c->bias.ensure(1));
+    std::vector<float> lc(m + 1, 0.0f);
+    for (uint32_t z = 1; z <= m; ++z) lc[z] = (float)((double)m * std::log((double)m / (double)z));
+    HB_TRY(hipMemcpyAsync(c->lc.ptr, lc.data(), (m + 1) * 4, hipMemcpyHostToDevice, c->stream));
+    HB_TRY(hipMemcpyAsync(c->bias.ptr, hb::bias_table(p), sizeof(HbBiasTable), hipMemcpyHostToDevice, c->stream));
+    if (n) {
+        if (c->relabelled()) {  // device row i = the caller's row perm[i]
+            HB_TRY(hipMemcpyAsync(c->regs1.ptr, regs, rbytes, hipMemcpyHostToDevice, c->stream));
+            HB_TRY(hb::unpermute_rows(c->regs1.ptr, c->regs0.ptr, c->perm.ptr, n, m, c->stream));
+            HB_TRY(hipMemcpyAsync(c->cent.ptr, dist, (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
+            HB_TRY(hb::unpermute_rows(c->cent.ptr, c->dist.ptr, c->perm.ptr, n, 8, c->stream));
+            HB_TRY(hipMemcpyAsync(c->cent.ptr, inv, (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
+            HB_TRY(hb::unpermute_rows(c->cent.ptr, c->inv.ptr, c->perm.ptr, n, 8, c->stream));
+        } else {
+            HB_TRY(hipMemcpyAsync(c->regs0.ptr, regs, rbytes, hipMemcpyHostToDevice, c->stream));
+            HB_TRY(hipMemcpyAsync(c->dist.ptr, dist, (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
+            HB_TRY(hipMemcpyAsync(c->inv.ptr, inv, (size_t)n * 8, hipMemcpyHostToDevice, c->stream));
+        }
+        HB_TRY(hipMemcpyAsync(c->regs1.ptr, c->regs0.ptr, rbytes, hipMemcpyDeviceToDevice, c->stream));
+    }
+    HB_TRY(hipStreamSynchronize(c->stream));
+    HbDev &d = c->dev;
+    d = HbDev{};
+    d.off = c->run_off();
+    d.nbr = c->run_nbr();
+    d.orig = c->run_orig();
+    d.node_id = c->run_node_id();
+    d.regs0 = c->regs0.ptr;
+    d.regs1 = c->regs1.ptr;
+    d.dist = c->dist.ptr;
+    d.inv = c->inv.ptr;
+    d.lc = c->lc.ptr;
+    d.bias = c->bias.ptr;
+    d.n = n;
+    d.p = (uint32_t)p;
+    d.am = alpha_f32(m) * ((float)m * (float)m);
+    d.five_m = 5.0f * (float)m;
+    d.lc_thr = kLcThreshold[p];
+    c->h_st = HbIterState{};
+    c->h_st.parity = 1;  // HB_REGS_LATEST selects regs0 as well
+    c->h_st.converged = 1;
+    c->h_rec.assign(1, HbIterRec{});
+    c->p = p;
+    c->run_snap_k = 0;
+    c->ran = true;
+    g_err.clear();
+    return HB_OK;
+}
+
 extern "C" int hb_set_snapshots(hb_ctx *c, int k)
 {
     if (!c) return fail(HB_EINVAL, "hb_set_snapshots: ctx is NULL");

@@ -9,6 +9,7 @@ host buffers and keeps the reference's caching rules for centralities.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from typing import Optional
 
 import numpy as np
@@ -44,6 +45,7 @@ class HyperBall:
         self._harmonic = [None, None]
         self._lin = None
         self.stats = None
+        self._cache = None  # (cache_dir, graph id) when opened through get_or_compute
         try:
             self.set_layout(layout)
             if schedule is not None:
@@ -69,6 +71,53 @@ class HyperBall:
                       p: Optional[int] = None, **kw) -> "HyperBall":
         del max_depth, threads  # no effect on results in the reference either
         return cls(g, cls.PRECISION if p is None else p, **kw)
+
+    @classmethod
+    def get_or_compute(cls, g: Graph, cache_dir: str, gid: str, p: Optional[int] = None, **kw) -> "HyperBall":
+        """HyperBallInner::get_or_compute (hyper_ball.rs:165-201).  When cache_dir holds
+        the registers and both accumulators of graph `gid` (hyperballregisters_<p>_<id>,
+        this build's own file -- the reference's counters file holds hyperloglog-rs
+        structs whose layout is not available, so it is neither written nor read --
+        and hyperballdistances_<id> / hyperballinvdistances_<id>, the reference's
+        names), the run is rebuilt from them on the device and every cached
+        centrality is searched for, as the reference does; the rebuilt run reports 0
+        iterations, as the reference's does.  Otherwise HyperBall runs and its
+        registers and accumulators are saved there (the reference's run creates its
+        files).  Either way the object keeps the cache: compute_* write their files."""
+        from . import cache
+        p = cls.PRECISION if p is None else p
+        n = g.size()
+        files = [cache.cache_file(cache_dir, "registers", gid, p), cache.cache_file(cache_dir, "distances", gid),
+                 cache.cache_file(cache_dir, "inv_distances", gid)]
+        if all(os.path.exists(f) for f in files):
+            regs = np.fromfile(files[0], np.uint8)
+            dist = np.fromfile(files[1], "<f8")
+            inv = np.fromfile(files[2], "<f8")
+            if regs.size == n << p and dist.size == n and inv.size == n:
+                h = cls(g, p, _run=False, **kw)
+                try:
+                    _lib.check(h._L.hb_restore(h._ctx, p, _lib.ptr(regs, C.c_uint8), _lib.ptr(dist, C.c_double),
+                                               _lib.ptr(inv, C.c_double)))
+                    h._ran(None)
+                    h._cache = (cache_dir, gid)
+                    for norm in (False, True):
+                        for search in (h.search_cache_closeness_centrality, h.search_cache_harmonic_centrality):
+                            try:
+                                search(norm)
+                            except LookupError:
+                                pass
+                    try:
+                        h.search_cache_lins_centrality()
+                    except LookupError:
+                        pass
+                except Exception:
+                    h.close()
+                    raise
+                return h
+        h = cls(g, p, **kw)
+        h._cache = (cache_dir, gid)
+        cache.write_results(h, cache_dir, gid)
+        return h
 
     @classmethod
     def from_mtx(cls, text, p: Optional[int] = None, **kw) -> "HyperBall":
@@ -196,20 +245,62 @@ class HyperBall:
         _lib.check(self._L.hb_centrality(self._ctx, kind, 1 if normalize else 0, _lib.ptr(out, C.c_double)))
         return out
 
+    def _saved(self, kind: str, normalize: bool, values: np.ndarray) -> np.ndarray:
+        """A computed centrality also goes to its cache file when the object has a cache
+        (the reference's compute_* write into their file, hyper_ball.rs:441-445)."""
+        if self._cache is not None:
+            from . import cache
+            cache.write_centrality(self._cache[0], kind, self._cache[1], normalize, values)
+        return values
+
     # hyper_ball.rs:434-477
     def compute_closeness_centrality(self, normalize: bool) -> np.ndarray:
-        self._closeness[int(normalize)] = self._centrality(_lib.HB_CLOSENESS, normalize)
+        self._closeness[int(normalize)] = self._saved("closeness", normalize,
+                                                      self._centrality(_lib.HB_CLOSENESS, normalize))
         return self._closeness[int(normalize)]
 
     # hyper_ball.rs:487-529
     def compute_harmonic_centrality(self, normalize: bool) -> np.ndarray:
-        self._harmonic[int(normalize)] = self._centrality(_lib.HB_HARMONIC, normalize)
+        self._harmonic[int(normalize)] = self._saved("harmonic", normalize,
+                                                     self._centrality(_lib.HB_HARMONIC, normalize))
         return self._harmonic[int(normalize)]
 
     # hyper_ball.rs:532-555
     def compute_lins_centrality(self) -> np.ndarray:
-        self._lin = self._centrality(_lib.HB_LIN, False)
+        self._lin = self._saved("lin", False, self._centrality(_lib.HB_LIN, False))
         return self._lin
+
+    # hyper_ball.rs:302-317, :359-374, :405-415: the cached file into its slot, or Err
+    def _search(self, kind: str, normalize: bool) -> Optional[np.ndarray]:
+        if self._cache is None:
+            return None
+        from . import cache
+        path = cache.cache_file(self._cache[0], kind, self._cache[1], cache.centrality_seq(kind, normalize))
+        if not os.path.exists(path):
+            return None
+        v = np.fromfile(path, "<f8")
+        return v if v.size == self.g.size() else None
+
+    def search_cache_closeness_centrality(self, normalization: bool) -> np.ndarray:
+        v = self._search("closeness", normalization)
+        if v is None:
+            raise LookupError("error no such cached centrality")
+        self._closeness[int(normalization)] = v
+        return v
+
+    def search_cache_harmonic_centrality(self, normalization: bool) -> np.ndarray:
+        v = self._search("harmonic", normalization)
+        if v is None:
+            raise LookupError("error no such cached centrality")
+        self._harmonic[int(normalization)] = v
+        return v
+
+    def search_cache_lins_centrality(self) -> np.ndarray:
+        v = self._search("lin", False)
+        if v is None:
+            raise LookupError("error no such cached centrality")
+        self._lin = v
+        return v
 
     # hyper_ball.rs:290-299, :347-356, :397-402: Err when not computed
     def get_closeness_centrality(self, normalization: bool) -> np.ndarray:

@@ -116,10 +116,10 @@ def test_hyperball_rejects_bad_precision():
         HyperBall(g, 17)
 
 
-def _build_cpp_mirror(tmp_path):
+def _build_cpp_mirror(tmp_path, name="hyperball_cpp_mirror"):
     import subprocess
-    src = os.path.join(ROOT, "tests", "cpp", "hyperball_cpp_mirror.cpp")
-    exe = str(tmp_path / "hyperball_cpp_mirror")
+    src = os.path.join(ROOT, "tests", "cpp", name + ".cpp")
+    exe = str(tmp_path / name)
     cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-Werror", src,
            "-L" + os.path.join(ROOT, "sparkx_amd"), "-lsparkx_hb", "-o", exe]
     res = subprocess.run(cmd, capture_output=True, text=True)
@@ -128,8 +128,10 @@ def _build_cpp_mirror(tmp_path):
 
 
 def test_cpp_mirror_compiles_and_links(tmp_path):
-    """The header-only C++ mirror of HyperBallInner builds against the C ABI."""
+    """The header-only C++ mirror of HyperBallInner builds against the C ABI (and its
+    get_or_compute client too)."""
     assert os.path.exists(_build_cpp_mirror(tmp_path))
+    assert os.path.exists(_build_cpp_mirror(tmp_path, "hyperball_cpp_cache"))
 
 
 def _write_dbg_mtx(path, sigma, k):
@@ -226,17 +228,20 @@ def test_drop_cache_follows_the_reference_quirks(tmp_path):
 
     for k in ("distances", "inv_distances"):
         touch(k)
+    touch("registers", 7)
     touch("closeness", 1)
     touch("harmonic", 1)
     touch("lin", 0)
-    hb = SimpleNamespace(_closeness=[1, None], _harmonic=[None, None], _lin=1)
+    info = lambda: (7, 0, 0)  # noqa: E731  (p of the run: the registers file's seq)
+    hb = SimpleNamespace(_closeness=[1, None], _harmonic=[None, None], _lin=1, result_info=info)
     cache.drop_cache(hb, d, gid)
     assert os.listdir(d) == []
     for k in ("distances", "inv_distances"):
         touch(k)
+    touch("registers", 7)
     touch("closeness", 1)
     touch("closeness", 2)
-    hb = SimpleNamespace(_closeness=[1, 1], _harmonic=[None, None], _lin=None)
+    hb = SimpleNamespace(_closeness=[1, 1], _harmonic=[None, None], _lin=None, result_info=info)
     with pytest.raises(FileNotFoundError):
         cache.drop_cache(hb, d, gid)  # the second slot names seq 1 again
     assert os.path.exists(cache.cache_file(d, "closeness", gid, 2))

@@ -552,11 +552,75 @@ def test_results_written_under_reference_cache_names(hb, oracle, tmp_path):
         c = h.compute_closeness_centrality(True)
         hh = h.compute_harmonic_centrality(False)
         paths = cache.write_results(h, str(tmp_path), "g1")
-    assert len(paths) == 4
+    assert len(paths) == 5
+    regs = np.fromfile(cache.cache_file(str(tmp_path), "registers", "g1", 8), np.uint8)
+    assert np.array_equal(regs, ref.regs_ref)
     assert np.array_equal(cache.read_result(str(tmp_path), "distances", "g1"), ref.dist)
     assert np.array_equal(cache.read_result(str(tmp_path), "inv_distances", "g1"), ref.inv)
     assert np.array_equal(cache.read_result(str(tmp_path), "closeness", "g1", True), c)
     assert np.array_equal(cache.read_result(str(tmp_path), "harmonic", "g1", False), hh)
+
+
+@pytest.mark.parametrize("layout", ["auto", "input"])
+def test_get_or_compute_reopens_cached_results(hb, oracle, tmp_path, layout):
+    """get_or_compute (hyper_ball.rs:165-201): the first call runs and saves registers
+    and accumulators; computed centralities go to their files; the second call
+    rebuilds the run from the files (0 iterations, as the reference's restored struct)
+    and finds the cached centralities; every reader and epilogue of the reopened run
+    gives the oracle's values."""
+    g = hb.Graph.erdos_renyi(3000, 9000, seed=15)
+    p = 8
+    ref = _oracle_run(oracle, g, p)
+    est = oracle.estimates(p, ref.regs_ref)
+    d = str(tmp_path)
+    with hb.HyperBall.get_or_compute(g, d, "g3", p=p, layout=layout) as h:
+        assert h.get_iters() == ref.iterations
+        c1 = h.compute_closeness_centrality(True)
+        l1 = h.compute_lins_centrality()
+    with hb.HyperBall.get_or_compute(g, d, "g3", p=p, layout=layout) as h:
+        assert h.get_iters() == 0
+        assert np.array_equal(h.get_closeness_centrality(True), c1)
+        assert np.array_equal(h.get_lins_centrality(), l1)
+        with pytest.raises(LookupError):
+            h.get_hamonic_centrality(False)
+        assert np.array_equal(h.registers().ravel(), ref.regs_ref)
+        assert np.array_equal(h.estimates(), est)
+        dist, inv = h._accumulators()
+        assert np.array_equal(dist, ref.dist) and np.array_equal(inv, ref.inv)
+        for kind, norm, fn in ((oracle.HARMONIC, True, h.compute_harmonic_centrality),
+                               (oracle.CLOSENESS, False, h.compute_closeness_centrality)):
+            assert np.array_equal(fn(norm), oracle.centrality(kind, norm, est, ref.dist, ref.inv))
+    with hb.HyperBall.get_or_compute(g, d, "g3", p=p, layout=layout) as h:
+        assert np.array_equal(h.get_hamonic_centrality(True),  # written by the second object
+                              oracle.centrality(oracle.HARMONIC, True, est, ref.dist, ref.inv))
+    # the same files for another precision are not there: that one runs
+    with hb.HyperBall.get_or_compute(g, d, "g3", p=6, layout=layout) as h:
+        assert h.get_iters() == _oracle_run(oracle, g, 6).iterations
+
+
+def test_cpp_mirror_get_or_compute(hb, oracle, tmp_path):
+    """The C++ mirror's get_or_compute against a cache directory: run and save, then
+    reopen; the reopened run's accumulators, cached closeness and a new Lin epilogue
+    are the oracle's."""
+    import subprocess
+    from tests.test_cpu_host import _build_cpp_mirror
+    exe = _build_cpp_mirror(tmp_path, "hyperball_cpp_cache")
+    cache_dir = tmp_path / "cache"
+    cache_dir.mkdir()
+    env = dict(os.environ, LD_LIBRARY_PATH=os.path.join(ROOT, "sparkx_amd"))
+    res = subprocess.run([exe, str(cache_dir)], capture_output=True, text=True, env=env, timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    lines = res.stdout.split("\n")
+    off = np.array([0, 1, 3, 5, 7, 8, 10, 12, 14], np.uint64)
+    nb = np.array([1, 0, 2, 1, 3, 2, 4, 3, 6, 7, 5, 7, 5, 6], np.uint32)
+    ref = oracle.hyperball(off, nb, 6)
+    est = oracle.estimates(6, ref.regs_ref)
+    c = oracle.centrality(oracle.CLOSENESS, True, est, ref.dist, ref.inv)
+    lin = oracle.centrality(oracle.LIN, False, est, ref.dist, ref.inv)
+    assert lines[0] == f"first iters {ref.iterations}" and lines[1] == "second iters 0"
+    for u in range(8):
+        _, uu, d, c1, c2, ll = lines[2 + u].split()
+        assert (float(d), float(c1), float(c2), float(ll)) == (ref.dist[u], c[u], c[u], lin[u])
 
 
 def _layout_graph(hb, name):
