@@ -185,7 +185,6 @@ struct hb_ctx {
     bool sym = false;              // staged CSR is symmetric (arc multiset fingerprints)
     DevBuf<HbIterState> st;
     DevBuf<HbIterRec> rec;
-    uint32_t rec_cap = 0;
     hb::HbSchedule sched;
     HbDev dev{};
     HbIterState h_st{};
@@ -714,10 +713,11 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
     HB_TRY(c->bits0.ensure(words));
     HB_TRY(c->bits1.ensure(words));
     HB_TRY(c->st.ensure(1));
-    const uint32_t cap = max_iters > 0 ? (uint32_t)max_iters + 2
-                                       : (uint32_t)std::min<uint64_t>(c->n + 3, 1u << 16);
-    HB_TRY(c->rec.ensure(cap));
-    c->rec_cap = cap;
+    // No iteration cap: a run ends by the reference's rule or at max_iters.  A graph
+    // cannot change a register after its diameter, so n + 2 iterations bound a correct
+    // run; the host loop stops there with an error rather than going on.
+    const uint64_t bound = max_iters > 0 ? (uint64_t)max_iters : (uint64_t)c->n + 2;
+    HB_TRY(c->rec.ensure(HB_REC_RING));
     HB_TRY(c->lc.ensure(m + 1));
     HB_TRY(c->bias.ensure(1));
     if (c->snap_k) HB_TRY(c->snaps.ensure((size_t)c->snap_k * c->n));
@@ -786,7 +786,6 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
     st0.t = 1.0;
     st0.inv_t = 1.0;
     HB_TRY(hipMemcpyAsync(c->st.ptr, &st0, sizeof st0, hipMemcpyHostToDevice, c->stream));
-    HB_TRY(hipMemsetAsync(c->rec.ptr, 0, (size_t)cap * sizeof(HbIterRec), c->stream));
     HB_TRY(hb::launch_fill(c->bits0.ptr, words, 0u, c->stream));
     HB_TRY(hb::launch_fill(c->bits1.ptr, words, 0u, c->stream));
 
@@ -834,14 +833,20 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
             }
         }
         // iterations: launches after convergence return at once (done flag), so a
-        // batch can run ahead of the host's check
-        uint32_t launched = 0;
+        // batch can run ahead of the host's check; each check drains the finished
+        // iterations' records from the device ring into c->h_rec (index t)
+        uint64_t launched = 0;
         HbIterState hs{};
+        uint32_t drained = 0;
+        HbIterRec ring[HB_REC_RING];
+        c->h_rec.assign(1, HbIterRec{});
         for (;;) {
             const uint32_t batch = timing ? 1u : (launched < 8 ? 2u : 8u);
-            const uint32_t room = cap - 2 - launched;
-            if (room == 0) { rc = fail(HB_ERANGE, "hb_run: more than %u iterations", cap - 2); break; }
-            const uint32_t k = std::min(batch, room);
+            if (launched >= bound + 16) {
+                rc = fail(HB_ESTATE, "hb_run: still changing after %llu iterations", (unsigned long long)launched);
+                break;
+            }
+            const uint32_t k = batch;
             hipEvent_t ib = nullptr, ie = nullptr;
             if (timing) {
                 (void)hipEventCreate(&ib);
@@ -859,10 +864,14 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
             }
             if (rc != HB_OK) break;
             if (hipMemcpyAsync(&hs, c->st.ptr, sizeof hs, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+                hipMemcpyAsync(ring, c->rec.ptr, sizeof ring, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
                 hipStreamSynchronize(c->stream) != hipSuccess) {
                 rc = fail(HB_EHIP, "iteration failed: %s", hipGetErrorString(hipGetLastError()));
                 break;
             }
+            const uint32_t finished = hs.done ? hs.executed : hs.iter - 1;
+            for (uint32_t t = drained + 1; t <= finished; ++t) c->h_rec.push_back(ring[t % HB_REC_RING]);
+            drained = finished;
             if (timing) {
                 float ms = 0.0f;
                 (void)hipEventElapsedTime(&ms, ib, ie);
@@ -910,8 +919,6 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
     for (uint32_t t = executed + 1; t <= c->snap_k; ++t)
         HB_TRY(hipMemcpyAsync(c->snaps.ptr + (size_t)(t - 1) * c->n, c->est.ptr, c->n * 4, hipMemcpyDeviceToDevice,
                               c->stream));
-    c->h_rec.assign(executed + 1, HbIterRec{});
-    HB_TRY(hipMemcpy(c->h_rec.data(), c->rec.ptr, (executed + 1) * sizeof(HbIterRec), hipMemcpyDeviceToHost));
     c->p = p;
     c->run_snap_k = c->snap_k;
     c->ran = true;

@@ -36,6 +36,11 @@ struct HbIterState {
 #define HB_SLOT_CHUNK 3
 #define HB_SLOT_MERGE 4
 
+// Per-iteration records live in a ring of HB_REC_RING entries (iteration t in slot
+// t % HB_REC_RING, cleared when t begins); the host drains the finished ones at every
+// check, which it makes at most 8 launches apart, so a run has no iteration cap.
+#define HB_REC_RING 64
+
 struct HbIterRec {  // per-iteration counters, per kernel slot
     unsigned long long inc[HB_NSLOT];    // nodes whose estimate increased
     unsigned long long arcs[HB_NSLOT];   // neighbour sketches gathered
@@ -60,7 +65,7 @@ struct HbDev {
     double *dist, *inv;    // accumulators
     uint32_t *bits0, *bits1;  // changed-node bitmaps (bit set: registers changed)
     HbIterState *st;
-    HbIterRec *rec;        // indexed by t
+    HbIterRec *rec;        // ring: iteration t in rec[t % HB_REC_RING]
     const float *lc;       // lc[z] = (float)(m ln(m/z)), z = 1..m
     const HbBiasTable *bias;  // HLL++ bias windows for this p
     float *snaps;          // estimates after iterations 1..snap_k, iteration-major

@@ -117,7 +117,11 @@ __global__ __launch_bounds__(kBlock) void k_begin(HbDev d, uint32_t words)
     uint32_t *bw = (d.st->iter & 1u) ? d.bits1 : d.bits0;
     const bool push = d.st->push != 0;
     if (push && blockIdx.x == 0 && threadIdx.x < 3) d.dyn_count[threadIdx.x] = 0u;
-    if (push && blockIdx.x == 0 && threadIdx.x == 0) d.rec[d.st->iter].push = 1ull;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {  // this iteration's ring record, cleared
+        HbIterRec *r = d.rec + d.st->iter % HB_REC_RING;
+        *r = HbIterRec{};
+        if (push) r->push = 1ull;
+    }
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += gridDim.x * blockDim.x) {
         bw[i] = 0u;
         if (push) d.active[i] = 0u;
@@ -319,7 +323,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(P >= 7 ?
         for (uint32_t team = blockIdx.x * (kBlock / T) + threadIdx.x / T; team < count; team += teams_per_grid)
             union_node<P, T, false>(v, tb, list[team], tl, c);
     }
-    block_add5(c.inc, c.arcs, c.nodes, c.chg, c.gathered, d.rec + it, slot_id);
+    block_add5(c.inc, c.arcs, c.nodes, c.chg, c.gathered, d.rec + it % HB_REC_RING, slot_id);
 }
 
 // ---------------------------------------------------------------------------
@@ -372,7 +376,7 @@ __global__ __launch_bounds__(kBlock) void k_chunk(HbDev d, const uint32_t *__res
         if (slot == 0) *(uint4 *)(d.scratch + ((uint64_t)task << P) + cl * 16) = acc;
         if (tl == 0) c_arcs += len;
     }
-    block_add5(0, c_arcs, 0, 0, c_gathered, d.rec + it, HB_SLOT_CHUNK);
+    block_add5(0, c_arcs, 0, 0, c_gathered, d.rec + it % HB_REC_RING, HB_SLOT_CHUNK);
 }
 
 // hubs, phase 2: one wavefront per hub merges its own registers with the
@@ -430,7 +434,7 @@ __global__ __launch_bounds__(kBlock) void k_merge(HbDev d, const uint32_t *__res
             c_nodes += 1;
         }
     }
-    block_add5(c_inc, 0, c_nodes, c_chg, 0, d.rec + it, HB_SLOT_MERGE);
+    block_add5(c_inc, 0, c_nodes, c_chg, 0, d.rec + it % HB_REC_RING, HB_SLOT_MERGE);
 }
 
 // ---------------------------------------------------------------------------
@@ -581,7 +585,7 @@ __global__ __launch_bounds__(kBlock) void k_union_wide(HbDev d, const uint32_t *
             c_nodes += 1;
         }
     }
-    block_add5(c_inc, c_arcs, c_nodes, c_chg, c_arcs, d.rec + it, slot_id);  // gathers every row
+    block_add5(c_inc, c_arcs, c_nodes, c_chg, c_arcs, d.rec + it % HB_REC_RING, slot_id);  // gathers every row
 }
 
 template <int P>
@@ -638,7 +642,7 @@ __global__ __launch_bounds__(kBlock) void k_nf(HbDev d)
         lo = nlo;
     }
     if ((threadIdx.x & (HB_WAVE - 1)) == 0) {
-        HbIterRec *r = d.rec + st->iter;
+        HbIterRec *r = d.rec + st->iter % HB_REC_RING;
         const unsigned long long old = atomicAdd(&r->nf_lo, lo);
         atomicAdd(&r->nf_hi, hi + ((old + lo) < old ? 1ull : 0ull));
     }
@@ -661,8 +665,8 @@ __global__ void k_finalize(HbDev d)
     const uint32_t it = st->iter;
     unsigned long long inc = 0, chg = 0;
     for (int k = 0; k < HB_NSLOT; ++k) {
-        inc += d.rec[it].inc[k];
-        chg += d.rec[it].chg[k];
+        inc += d.rec[it % HB_REC_RING].inc[k];
+        chg += d.rec[it % HB_REC_RING].chg[k];
     }
     if (inc == 0 || (st->max_iters && it >= st->max_iters)) {
         st->done = 1;

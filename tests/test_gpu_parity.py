@@ -208,6 +208,53 @@ def test_edge_case_graphs(hb, oracle, name):
             _assert_same(h, ref, oracle, p, name)
 
 
+def _ball_sketches(oracle, n, p, t):
+    """u8 registers of the HLL sketch of every node's ball [u - t, u + t] on the path
+    0 - 1 - ... - n-1: each node inserts its own id; a sketch is the register-wise max
+    of its members' insertions (sliding-window maxima from a sparse table)."""
+    m = 1 << p
+    one = np.zeros((n, m), np.uint8)
+    for u in range(n):
+        idx, rank = oracle.index_rank(p, u)
+        one[u, idx] = rank
+    table = [one]
+    while (1 << len(table)) <= 2 * t + 1:
+        prev, h = table[-1], 1 << (len(table) - 1)
+        table.append(np.maximum(prev[:-h], prev[h:]) if prev.shape[0] > h else prev[:0])
+    lo = np.maximum(np.arange(n) - t, 0)
+    hi = np.minimum(np.arange(n) + t, n - 1)
+    k = np.floor(np.log2(hi - lo + 1)).astype(int)
+    out = np.empty((n, m), np.uint8)
+    for lev in np.unique(k):
+        rows = np.flatnonzero(k == lev)
+        tab = table[lev]
+        out[rows] = np.maximum(tab[lo[rows]], tab[hi[rows] - (1 << lev) + 1])
+    return out
+
+
+def test_run_longer_than_65534_iterations(hb, oracle):
+    """No iteration cap (the reference has none, hyper_ball.rs:710-712): a path of
+    70,000 nodes runs until the reference's stopping rule, past 65,534 iterations, with
+    the per-iteration records drained from the device ring along the way.  After t
+    iterations node u's registers are the sketch of its exact ball [u - t, u + t]:
+    the last iteration's output buffer holds that for t = iterations, the reference's
+    counters buffer for the iteration count rounded down to even."""
+    from sparkx_amd import _lib
+    n, p = 70_000, 4
+    off, nb = _csr(n, [(i, i + 1) for i in range(n - 1)] + [(i + 1, i) for i in range(n - 1)])
+    g = hb.Graph(off, nb, "path70k")
+    say("long path", n, p)
+    with hb.HyperBall(g, p) as h:
+        it = h.stats.iterations
+        say("long path iterations", it)
+        assert it > 65_534
+        last = h.registers(_lib.HB_REGS_LATEST)
+        refbuf = h.registers(_lib.HB_REGS_REFERENCE)
+        assert np.array_equal(last, _ball_sketches(oracle, n, p, it))
+        assert np.array_equal(refbuf, last if it % 2 == 0 else _ball_sketches(oracle, n, p, it - 1))
+        assert np.array_equal(h.estimates(), oracle.estimates(p, refbuf.ravel()))
+
+
 def test_empty_graph(hb):
     g = hb.Graph(np.zeros(1, np.uint64), np.zeros(0, np.uint64))
     say("empty graph")
