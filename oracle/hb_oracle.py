@@ -28,7 +28,8 @@ from dataclasses import dataclass
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "libhb_oracle.so")
+# HB_ORACLE_LIB selects another build of the oracle (the Makefile's sanitizer builds)
+_LIB_PATH = os.environ.get("HB_ORACLE_LIB") or os.path.join(_HERE, "libhb_oracle.so")
 
 
 class Variant(C.Structure):

@@ -28,9 +28,34 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose: bool = False) -> str:
+def sanitizer_runtime(kind: str) -> str:
+    """Path of the ROCm clang's shared sanitizer runtime for `kind` (address, thread)."""
+    clang = os.path.join(os.path.dirname(os.path.realpath(hipcc())), "..", "llvm", "bin", "clang")
+    if not os.path.exists(clang):
+        clang = "/opt/rocm/llvm/bin/clang"
+    name = f"libclang_rt.{'asan' if kind == 'address' else 'tsan'}-x86_64.so"
+    res = subprocess.run([clang, "-print-resource-dir"], capture_output=True, text=True, check=True).stdout.strip()
+    rt = subprocess.run([clang, "--print-runtime-dir"], capture_output=True, text=True, check=True).stdout.strip()
+    for d in (rt, os.path.join(res, "lib", "linux")):
+        if os.path.exists(os.path.join(d, name)):
+            return os.path.join(d, name)
+    raise FileNotFoundError(f"{name} not found under {rt} or {res}/lib/linux")
+
+
+def build(verbose: bool = False, sanitize: str | None = None) -> str:
+    """Builds the library; with `sanitize` ("address" or "thread") the host code is
+    instrumented (device code is not; "address" adds UndefinedBehaviorSanitizer) and the result is libsparkx_hb_<sanitize>.so,
+    linked against the ROCm clang's sanitizer runtime (libclang_rt.asan / .tsan) --
+    for CPU test runs under SPARKX_HB_LIB with that runtime in LD_PRELOAD."""
     import tempfile
     compile_flags = [f for f in FLAGS if f != "-shared"]
+    target = LIB
+    if sanitize:
+        if sanitize not in ("address", "thread"):
+            raise ValueError(f"sanitize={sanitize!r}: address or thread")
+        checks = "address,undefined" if sanitize == "address" else sanitize
+        compile_flags += ["-O1", "-g", "-fno-omit-frame-pointer", "-Xarch_host", "-fsanitize=" + checks]
+        target = os.path.join(HERE, f"libsparkx_hb_{sanitize}.so")
     with tempfile.TemporaryDirectory(prefix="sparkx_hb_build_") as tmp:
         objs, procs = [], []
         for src in SOURCES:  # one hipcc per source, in parallel
@@ -47,13 +72,21 @@ def build(verbose: bool = False) -> str:
                 errors.append(f"{src}:\n{out}\n{err}")
         if errors:
             raise RuntimeError("hipcc failed:\n" + "\n".join(errors))
-        link = [hipcc(), *FLAGS, *objs, "-o", LIB + ".tmp"]
+        if sanitize:  # host objects call clang's sanitizer runtime: link against it
+            rocm = os.path.dirname(os.path.dirname(os.path.realpath(hipcc())))
+            rt = sanitizer_runtime(sanitize)
+            link = ["g++", "-shared", "-pthread", *objs, "-L" + os.path.join(rocm, "lib"), "-lamdhip64",
+                    "-Wl,-rpath," + os.path.join(rocm, "lib"), rt, "-Wl,-rpath," + os.path.dirname(rt),
+                    "-o", target + ".tmp"]
+        else:
+            link = [hipcc(), *FLAGS, *objs, "-o", target + ".tmp"]
         res = subprocess.run(link, capture_output=True, text=True)
         if res.returncode != 0:
             raise RuntimeError(f"link failed ({res.returncode}):\n{res.stdout}\n{res.stderr}")
-    os.replace(LIB + ".tmp", LIB)
-    return LIB
+    os.replace(target + ".tmp", target)
+    return target
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    san = next((a.split("=", 1)[1] for a in sys.argv[1:] if a.startswith("--sanitize=")), None)
+    print(build(verbose="-v" in sys.argv, sanitize=san))
