@@ -119,6 +119,12 @@ void hb_destroy(hb_ctx *ctx);
 /* Message of the last failing call on this thread ("" if none). */
 const char *hb_last_error(void);
 
+/* SHA-256 (hex) of the compile flags and the source files this library was built from,
+ * as sparkx_amd/build.py computes it; "unknown" for a build made without it.  No
+ * counterpart in the reference: it lets a caller or a test show which sources a loaded
+ * binary comes from. */
+const char *hb_build_digest(void);
+
 /* Stages a CSR graph into HBM once, through pinned host buffers.  offsets has n+1
  * entries and nbrs has arcs entries, each 8- or 4-byte unsigned (the reference's
  * mmap'd index and edge files are 8-byte).  Neighbour ids must be < n and offsets

@@ -21,7 +21,7 @@ KERNEL_SLOTS = ("union_bin0", "union_bin1", "union_bin2", "hub_chunk", "hub_merg
 
 # every extern "C" symbol the headers declare
 EXPORTS = (
-    "hb_create", "hb_destroy", "hb_last_error", "hb_stage_csr", "hb_run",
+    "hb_create", "hb_destroy", "hb_last_error", "hb_build_digest", "hb_stage_csr", "hb_run",
     "hb_read_registers", "hb_read_estimates", "hb_read_accumulators", "hb_centrality",
     "hb_neighbourhood_function", "hb_set_schedule", "hb_num_nodes", "hb_num_arcs",
     "hb_graph_er", "hb_graph_rmat", "hb_graph_kmer", "hb_graph_nodes", "hb_graph_arcs",
@@ -127,6 +127,7 @@ def load() -> C.CDLL:
         "hb_create": (C.c_int, [C.POINTER(vp), C.c_int]),
         "hb_destroy": (None, [vp]),
         "hb_last_error": (C.c_char_p, []),
+        "hb_build_digest": (C.c_char_p, []),
         "hb_stage_csr": (C.c_int, [vp, C.c_void_p, C.c_int, C.c_uint64, C.c_void_p, C.c_int, C.c_uint64]),
         "hb_run": (C.c_int, [vp, C.c_int, C.c_int, C.c_uint32, C.POINTER(HbStats)]),
         "hb_read_registers": (C.c_int, [vp, C.c_int, u8p]),

@@ -4,6 +4,7 @@ Run: python -m sparkx_amd.build   (also called by __graft_entry__.build()).
 """
 from __future__ import annotations
 
+import hashlib
 import os
 import subprocess
 import sys
@@ -14,6 +15,7 @@ LIB = os.path.join(HERE, "libsparkx_hb.so")
 
 SOURCES = ["hb_kernels.hip", "hb_api.hip", "hb_batch.hip", "hb_mlp.hip", "hb_exact.hip", "hb_eval.hip", "hb_layout.hip", "hb_ingest.hip", "hb_walk.hip", "hb_graphgen.cpp"]
 HEADERS = ["hb_device.h", "hb_kernels.h", "hb_union.h", "hll_bias_table.inc"]
+ABI_HEADERS = ["sparkx_hb.h", "sparkx_hb_graphgen.h"]  # under include/ at the repo root
 
 # -ffp-contract=off: the f64 accumulators and f32 estimates must not fuse a*b+c
 # (the reference never does); packed-fp32 instruction selection is disabled.
@@ -26,6 +28,21 @@ def hipcc() -> str:
         if cand and (os.path.sep not in cand or os.path.exists(cand)):
             return cand
     raise RuntimeError("hipcc not found")
+
+
+def source_digest() -> str:
+    """SHA-256 over the compile flags and every file compiled into the library (sources,
+    kernel headers, the C ABI headers), in a fixed order.  The build passes it to the
+    compiler and the library reports it (hb_build_digest), so a test can show that the
+    binary it loaded was built from the files beside it."""
+    h = hashlib.sha256(" ".join(FLAGS).encode())
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    files = [os.path.join(CSRC, f) for f in (*SOURCES, *HEADERS)] + [os.path.join(inc, f) for f in ABI_HEADERS]
+    for path in files:
+        with open(path, "rb") as f:
+            data = f.read()
+        h.update(os.path.basename(path).encode() + b"\0" + len(data).to_bytes(8, "little") + data)
+    return h.hexdigest()
 
 
 def sanitizer_runtime(kind: str) -> str:
@@ -44,11 +61,12 @@ def sanitizer_runtime(kind: str) -> str:
 
 def build(verbose: bool = False, sanitize: str | None = None) -> str:
     """Builds the library; with `sanitize` ("address" or "thread") the host code is
-    instrumented (device code is not; "address" adds UndefinedBehaviorSanitizer) and the result is libsparkx_hb_<sanitize>.so,
-    linked against the ROCm clang's sanitizer runtime (libclang_rt.asan / .tsan) --
-    for CPU test runs under SPARKX_HB_LIB with that runtime in LD_PRELOAD."""
+    instrumented (device code is not; "address" adds UndefinedBehaviorSanitizer) and the
+    result is libsparkx_hb_<sanitize>.so, linked against the ROCm clang's sanitizer runtime
+    (libclang_rt.asan / .tsan) -- for CPU test runs under SPARKX_HB_LIB with that runtime
+    in LD_PRELOAD."""
     import tempfile
-    compile_flags = [f for f in FLAGS if f != "-shared"]
+    compile_flags = [f for f in FLAGS if f != "-shared"] + [f'-DHB_BUILD_DIGEST="{source_digest()}"']
     target = LIB
     if sanitize:
         if sanitize not in ("address", "thread"):

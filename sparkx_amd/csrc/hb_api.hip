@@ -299,6 +299,11 @@ extern "C" {
 
 const char *hb_last_error(void) { return g_err.c_str(); }
 
+#ifndef HB_BUILD_DIGEST
+#define HB_BUILD_DIGEST "unknown"
+#endif
+const char *hb_build_digest(void) { return HB_BUILD_DIGEST; }
+
 int hb_create(hb_ctx **out, int device_ordinal)
 {
     if (!out) return fail(HB_EINVAL, "hb_create: out is NULL");

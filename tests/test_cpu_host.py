@@ -32,6 +32,12 @@ def test_library_exports_every_declared_symbol():
     assert declared <= exported
 
 
+def test_library_was_built_from_these_sources():
+    """The loaded binary's recorded digest equals the digest of the sources in the tree."""
+    from sparkx_amd import _lib, build
+    assert _lib.load().hb_build_digest().decode() == build.source_digest()
+
+
 def test_create_reports_errors_without_crashing():
     from sparkx_amd import _lib
     L = _lib.load()

@@ -28,6 +28,15 @@ def hb():
     return sparkx_amd
 
 
+def test_tested_library_was_built_from_these_sources(hb):
+    """Run with the GPU tests, on the machine that runs them: the library every test in
+    this run calls was compiled from the source files in this tree."""
+    from sparkx_amd import _lib, build
+    digest = _lib.load().hb_build_digest().decode()
+    say("library", _lib.LIB_PATH, "digest", digest)
+    assert digest == build.source_digest()
+
+
 def _oracle_run(oracle, g, p, **kw):
     return oracle.hyperball(g.offsets.astype(np.uint64), g.neighbours.astype(np.uint32), p,
                             threads=THREADS, **kw)
