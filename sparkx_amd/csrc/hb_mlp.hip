@@ -1,13 +1,18 @@
 // hb_mlp.hip -- the learned surrogate: an MLP 6 -> 64 -> 64 -> 1 with GELU that maps
 // a node's degree, log-degree and log HyperBall estimates N_1..N_4 to its log
-// harmonic centrality.  Matmuls run on BF16 MFMA (v_mfma_f32_16x16x16bf16_1k:
-// 16 x 16 x 16 per wavefront, bf16 operands, f32 accumulation); GELU, biases,
-// loss and the Adam update run in f32.  One wavefront handles a tile of 16
-// samples; activations move between MFMA output and operand layouts through LDS.
+// harmonic centrality.  Matmuls run on BF16 MFMA with f32 accumulation: products
+// over the 64 hidden units on gfx950's v_mfma_f32_16x16x32_bf16 (K = 32 per
+// instruction), products over 16 samples or the 6 (padded) inputs on
+// v_mfma_f32_16x16x16_bf16; GELU, biases, loss and the Adam update run in f32, and
+// every rounding to bf16 is the hardware's round-to-nearest-even convert.  One
+// wavefront handles a tile of 16 samples; activations move between MFMA output and
+// operand layouts through LDS.
 //
-// MFMA 16x16x16 lane layout (lane l, j = 0..3):
-//   A (M x K): A[l % 16][4 (l / 16) + j]     B (K x N): B[4 (l / 16) + j][l % 16]
-//   D (M x N): D[4 (l / 16) + j][l % 16]
+// MFMA lane layout (lane l, g = l / 16): D (M x N, both shapes): D[4 g + j][l % 16],
+// j = 0..3.  An A (M x K) register j of lane l multiplies the B (K x N) register j
+// of the lane with the same g, so any map (g, j) -> k used for both operands is a
+// valid K order; the gathers here use k = 4 g + j (16x16x16, j < 4) and k = 8 g + j
+// (16x16x32, j < 8), with A[l % 16][k] and B[k][l % 16].
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -27,6 +32,7 @@ namespace hb {
 namespace mlp {
 
 typedef short v4s __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 constexpr int kDIn = 6, kDH = 64;
@@ -34,13 +40,9 @@ constexpr int oW1 = 0, oB1 = 384, oW2 = 448, oB2 = 4544, oW3 = 4608, oB3 = 4672,
 constexpr int kBlock = 128;  // two wavefronts, one 16-sample tile each per round
 constexpr int kWaves = kBlock / 64;
 
-__device__ __forceinline__ uint16_t f2bf(float f)
-{
-    uint32_t u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
-    return (uint16_t)(u >> 16);
-}
-__device__ __forceinline__ float bfr(float f) { return __uint_as_float((uint32_t)f2bf(f) << 16); }
+// round to nearest even (v_cvt_pk_bf16_f32); inputs are finite
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+__device__ __forceinline__ float bfr(float f) { return (float)(__bf16)f; }
 
 __device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
 __device__ __forceinline__ float gelu_grad(float x)
@@ -50,6 +52,7 @@ __device__ __forceinline__ float gelu_grad(float x)
 }
 
 __device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+__device__ __forceinline__ v4f mfma32(v8bf a, v8bf b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
 
 struct Shared {
     uint16_t w1[16][kDH];  // bf16, input rows 6..15 zero
@@ -92,16 +95,32 @@ __device__ __forceinline__ v4s opBw(const uint16_t *w, int ld, int kc, int nc, i
     for (int j = 0; j < 4; ++j) r[j] = (short)w[(kc + 4 * (l >> 4) + j) * ld + nc + (l & 15)];
     return r;
 }
-__device__ __forceinline__ v4s opBwT(const uint16_t *w, int ld, int kc, int nc, int l)  // B = W^T
-{
-    v4s r;
-    for (int j = 0; j < 4; ++j) r[j] = (short)w[(nc + (l & 15)) * ld + kc + 4 * (l >> 4) + j];
-    return r;
-}
 __device__ __forceinline__ v4s opBf(const float *src, int ld, int nc, int l)  // B[k][n] = src[k][n]
 {
     v4s r;
     for (int j = 0; j < 4; ++j) r[j] = (short)f2bf(src[(4 * (l >> 4) + j) * ld + nc + (l & 15)]);
+    return r;
+}
+
+// K = 32 gathers (k = kc + 8 g + j)
+__device__ __forceinline__ v8bf opA8(const float *a, int ld, int kc, int l)
+{
+    const float *p = a + (l & 15) * ld + kc + 8 * (l >> 4);
+    v8bf r;
+    for (int j = 0; j < 8; ++j) r[j] = (__bf16)p[j];
+    return r;
+}
+__device__ __forceinline__ v8bf opBw8(const uint16_t *w, int ld, int kc, int nc, int l)  // B = W
+{
+    v8bf r;
+    for (int j = 0; j < 8; ++j) r[j] = __builtin_bit_cast(__bf16, w[(kc + 8 * (l >> 4) + j) * ld + nc + (l & 15)]);
+    return r;
+}
+__device__ __forceinline__ v8bf opBwT8(const uint16_t *w, int ld, int kc, int nc, int l)  // B = W^T
+{
+    const uint16_t *p = w + (nc + (l & 15)) * ld + kc + 8 * (l >> 4);
+    v8bf r;
+    for (int j = 0; j < 8; ++j) r[j] = __builtin_bit_cast(__bf16, p[j]);
     return r;
 }
 
@@ -168,8 +187,8 @@ __device__ void forward_tile(const Shared &sh, Wave &w, const Dev &d, uint64_t b
     __syncthreads();
     for (int nb = 0; nb < 4; ++nb) {
         v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
-        for (int kc = 0; kc < 4; ++kc)
-            acc = mfma(opA(&w.h1[0][0], kDH, 16 * kc, l), opBw(&sh.w2[0][0], kDH, 16 * kc, 16 * nb, l), acc);
+        for (int kc = 0; kc < kDH; kc += 32)
+            acc = mfma32(opA8(&w.h1[0][0], kDH, kc, l), opBw8(&sh.w2[0][0], kDH, kc, 16 * nb, l), acc);
         for (int j = 0; j < 4; ++j) {
             const int s = 4 * (l >> 4) + j, n = 16 * nb + (l & 15);
             const float z = acc[j] + sh.b2[n];
@@ -191,20 +210,24 @@ __device__ void forward_tile(const Shared &sh, Wave &w, const Dev &d, uint64_t b
 }
 
 // Inference only, activations kept in registers.  Both layers are computed
-// transposed, z^T = W^T x^T, so the output of one MFMA (D[n][s]: lane l holds hidden
-// units 4 (l / 16) + j of sample l % 16) is exactly the B operand the next layer needs
-// (B[k][s], same lane mapping); no LDS round trip.  Layer 3 is a dot product reduced
-// over the four lane groups of each sample.  Same arithmetic as the training forward
-// pass (bf16 operands, f32 accumulation, bf16-rounded activations); the layer-3 sum
-// runs in another order.
+// transposed, z^T = W^T x^T, so the outputs of the layer-1 MFMAs (D[n][s]: lane l
+// holds hidden units 16 mb + 4 g + j of sample l % 16, g = l / 16) are the B operand
+// of layer 2 with no LDS round trip: its K = 32 chunk c takes the tiles mb = 2c and
+// 2c + 1, i.e. register j of lane l holds unit k = 32 c + 16 (j / 4) + 4 g + j % 4,
+// and W2^T is gathered with the same k.  Layer 3 is a dot product reduced over the
+// four lane groups of each sample.  Same arithmetic as the training forward pass
+// (bf16 operands, f32 accumulation, bf16-rounded activations); the K order inside the
+// layer-2 MFMAs and the layer-3 sum order differ.
 constexpr int kInferBlock = 256;
 
 __global__ __launch_bounds__(kInferBlock) void k_infer(Dev d)
 {
     const int l = threadIdx.x & 63, g = l >> 4, sl = l & 15;
     // constant operands: A1[mb] = W1^T rows 16 mb .. (input dim padded to 16),
-    // A2[kc][nb] = W2^T, per-lane biases / output weights of the units it holds
-    v4s a1[4], a2[4][4];
+    // A2[c][nb] = W2^T (K chunk c, k as above), per-lane biases / output weights of
+    // the units it holds
+    v4s a1[4];
+    v8bf a2[2][4];
     float b1[4][4], b2[4][4], w3[4][4];
     for (int mb = 0; mb < 4; ++mb) {
         for (int j = 0; j < 4; ++j) {
@@ -215,10 +238,13 @@ __global__ __launch_bounds__(kInferBlock) void k_infer(Dev d)
             b2[mb][j] = d.params[oB2 + n];
             w3[mb][j] = bfr(d.params[oW3 + n]);
         }
-        for (int nb = 0; nb < 4; ++nb)
-            for (int j = 0; j < 4; ++j)
-                a2[mb][nb][j] = (short)f2bf(d.params[oW2 + (16 * mb + 4 * g + j) * kDH + 16 * nb + sl]);
     }
+    for (int c = 0; c < 2; ++c)
+        for (int nb = 0; nb < 4; ++nb)
+            for (int j = 0; j < 8; ++j) {
+                const int k = 32 * c + 16 * (j >> 2) + 4 * g + (j & 3);
+                a2[c][nb][j] = (__bf16)d.params[oW2 + k * kDH + 16 * nb + sl];
+            }
     float mean[4], sd[4];
     for (int j = 0; j < 4; ++j) {
         const int k = 4 * g + j;
@@ -239,15 +265,15 @@ __global__ __launch_bounds__(kInferBlock) void k_infer(Dev d)
             if (valid && k < kDIn) v = bfr(__fdiv_rn(__fsub_rn(d.x[row * kDIn + k], mean[j]), sd[j]));
             xb[j] = (short)f2bf(v);
         }
-        v4s h1[4];
+        v8bf h1[2];
         for (int mb = 0; mb < 4; ++mb) {
             const v4f z = mfma(a1[mb], xb, v4f{0.0f, 0.0f, 0.0f, 0.0f});
-            for (int j = 0; j < 4; ++j) h1[mb][j] = (short)f2bf(gelu(z[j] + b1[mb][j]));
+            for (int j = 0; j < 4; ++j) h1[mb >> 1][4 * (mb & 1) + j] = (__bf16)gelu(z[j] + b1[mb][j]);
         }
         float part = 0.0f;
         for (int nb = 0; nb < 4; ++nb) {
             v4f z = {0.0f, 0.0f, 0.0f, 0.0f};
-            for (int kc = 0; kc < 4; ++kc) z = mfma(a2[kc][nb], h1[kc], z);
+            for (int c = 0; c < 2; ++c) z = mfma32(a2[c][nb], h1[c], z);
             for (int j = 0; j < 4; ++j) part += bfr(gelu(z[j] + b2[nb][j])) * w3[nb][j];
         }
         part += __shfl_xor(part, 16);
@@ -300,8 +326,8 @@ __global__ __launch_bounds__(kBlock) void k_train(Dev d)
         }
         for (int nb = 0; nb < 4; ++nb) {
             v4f acc = {0.0f, 0.0f, 0.0f, 0.0f};
-            for (int kc = 0; kc < 4; ++kc)
-                acc = mfma(opA(&w.d[0][0], kDH, 16 * kc, l), opBwT(&sh.w2[0][0], kDH, 16 * kc, 16 * nb, l), acc);
+            for (int kc = 0; kc < kDH; kc += 32)
+                acc = mfma32(opA8(&w.d[0][0], kDH, kc, l), opBwT8(&sh.w2[0][0], kDH, kc, 16 * nb, l), acc);
             for (int j = 0; j < 4; ++j) {
                 const int s = 4 * (l >> 4) + j, n = 16 * nb + (l & 15);
                 w.g2[s][n] = acc[j] * w.g1[s][n];
