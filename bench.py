@@ -89,9 +89,10 @@ def kernel_name(slot, p, team):
     return f"hb::k_chunk<{p}>" if slot == 3 else f"hb::k_merge<{p}>"
 
 
-def measured_traffic(kernel, workload, p):
+def measured_traffic(kernel, workload, p, layout):
     """HBM bytes per launch of `kernel` from a committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary (profiles/*_traffic.txt, copy-calibrated), or None."""
+    WRITE_SIZE summary (profiles/*_traffic.txt, copy-calibrated) of the same
+    workload, precision and node layout, or None."""
     import glob
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*_traffic.txt"))):
         kv = {}
@@ -101,7 +102,8 @@ def measured_traffic(kernel, workload, p):
                     continue
                 k, v = line.split(":", 1)
                 kv[k.strip()] = v.strip()
-        if kv.get("kernel") == kernel and kv.get("workload") == workload and kv.get("p") == str(p):
+        if (kv.get("kernel") == kernel and kv.get("workload") == workload and kv.get("p") == str(p)
+                and kv.get("layout") == layout):
             return int(kv["traffic_bytes_per_launch"]), os.path.relpath(path, ROOT)
     return None, None
 
@@ -651,7 +653,8 @@ def main():
                 "alg_bytes_per_launch": int(slot_bytes[s] / slot_launch[s]),
                 "GBps": round(slot_bytes[s] / (slot_ms[s] * 1e-3) / 1e9, 1) if slot_ms[s] else None}
     dom_kernel = kernel_name(dom, p, int(st.kernel_team[dom]))
-    traffic, traffic_src = measured_traffic(dom_kernel, args.config, p)
+    layout_used = {v: k for k, v in HyperBall.LAYOUTS.items()}[int(st.layout)]
+    traffic, traffic_src = measured_traffic(dom_kernel, args.config, p, layout_used)
     all_ms = sum(slot_ms)
     iter_gbs = sum(slot_bytes) / (all_ms * 1e-3) / 1e9 if all_ms else 0.0
 
@@ -683,6 +686,7 @@ def main():
                    "nodes": g.size(), "arcs": g.width(), "iterations_per_step": st.iterations,
                    "max_iters": max_iters},
         "layout": args.layout,
+        "layout_used": layout_used,
         "layout_build_ms": round(float(st.ms_layout), 2),
         "gteps": round(arcs_g / total / 1e9, 3),
         "gteps_dense_equivalent": round(iters * g.width() / total / 1e9, 3),

@@ -565,12 +565,20 @@ static int build_schedule(hb_ctx *c, int p)
     size_t at = 0;
     const int teams[3] = {(int)G, (int)std::min(64u, 4 * G), 64};
     for (int b = 0; b < 3; ++b) {
-        s.bin_list[s.nbins] = c->sched_buf.ptr + at;
         s.bin_count[s.nbins] = (uint32_t)bins[b].size();
         s.bin_team[s.nbins] = teams[b];
+        // ids ascend within a bin, so it is one id range exactly when its ends are
+        // size - 1 apart (every bin of the degree and chain layouts): the kernels then
+        // compute the ids instead of loading them
+        if (p <= 10 && !bins[b].empty() && bins[b].back() - bins[b].front() + 1 == bins[b].size()) {
+            s.bin_list[s.nbins] = nullptr;
+            s.bin_base[s.nbins] = bins[b].front();
+        } else {
+            s.bin_list[s.nbins] = c->sched_buf.ptr + at;
+            host.insert(host.end(), bins[b].begin(), bins[b].end());
+            at += bins[b].size();
+        }
         ++s.nbins;
-        host.insert(host.end(), bins[b].begin(), bins[b].end());
-        at += bins[b].size();
     }
     auto put = [&](const std::vector<uint32_t> &v) -> const uint32_t * {
         const uint32_t *ptr = c->sched_buf.ptr + at;
@@ -784,6 +792,7 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
     d.five_m = 5.0f * (float)m;
     d.lc_thr = kLcThreshold[p];
     d.frontier = (flags & HB_FLAG_DENSE) ? 0 : 1;
+    d.window = c->relabelled() && c->layout_used == HB_LAYOUT_CHAIN ? 1 : 0;
 
     HbIterState st0{};
     st0.iter = 1;

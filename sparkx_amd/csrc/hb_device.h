@@ -87,6 +87,7 @@ struct HbDev {
     float five_m;          // 5m
     double lc_thr;         // HLL++ linear-counting threshold for p
     int frontier;
+    int window;            // dense bin-0 iterations read in-window neighbours from LDS (chain layout)
 };
 
 // ---------------------------------------------------------------------------

@@ -14,7 +14,8 @@ namespace hb {
 // Degree-binned schedule of one run (device pointers).
 struct HbSchedule {
     int nbins = 0;
-    const uint32_t *bin_list[4] = {};
+    const uint32_t *bin_list[4] = {};  // null: the bin is the id range bin_base .. + bin_count
+    uint32_t bin_base[4] = {};
     uint32_t bin_count[4] = {};
     int bin_team[4] = {};              // lanes per node in this bin
     const uint32_t *task_node = nullptr, *task_beg = nullptr, *task_end = nullptr;

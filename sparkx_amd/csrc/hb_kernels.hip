@@ -283,7 +283,7 @@ __global__ __launch_bounds__(kBlock) void k_arc_fingerprint(const uint32_t *off,
 // (p = 7) 41.8 ms at 8, 41.0 at 7, while the k-mer graph at 7 takes 57.0.
 template <int P, int T>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(P >= 7 ? 7 : 8))) void k_union(
-    HbDev d, const uint32_t *__restrict__ list, uint32_t count, int slot_id)
+    HbDev d, const uint32_t *__restrict__ list, uint32_t base, uint32_t count, int slot_id)
 {
     const HbIterState *st = d.st;
     if (st->done) return;
@@ -310,18 +310,32 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(P >= 7 ?
     v.use_frontier = d.frontier && it > 1 && !st->skip_scan && !st->push;
     // outside dense iterations only the neighbours that changed at t-1 are gathered
     v.gather_bits = d.frontier && it > 1 && !st->skip_scan ? v.bitsR : nullptr;
-    __shared__ uint32_t s_first[kBlock * 16];
+    __shared__ __attribute__((aligned(16))) uint32_t s_first[kBlock * 16];
     v.first = it == 1 && P <= 10;
     v.scr = s_first;
     const uint32_t tl = threadIdx.x % T;
     const uint32_t teams_per_grid = gridDim.x * (kBlock / T);
     HbCounts c;
+    if constexpr (T == (1 << P) / 16) {
+        // dense iteration over an id-range bin: each wavefront's teams hold consecutive
+        // nodes, and neighbours among them come from the wavefront's LDS window
+        if (d.window && !list && !v.first && !v.gather_bits) {
+            uint4 *win = reinterpret_cast<uint4 *>(s_first) + (threadIdx.x / HB_WAVE) * HB_WAVE;
+            const uint32_t in_wave = (threadIdx.x % HB_WAVE) / T;
+            for (uint32_t team = blockIdx.x * (kBlock / T) + threadIdx.x / T; team < count; team += teams_per_grid) {
+                const uint32_t lo = base + team - in_wave;
+                union_node_window<P, T>(v, tb, base + team, tl, c, lo, min(lo + HB_WAVE / T, base + count), win);
+            }
+            block_add5(c.inc, c.arcs, c.nodes, c.chg, c.gathered, d.rec + it % HB_REC_RING, slot_id);
+            return;
+        }
+    }
     if (v.gather_bits) {
         for (uint32_t team = blockIdx.x * (kBlock / T) + threadIdx.x / T; team < count; team += teams_per_grid)
-            union_node<P, T, true>(v, tb, list[team], tl, c);
+            union_node<P, T, true>(v, tb, list ? list[team] : base + team, tl, c);
     } else {
         for (uint32_t team = blockIdx.x * (kBlock / T) + threadIdx.x / T; team < count; team += teams_per_grid)
-            union_node<P, T, false>(v, tb, list[team], tl, c);
+            union_node<P, T, false>(v, tb, list ? list[team] : base + team, tl, c);
     }
     block_add5(c.inc, c.arcs, c.nodes, c.chg, c.gathered, d.rec + it % HB_REC_RING, slot_id);
 }
@@ -732,7 +746,7 @@ static unsigned grid_for(uint64_t threads, unsigned cap)
 }
 
 template <int P>
-static hipError_t launch_union_T(const HbDev &d, const uint32_t *list, uint32_t count, int T,
+static hipError_t launch_union_T(const HbDev &d, const uint32_t *list, uint32_t base, uint32_t count, int T,
                                  unsigned max_blocks, int slot, hipStream_t s)
 {
     constexpr int G = (1 << P) / 16;
@@ -741,7 +755,7 @@ static hipError_t launch_union_T(const HbDev &d, const uint32_t *list, uint32_t 
 #define HB_CASE(TT)                                                                        \
     case TT:                                                                               \
         if constexpr (TT >= G) {                                                           \
-            k_union<P, TT><<<grid, kBlock, 0, s>>>(d, list, count, slot);                  \
+            k_union<P, TT><<<grid, kBlock, 0, s>>>(d, list, base, count, slot);            \
             return hipGetLastError();                                                      \
         }                                                                                  \
         break;
@@ -819,8 +833,8 @@ hipError_t launch_iteration(const HbDev &d, const HbSchedule &sc, bool nf, hipSt
                 e = hipGetLastError();
             });
         } else {
-            HB_DISPATCH_P(d.p, e = launch_union_T<P>(d, sc.bin_list[b], sc.bin_count[b], sc.bin_team[b],
-                                                     sc.max_blocks, b, s));
+            HB_DISPATCH_P(d.p, e = launch_union_T<P>(d, sc.bin_list[b], sc.bin_base[b], sc.bin_count[b],
+                                                     sc.bin_team[b], sc.max_blocks, b, s));
         }
         if (ev) (void)hipEventRecord(ev[2 * b + 1], s);
         if (e != hipSuccess) return e;

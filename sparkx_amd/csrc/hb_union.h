@@ -224,4 +224,65 @@ __device__ __forceinline__ void union_node(const HbView &v, const HbTables &tb, 
     }
 }
 
+// union_node for a dense iteration (every neighbour gathered, no frontier check) with one
+// sketch copy per team (T = G), where the wavefront's 64 / G teams hold the consecutive
+// nodes lo .. hi - 1 (a bin that is an id range).  Each team puts its own row into the
+// wavefront's LDS window `win` (64 uint4, copy lane cl of node w at (w - lo) G + cl), and a
+// neighbour inside [lo, hi) is read from there instead of memory: in a numbering that
+// keeps chains together (the chain layout) most neighbours are in the window, and the
+// union is bound by the rate of memory requests, not by bytes.  Same registers,
+// estimates and accumulators as union_node.
+template <int P, int T>
+__device__ __forceinline__ void union_node_window(const HbView &v, const HbTables &tb, uint32_t u, uint32_t tl,
+                                                  HbCounts &c, uint32_t lo, uint32_t hi, uint4 *win)
+{
+    constexpr uint32_t G = (1u << P) / 16;
+    static_assert(T == (int)G, "one copy per team");
+    const uint32_t cl = tl;
+    const uint32_t e0 = v.off[u], deg = v.off[u + 1] - e0;
+    const uint32_t *__restrict__ nb = v.nbr + e0;
+    const uint8_t *__restrict__ in = v.in;
+    const uint4 own = *(const uint4 *)(in + ((uint64_t)u << P) + cl * 16);
+    win[(u - lo) * G + cl] = own;
+    // LDS operations of one wavefront complete in order; the barrier and fence keep the
+    // compiler from moving the window reads above the write
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    const uint32_t span = hi - lo;
+    HbAcc pa = hb_acc(own);
+    uint32_t j = 0;
+    for (; j + 1 < deg; j += 2) {
+        const uint32_t w0 = nb[j], w1 = nb[j + 1];
+        const uint4 x0 = w0 - lo < span ? win[(w0 - lo) * G + cl] : *(const uint4 *)(in + ((uint64_t)w0 << P) + cl * 16);
+        const uint4 x1 = w1 - lo < span ? win[(w1 - lo) * G + cl] : *(const uint4 *)(in + ((uint64_t)w1 << P) + cl * 16);
+        hb_acc_max(pa, x0);
+        hb_acc_max(pa, x1);
+    }
+    if (j < deg) {
+        const uint32_t w0 = nb[j];
+        hb_acc_max(pa, w0 - lo < span ? win[(w0 - lo) * G + cl] : *(const uint4 *)(in + ((uint64_t)w0 << P) + cl * 16));
+    }
+    const uint4 acc = hb_acc_get(pa);
+    if (cl == 0) c.gathered += deg;
+    // every team has read the window before the next trip overwrites it
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+
+    const uint32_t diffw = (acc.x != own.x) | (acc.y != own.y) | (acc.z != own.z) | (acc.w != own.w);
+    const bool changed = hb_copy_or<G>(diffw) != 0;
+    const float curr = hb_estimate_copy<G>(tb, acc);
+    *(uint4 *)(v.out + ((uint64_t)u << P) + cl * 16) = acc;
+    if (tl == 0) {
+        const float prev = v.est[u];
+        if (accumulate(v, u, prev, curr)) c.inc += 1;
+        v.est[u] = curr;
+        if (changed) {
+            atomicOr(v.bitsW + (u >> 5), 1u << (u & 31u));
+            c.chg += 1;
+        }
+        c.arcs += deg;
+        c.nodes += 1;
+    }
+}
+
 }  // namespace hb

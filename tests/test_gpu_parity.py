@@ -738,6 +738,52 @@ def test_device_layouts_match_input_layout_and_oracle(hb, oracle, name, p):
             assert np.array_equal(a, b)
 
 
+def _chain_graph(hb, kind):
+    """Graphs whose chain numbering keeps every linked node in one id-range bin, so the
+    dense iterations read in-window neighbours from LDS: a de Bruijn graph with
+    duplicated arcs, self-loops and isolated nodes appended, and bare directed cycles
+    whose lengths do not divide a wavefront's teams."""
+    if kind == "kmer":
+        g = hb.Graph.kmer_dbg(12000, 4, 0.02, 9, seed=8)
+        off, nb = g.offsets.astype(np.int64), g.neighbours.astype(np.int64)
+        n = g.size()
+        src = np.repeat(np.arange(n), np.diff(off))
+        extra = np.arange(0, n, 7)  # self-loops and a repeated arc per 7th node
+        src = np.concatenate([src, extra, src[: n // 3]])
+        dst = np.concatenate([nb, extra, nb[: n // 3]])
+        n_all = n + 37  # isolated nodes after the linked ones
+    else:
+        lens = [5, 63, 1000, 17, 2]
+        src, dst, at = [], [], 0
+        for m in lens:
+            ids = np.arange(at, at + m)
+            src.append(ids)
+            dst.append(np.roll(ids, -1))
+            at += m
+        src, dst = np.concatenate(src), np.concatenate(dst)
+        n_all = at + 3
+    order = np.lexsort((dst, src))
+    src, dst = src[order], dst[order]
+    off = np.zeros(n_all + 1, np.uint64)
+    np.add.at(off, src + 1, 1)
+    return hb.Graph(np.cumsum(off).astype(np.uint64), dst.astype(np.uint32), kind)
+
+
+@pytest.mark.parametrize("kind", ["kmer", "cycles"])
+@pytest.mark.parametrize("p", [4, 5, 6, 7, 8, 10])
+def test_chain_layout_window_matches_oracle(hb, oracle, kind, p):
+    """Chain numbering, dense iterations: the union reads neighbours inside each
+    wavefront's window of consecutive nodes from LDS (team widths 1..64).  Default and
+    dense schedules, hipGraph replay included, equal the oracle bit for bit."""
+    g = _chain_graph(hb, kind)
+    say("chain window", kind, p, g.size(), g.width())
+    ref = _oracle_run(oracle, g, p, max_iters=300)
+    for kw in (dict(), dict(dense=True), dict(dense=True, graph_capture=True)):
+        with hb.HyperBall(g, p, layout="chain", max_iters=300, **kw) as h:
+            assert h.stats.layout == 2
+            _assert_same(h, ref, oracle, p, f"{kind} p={p} {kw}")
+
+
 def test_layout_switch_and_snapshots(hb, oracle):
     """Switching layouts on one context discards the run; snapshots come back in the
     caller's order under both."""
