@@ -14,6 +14,8 @@
 // of unchanged sets), so u is skipped; when u itself changed at t-1 its row is
 // copied into the output buffer so both buffers stay complete.  Results are
 // identical to recomputing every node.
+#include <cmath>
+
 #include "hb_device.h"
 #include "hb_kernels.h"
 #include "hb_union.h"
@@ -81,22 +83,38 @@ __device__ __forceinline__ bool accumulate(const HbDev &d, uint32_t u, float pre
 template <int P>
 __global__ __launch_bounds__(kBlock) void k_init(HbDev d)
 {
-    constexpr uint32_t M = 1u << P, G = M / 16;
-    HB_TABLES_LDS(P, d, tb)
-    const uint64_t gtid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t u = (uint32_t)(gtid / G), cl = (uint32_t)(gtid % G);
-    if (u >= d.n) return;  // whole copies leave together (G divides the block)
-    uint32_t idx, rank;
-    hb_index_rank(d.node_id ? d.node_id[u] : u, P, &idx, &rank);
-    uint32_t w[4] = {0, 0, 0, 0};
-    if (idx / 16 == cl) w[(idx % 16) / 4] = rank << (8 * (idx % 4));
-    const uint4 r = make_uint4(w[0], w[1], w[2], w[3]);
-    *(uint4 *)(d.regs0 + ((uint64_t)u << P) + cl * 16) = r;
-    // a node without neighbours keeps c_0 forever and is never scheduled: both buffers
-    if (d.off[u + 1] == d.off[u]) *(uint4 *)(d.regs1 + ((uint64_t)u << P) + cl * 16) = r;
-    const float e = hb_estimate_copy<G>(tb, r);
-    if (cl == 0) {
-        d.est[u] = e;
+    // One thread per node computes the node's insertion (one SipHash-1-3 per node); the
+    // wavefront then writes the rows of its 64 nodes, G lanes per row and 64 / G rows per
+    // store, taking each row's insertion by shuffle, so the stores stay coalesced.
+    constexpr uint32_t M = 1u << P, G = M / 16, RPI = HB_WAVE / G;
+    const uint32_t lane = threadIdx.x % HB_WAVE, cl = lane % G;
+    const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t first = u - lane;  // the wavefront's first node
+    uint32_t word = 0;                // rank | idx << 8 | (no neighbours) << 31
+    if (u < d.n) {
+        uint32_t idx, rank;
+        hb_index_rank(d.node_id ? d.node_id[u] : (uint32_t)u, P, &idx, &rank);
+        word = rank | (idx << 8) | (d.off[u + 1] == d.off[u] ? 1u << 31 : 0u);
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < G; ++k) {
+        const uint32_t r = k * RPI + lane / G;  // this store's row among the wavefront's nodes
+        const uint32_t w = (uint32_t)__shfl((int)word, (int)r);
+        const uint64_t node = first + r;
+        if (node < d.n) {
+            const uint32_t idx = (w >> 8) & 0xffffu, rank = w & 0xffu;
+            uint32_t q[4] = {0, 0, 0, 0};
+            if (idx / 16 == cl) q[(idx % 16) / 4] = rank << (8 * (idx % 4));
+            const uint4 row = make_uint4(q[0], q[1], q[2], q[3]);
+            *(uint4 *)(d.regs0 + (node << P) + cl * 16) = row;
+            // a node without neighbours keeps c_0 forever and is never scheduled: both buffers
+            if (w >> 31) *(uint4 *)(d.regs1 + (node << P) + cl * 16) = row;
+        }
+    }
+    if (u < d.n) {
+        // c_0 holds one register of rank >= 1 and m - 1 empty ones, so the estimator returns
+        // linear counting of m - 1 zeros (launch_init checks that the HLL++ threshold admits it)
+        d.est[u] = d.lc[M - 1];
         d.dist[u] = 0.0;
         d.inv[u] = 0.0;
     }
@@ -796,9 +814,14 @@ hipError_t launch_init(const HbDev &d, hipStream_t s)
         });
         return hipGetLastError();
     }
+    // k_init stores the linear-counting estimate of m - 1 zeros for every node; it is what
+    // the estimator returns for a one-insertion sketch when LC <= the HLL++ threshold,
+    // which holds at every p (LC < 1.04, thresholds >= 10), checked here with the same
+    // expression the LC table is filled with
+    const double m = (double)(1u << d.p);
+    if ((double)(float)(m * std::log(m / (m - 1.0))) > d.lc_thr) return hipErrorInvalidValue;
     HB_DISPATCH_P(d.p, {
-        const uint64_t threads = (uint64_t)d.n * ((1 << P) / 16);
-        k_init<P><<<(unsigned)((threads + kBlock - 1) / kBlock), kBlock, 0, s>>>(d);
+        k_init<P><<<(unsigned)(((uint64_t)d.n + kBlock - 1) / kBlock), kBlock, 0, s>>>(d);
     });
     return hipGetLastError();
 }
