@@ -473,6 +473,19 @@ bool trace_on()
 struct hb_batch {
     int device = 0;
     hipStream_t stream = nullptr;
+    // the precisions touch disjoint graphs, so each precision's kernels run on a stream
+    // of their own between a fork after the tile compaction and a join before finalize
+    // (in the captured graph: parallel branches)
+    hipStream_t side[hb::kBatchNP] = {};
+    hipEvent_t fork = nullptr, join[hb::kBatchNP] = {};
+    // captured iterations, one per (precisions still running, dense flag); they depend
+    // only on the batch's buffers and tiles, so they are kept until the tiles change
+    std::vector<std::pair<uint32_t, hipGraphExec_t>> execs;
+    void drop_execs()
+    {
+        for (const auto &x : execs) (void)hipGraphExecDestroy(x.second);
+        execs.clear();
+    }
     std::vector<hb::HbBatchGraph> hg;  // host copy (bases, constants)
     std::vector<std::vector<uint32_t>> h_off;
     std::vector<int> max_iters;
@@ -503,6 +516,12 @@ struct hb_batch {
 
     ~hb_batch()
     {
+        drop_execs();
+        for (int i = 0; i < hb::kBatchNP; ++i) {
+            if (join[i]) (void)hipEventDestroy(join[i]);
+            if (side[i]) (void)hipStreamDestroy(side[i]);
+        }
+        if (fork) (void)hipEventDestroy(fork);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -546,6 +565,7 @@ static void bin_bounds(const hb_batch *b, uint32_t p, uint32_t *small_max, uint3
 static int build_tiles(hb_batch *b)
 {
     if (b->sched_ok) return HB_OK;
+    b->drop_execs();
     struct Item {
         hb::HbBatchTile t;
         uint64_t work;
@@ -649,6 +669,11 @@ int hb_batch_create(hb_batch **out, int device, const hb_graph_desc *graphs, int
     auto b = std::make_unique<hb_batch>();
     b->device = device;
     HB_BTRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    HB_BTRY(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    for (int i = 0; i < hb::kBatchNP; ++i) {
+        HB_BTRY(hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking));
+        HB_BTRY(hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming));
+    }
     b->hg.resize(count);
     b->h_off.resize(count);
     b->max_iters.resize(count);
@@ -840,33 +865,46 @@ int hb_batch_run(hb_batch *b, uint32_t flags, hb_batch_stats *stats)
     HB_BTRY(hipEventCreate(&e1));
     HB_BTRY(hipEventCreate(&e2));
     int rc = HB_OK;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
     uint32_t launches = 0;
     const bool trace = trace_on();
-    auto launch_iter = [&]() -> hipError_t {
+    // precisions with any tile or hub: each gets a branch
+    bool has_work[hb::kBatchNP] = {};
+    for (int pi = 0; pi < hb::kBatchNP; ++pi) {
+        has_work[pi] = hub_lists[pi].count != 0;
+        for (int c = 0; c < hb::kClasses; ++c) has_work[pi] = has_work[pi] || lists[pi * hb::kClasses + c].count;
+    }
+    // `running`: bit pi set while a graph of precision kBatchPMin + pi may still run.  A
+    // stopped graph never runs again and its kernels would return at once, so leaving
+    // out a precision whose graphs have all stopped changes nothing.
+    auto launch_iter = [&](uint32_t running) -> hipError_t {
         hb::k_batch_begin<<<(unsigned)std::min(ng, 1024), hb::kBatchBlock, 0, b->stream>>>(d);
         hb::k_batch_compact<<<hb::kQueues, 1024, 0, b->stream>>>(d, classes);
-        // per precision: the three node bins (team widths G, 4G capped at 64, and 64),
-        // the hub chunks, then the hubs
+        hipError_t e = hipEventRecord(b->fork, b->stream);
+        // per precision, on its own stream: the three node bins (team widths G, 4G capped
+        // at 64, and 64), the hub chunks, then the hubs
 #define HB_BATCH_LAUNCH(q, kernel)                                                                     \
-        if (lists[q].count) kernel<<<b->grid[q], hb::kBatchBlock, 0, b->stream>>>(d, lists[q]);
+        if (lists[q].count) kernel<<<b->grid[q], hb::kBatchBlock, 0, s>>>(d, lists[q]);
 #define HB_BATCH_P(P)                                                                                  \
-        {                                                                                              \
+        if (has_work[P - hb::kBatchPMin] && ((running >> (P - hb::kBatchPMin)) & 1u) && e == hipSuccess) { \
             constexpr int G_ = (1 << P) / 16, q0 = (P - hb::kBatchPMin) * hb::kClasses;               \
+            const hipStream_t s = b->side[P - hb::kBatchPMin];                                         \
+            e = hipStreamWaitEvent(s, b->fork, 0);                                                     \
             HB_BATCH_LAUNCH(q0, (hb::k_batch_nodes<P, G_>))                                            \
             HB_BATCH_LAUNCH(q0 + 1, (hb::k_batch_nodes<P, (4 * G_ < 64 ? 4 * G_ : 64)>))              \
             HB_BATCH_LAUNCH(q0 + 2, (hb::k_batch_nodes<P, 64>))                                        \
             HB_BATCH_LAUNCH(q0 + hb::kClassChunk, hb::k_batch_chunk<P>)                                \
             if (hub_lists[P - hb::kBatchPMin].count)                                                   \
-                hb::k_batch_merge<P><<<(hub_lists[P - hb::kBatchPMin].count + 3) / 4, hb::kBatchBlock, 0, b->stream>>>( \
+                hb::k_batch_merge<P><<<(hub_lists[P - hb::kBatchPMin].count + 3) / 4, hb::kBatchBlock, 0, s>>>( \
                     d, hub_lists[P - hb::kBatchPMin]);                                                 \
+            if (e == hipSuccess) e = hipEventRecord(b->join[P - hb::kBatchPMin], s);                   \
+            if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->join[P - hb::kBatchPMin], 0);    \
         }
         HB_BATCH_P(4) HB_BATCH_P(5) HB_BATCH_P(6) HB_BATCH_P(7) HB_BATCH_P(8)
 #undef HB_BATCH_P
 #undef HB_BATCH_LAUNCH
         hb::k_batch_finalize<<<(ng + 63) / 64, 64, 0, b->stream>>>(d);
-        return hipGetLastError();
+        const hipError_t el = hipGetLastError();
+        return e != hipSuccess ? e : el;
     };
     b->history.assign(ng, {});
     const auto t_start = std::chrono::steady_clock::now();
@@ -882,18 +920,29 @@ int hb_batch_run(hb_batch *b, uint32_t flags, hb_batch_stats *stats)
             if (hb::launch_init(graph_dev(b, i), b->stream) != hipSuccess) rc = bfail(HB_EHIP, "init kernel failed");
         if (rc != HB_OK) break;
         (void)hipEventRecord(e1, b->stream);
-        if (flags & HB_FLAG_GRAPH) {
-            if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-                rc = bfail(HB_EHIP, "hipStreamBeginCapture failed");
-                break;
-            }
-            const hipError_t ec = launch_iter();
+        // the replayed iteration for a set of precisions, captured on first use
+        auto exec_for = [&](uint32_t running, hipGraphExec_t *out) -> int {
+            const uint32_t key = running | (d.frontier ? 0u : 1u << 16);
+            for (const auto &x : b->execs)
+                if (x.first == key) {
+                    *out = x.second;
+                    return HB_OK;
+                }
+            hipGraph_t graph = nullptr;
+            hipGraphExec_t ge = nullptr;
+            if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+                return bfail(HB_EHIP, "hipStreamBeginCapture failed");
+            const hipError_t ec = launch_iter(running);
             const hipError_t ee = hipStreamEndCapture(b->stream, &graph);
-            if (ec != hipSuccess || ee != hipSuccess || hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0) != hipSuccess) {
-                rc = bfail(HB_EHIP, "hb_batch_run: graph capture failed");
-                break;
-            }
-        }
+            const bool ok = ec == hipSuccess && ee == hipSuccess &&
+                            hipGraphInstantiate(&ge, graph, nullptr, nullptr, 0) == hipSuccess;
+            if (graph) (void)hipGraphDestroy(graph);
+            if (!ok) return bfail(HB_EHIP, "hb_batch_run: graph capture failed");
+            b->execs.emplace_back(key, ge);
+            *out = ge;
+            return HB_OK;
+        };
+        uint32_t running_p = (1u << hb::kBatchNP) - 1;
         std::vector<hb::HbBatchGraph> hs(ng);
         std::vector<hb::HbBatchRec> ring((size_t)ng * hb::kRing);
         std::vector<uint32_t> harvested(ng, 0);  // iterations copied into history
@@ -905,8 +954,10 @@ int hb_batch_run(hb_batch *b, uint32_t flags, hb_batch_stats *stats)
                 rc = bfail(HB_ESTATE, "hb_batch_run: %u launches without every graph stopping", launches);
                 break;
             }
+            hipGraphExec_t gexec = nullptr;
+            if (flags & HB_FLAG_GRAPH) rc = exec_for(running_p, &gexec);
             for (uint32_t i = 0; i < k && rc == HB_OK; ++i) {
-                const hipError_t el = gexec ? hipGraphLaunch(gexec, b->stream) : launch_iter();
+                const hipError_t el = gexec ? hipGraphLaunch(gexec, b->stream) : launch_iter(running_p);
                 if (el != hipSuccess) rc = bfail(HB_EHIP, "iteration launch: %s", hipGetErrorString(el));
             }
             launches += k;
@@ -920,8 +971,10 @@ int hb_batch_run(hb_batch *b, uint32_t flags, hb_batch_stats *stats)
             }
             bool all = true;
             uint32_t running = 0, it_max = 0;
+            running_p = 0;
             for (int i = 0; i < ng; ++i) {
                 const hb::HbBatchGraph &g = hs[i];
+                if (!g.st.done) running_p |= 1u << (g.p - hb::kBatchPMin);
                 const uint32_t complete = g.st.done ? g.st.executed : g.st.iter - 1;  // iterations finished
                 for (uint32_t t = harvested[i] + 1; t <= complete; ++t)
                     b->history[i].push_back(ring[(size_t)i * hb::kRing + t % hb::kRing]);
@@ -946,8 +999,6 @@ int hb_batch_run(hb_batch *b, uint32_t flags, hb_batch_stats *stats)
             if (hs[i].overflow)
                 rc = bfail(HB_ESTATE, "hb_batch_run: graph %d still changing after %u iterations (n + 2)", i, hs[i].cap);
     } while (0);
-    if (gexec) (void)hipGraphExecDestroy(gexec);
-    if (graph) (void)hipGraphDestroy(graph);
     float ms_init = 0.0f, ms_loop = 0.0f;
     if (rc == HB_OK) {
         (void)hipEventElapsedTime(&ms_init, e0, e1);

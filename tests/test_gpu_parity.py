@@ -427,7 +427,9 @@ def test_batch_small_graphs_against_oracle_and_schedules(hb, oracle, batch_trace
     items.append((_hub_graph(hb, 9000, 5000, 30000, 72), 8))
     items.append((_hub_graph(hb, 20000, 17000, 10000, 73), 4))
     refs = [_oracle_run(oracle, g, p) for g, p in items]
-    variants = [dict(graph_capture=True), dict(graph_capture=False), dict(dense=True),
+    # the second run replays the iterations the first captured (kept per set of running
+    # precisions until the schedule changes)
+    variants = [dict(graph_capture=True), dict(graph_capture=True), dict(graph_capture=False), dict(dense=True),
                 dict(schedule=(2, 16, 512)), dict(schedule=(16, 256, 65536))]
     from sparkx_amd import _lib
     with HyperBallBatch(items) as bt:
