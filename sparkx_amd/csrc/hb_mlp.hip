@@ -44,11 +44,27 @@ constexpr int kWaves = kBlock / 64;
 __device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
 __device__ __forceinline__ float bfr(float f) { return (float)(__bf16)f; }
 
-__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
+// erf by Abramowitz & Stegun, Handbook of Mathematical Functions, 7.1.26 (absolute error
+// at most 1.5e-7): erf(z) = 1 - t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) exp(-z^2), with
+// t = 1 / (1 + p z), for z >= 0, odd in z.  About half the instructions of erff; the
+// activations it feeds are rounded to bf16 (8 bits) right after.  Training and inference
+// both use it, so the network is evaluated with the function it was trained with.
+__device__ __forceinline__ float erf_as(float x)
+{
+    const float z = fabsf(x);
+    const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, z, 1.0f));
+    float q = fmaf(t, 1.061405429f, -1.453152027f);
+    q = fmaf(t, q, 1.421413741f);
+    q = fmaf(t, q, -0.284496736f);
+    q = fmaf(t, q, 0.254829592f);
+    const float e = __builtin_amdgcn_exp2f(-(z * z) * 1.44269504088896341f);  // exp(-z^2)
+    return copysignf(fmaf(-(t * q), e, 1.0f), x);
+}
+__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erf_as(x * 0.70710678118654752f)); }
 __device__ __forceinline__ float gelu_grad(float x)
 {
-    const float cdf = 0.5f * (1.0f + erff(x * 0.70710678118654752f));
-    return cdf + x * (expf(-0.5f * x * x) * 0.39894228040143268f);
+    const float cdf = 0.5f * (1.0f + erf_as(x * 0.70710678118654752f));
+    return cdf + x * (__builtin_amdgcn_exp2f(-0.5f * x * x * 1.44269504088896341f) * 0.39894228040143268f);
 }
 
 __device__ __forceinline__ v4f mfma(v4s a, v4s b, v4f c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
@@ -425,16 +441,23 @@ __global__ __launch_bounds__(256) void k_feature_sums(const uint32_t *off, const
 // over nodes with a neighbour; a constant input maps to 0
 __global__ void k_standardize(const double *s, uint32_t n, float *x)
 {
-    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= n) return;
-    const double cnt = s[12] > 0.0 ? s[12] : 1.0;
-    float *r = x + (uint64_t)u * kDIn;
-    for (int k = 0; k < kDIn; ++k) {
+    // each column's mean and standard deviation, once per block (the same f64 operations
+    // for every block, so every node sees the same values)
+    __shared__ double s_mean[kDIn], s_sd[kDIn];
+    if (threadIdx.x < kDIn) {
+        const int k = (int)threadIdx.x;
+        const double cnt = s[12] > 0.0 ? s[12] : 1.0;
         const double mean = s[k] / cnt;
         const double var = s[6 + k] / cnt - mean * mean;
-        const double sd = var > 0.0 ? sqrt(var) : 0.0;
-        r[k] = sd > 0.0 ? (float)(((double)r[k] - mean) / sd) : 0.0f;
+        s_mean[k] = mean;
+        s_sd[k] = var > 0.0 ? sqrt(var) : 0.0;
     }
+    __syncthreads();
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    float *r = x + (uint64_t)u * kDIn;
+    for (int k = 0; k < kDIn; ++k)
+        r[k] = s_sd[k] > 0.0 ? (float)(((double)r[k] - s_mean[k]) / s_sd[k]) : 0.0f;
 }
 
 // nodes without neighbours have harmonic centrality 0: rank them last
