@@ -38,6 +38,34 @@ def test_library_was_built_from_these_sources():
     assert _lib.load().hb_build_digest().decode() == build.source_digest()
 
 
+def test_rust_binding_in_integration_matches_the_header():
+    """The Rust binding documented for a maintainer declares hb_stats' fields in the C
+    order with the same sizes, and only functions the header declares."""
+    c_types = {"int32_t": 4, "uint32_t": 4, "float": 4, "double": 8, "uint64_t": 8}
+    rust_types = {"i32": 4, "u32": 4, "f32": 4, "f64": 8, "u64": 8}
+    with open(os.path.join(ROOT, "include", "sparkx_hb.h")) as fh:
+        h = fh.read()
+    body = re.sub(r"/\*.*?\*/", "", h[h.index("typedef struct {"):h.index("} hb_stats;")], flags=re.S)
+    c_fields = []
+    for decl in body[len("typedef struct {"):].split(";"):
+        decl = decl.strip()
+        if decl:
+            ctype, names = decl.split(None, 1)
+            for name in names.split(","):
+                m = re.match(r"\s*(\w+)\s*(\[\w+\])?", name)
+                c_fields.append((m.group(1), c_types[ctype], bool(m.group(2))))
+    with open(os.path.join(ROOT, "INTEGRATION.md")) as fh:
+        doc = fh.read()
+    rust = doc[doc.index("pub struct HbStats {"):]
+    rust = rust[:rust.index("\n}")]
+    r_fields = [(name, rust_types[t], arr is not None and arr != "")
+                for name, arr, t in re.findall(r"pub (\w+): (\[)?(\w+)", rust)]
+    assert r_fields == c_fields
+    declared = _declared_functions()
+    for fn in re.findall(r"\bfn (hb_\w+)\s*\(", doc):
+        assert fn in declared, fn
+
+
 def test_create_reports_errors_without_crashing():
     from sparkx_amd import _lib
     L = _lib.load()
