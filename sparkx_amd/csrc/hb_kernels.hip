@@ -298,9 +298,11 @@ __global__ __launch_bounds__(kBlock) void k_arc_fingerprint(const uint32_t *off,
 // help until the register bound makes the compiler spill.  8 (at most 64 VGPRs) took the
 // k-mer graph (p = 6) from 59.5 to 49.5 ms per run; at p >= 7 the register rows are
 // wider and 7 (at most 72 VGPRs, no spill in the degree <= 256 bin) is faster: R-MAT-24
-// (p = 7) 41.8 ms at 8, 41.0 at 7, while the k-mer graph at 7 takes 57.0.
+// (p = 7) 41.8 ms at 8, 41.0 at 7, while the k-mer graph at 7 takes 57.0.  The teams of 32
+// and 64 lanes spill 10-12 VGPRs at 72; at 6 (80 VGPRs) they do not, and R-MAT-24's
+// degree 257..4096 bins run 1-2% faster.
 template <int P, int T>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(P >= 7 ? 7 : 8))) void k_union(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(P >= 7 ? (T >= 32 ? 6 : 7) : 8))) void k_union(
     HbDev d, const uint32_t *__restrict__ list, uint32_t base, uint32_t count, int slot_id)
 {
     const HbIterState *st = d.st;
