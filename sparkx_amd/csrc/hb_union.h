@@ -40,17 +40,25 @@ __device__ __forceinline__ bool bit_test(const uint32_t *bits, uint32_t v)
     return (bits[v >> 5] >> (v & 31u)) & 1u;
 }
 
-// f64 accumulation without contraction (hyper_ball.rs:819-828)
-__device__ __forceinline__ bool accumulate(const HbView &v, uint32_t u, float prev, float curr)
+// f64 accumulation without contraction (hyper_ball.rs:819-828); dist0 / inv0 are the
+// node's accumulators as loaded by the caller
+__device__ __forceinline__ bool accumulate(const HbView &v, uint32_t u, float prev, float curr, double dist0,
+                                           double inv0)
 {
     const float d32 = __fsub_rn(curr, prev);
     const double diff = (double)d32;
     if (diff > 0.0) {
-        v.dist[u] = __dadd_rn(v.dist[u], __dmul_rn(v.t, diff));
-        v.inv[u] = __dadd_rn(v.inv[u], __dmul_rn(v.inv_t, diff));
+        v.dist[u] = __dadd_rn(dist0, __dmul_rn(v.t, diff));
+        v.inv[u] = __dadd_rn(inv0, __dmul_rn(v.inv_t, diff));
         return true;
     }
     return false;
+}
+__device__ __forceinline__ bool accumulate(const HbView &v, uint32_t u, float prev, float curr)
+{
+    const float d32 = __fsub_rn(curr, prev);
+    if (!((double)d32 > 0.0)) return false;  // the accumulators are read only when needed
+    return accumulate(v, u, prev, curr, v.dist[u], v.inv[u]);
 }
 
 // (index << 6) | rank of the insertion of node id v at precision P (P <= 10)
@@ -242,6 +250,10 @@ __device__ __forceinline__ void union_node_window(const HbView &v, const HbTable
     const uint32_t e0 = v.off[u], deg = v.off[u + 1] - e0;
     const uint32_t *__restrict__ nb = v.nbr + e0;
     const uint8_t *__restrict__ in = v.in;
+    // the node's estimate is loaded beside its own row, its accumulators right after the
+    // gathers: each then arrives while the union and the estimate are computed instead of
+    // costing a memory round trip of its own after them (≈ 1% on the k-mer graph)
+    const float prev = tl == 0 ? v.est[u] : 0.0f;
     const uint4 own = *(const uint4 *)(in + ((uint64_t)u << P) + cl * 16);
     win[(u - lo) * G + cl] = own;
     // LDS operations of one wavefront complete in order; the barrier and fence keep the
@@ -263,6 +275,11 @@ __device__ __forceinline__ void union_node_window(const HbView &v, const HbTable
         hb_acc_max(pa, w0 - lo < span ? win[(w0 - lo) * G + cl] : *(const uint4 *)(in + ((uint64_t)w0 << P) + cl * 16));
     }
     const uint4 acc = hb_acc_get(pa);
+    double dist0 = 0.0, inv0 = 0.0;
+    if (tl == 0) {
+        dist0 = v.dist[u];
+        inv0 = v.inv[u];
+    }
     if (cl == 0) c.gathered += deg;
     // every team has read the window before the next trip overwrites it
     __builtin_amdgcn_wave_barrier();
@@ -273,8 +290,7 @@ __device__ __forceinline__ void union_node_window(const HbView &v, const HbTable
     const float curr = hb_estimate_copy<G>(tb, acc);
     *(uint4 *)(v.out + ((uint64_t)u << P) + cl * 16) = acc;
     if (tl == 0) {
-        const float prev = v.est[u];
-        if (accumulate(v, u, prev, curr)) c.inc += 1;
+        if (accumulate(v, u, prev, curr, dist0, inv0)) c.inc += 1;
         v.est[u] = curr;
         if (changed) {
             atomicOr(v.bitsW + (u >> 5), 1u << (u & 31u));
