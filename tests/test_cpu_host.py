@@ -4,6 +4,7 @@ HyperBallInner validates its arguments.  No GPU compute calls."""
 import ctypes as C
 import os
 import re
+import sys
 
 import numpy as np
 import pytest
@@ -279,6 +280,42 @@ def test_drop_cache_follows_the_reference_quirks(tmp_path):
     with pytest.raises(FileNotFoundError):
         cache.drop_cache(hb, d, gid)  # the second slot names seq 1 again
     assert os.path.exists(cache.cache_file(d, "closeness", gid, 2))
+
+
+def test_bench_algorithmic_bytes_per_unit():
+    """The roofline's algorithmic bytes, built up from the arrays a recomputed node
+    touches: its two u32 offsets, its 2^p-byte row read and written, its f32 estimate
+    and two f64 accumulators read and written; per neighbour a u32 id and, when the
+    neighbour's block is gathered, 2^p bytes."""
+    sys.path.insert(0, ROOT)
+    import bench
+    u32, f32, f64 = (np.dtype(t).itemsize for t in (np.uint32, np.float32, np.float64))
+    for p in (4, 7, 10):
+        R = 1 << p
+        node = 2 * u32 + 2 * R + 2 * f32 + 2 * 2 * f64
+        # a dense iteration: one node, three neighbours, all gathered
+        assert bench.algorithmic_bytes(0, 3, 1, 1, p, 3) == node + 3 * (u32 + R)
+        # selective: two of the three neighbours changed and are gathered
+        assert bench.algorithmic_bytes(1, 3, 1, 1, p, 2) == node + 3 * u32 + 2 * R
+        # iteration 1 hashes ids: no neighbour block and not the node's own block is read
+        assert bench.algorithmic_bytes(0, 3, 1, 1, p, 0, first_nodes=1) == node - R + 3 * u32
+        # hub chunks: one partial sketch written per task besides their gathers
+        assert bench.algorithmic_bytes(3, 10, 0, 2, p, 10) == 10 * (u32 + R) + 2 * R
+    # the dense R-MAT-24 p = 7 iteration quoted beside the kernels
+    assert abs(bench.algorithmic_bytes(0, 520768654, 16777216, 0, 7, 520768654) - 73.8e9) < 0.1e9
+
+
+def test_bench_traffic_files_match_kernel_workload_and_layout():
+    """The bench line's `traffic` comes from the committed counter summary of the same
+    kernel, workload, precision and node layout, and from no other."""
+    sys.path.insert(0, ROOT)
+    import bench
+    t, src = bench.measured_traffic("hb::k_union<7, 8>", "rmat24", 7, "degree")
+    assert t > 0 and src.endswith("k_union_p7_t8_rmat24_traffic.txt")
+    assert bench.measured_traffic("hb::k_union<7, 8>", "rmat24", 7, "chain") == (None, None)
+    t_chain, src = bench.measured_traffic("hb::k_union<6, 4>", "kmer", 6, "chain")
+    t_degree, _ = bench.measured_traffic("hb::k_union<6, 4>", "kmer", 6, "degree")
+    assert src.endswith("kmer_chain_traffic.txt") and 0 < t_chain < t_degree
 
 
 def test_bench_refuses_more_than_one_gpu():
