@@ -595,7 +595,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
-              "sharding of HyperBall is a design, not part of this build", file=sys.stderr)
+              "sharding of HyperBall is a design, not implemented here", file=sys.stderr)
         sys.exit(2)
 
     from sparkx_amd import HyperBall, _lib

@@ -56,7 +56,7 @@ public:
         return HyperBall(g, 0, 0u, device);
     }
     // HyperBallInner::get_or_compute (:165-201): reopens the registers
-    // (hyperballregisters_<P>_<id>.mmap, this build's own file: the reference's
+    // (hyperballregisters_<P>_<id>.mmap, this library's own file: the reference's
     // counters file holds hyperloglog-rs structs whose layout is not available) and the
     // accumulators (hyperballdistances_<id>.mmap, hyperballinvdistances_<id>.mmap) from
     // cache_dir, then searches the centrality caches; without them it runs and saves

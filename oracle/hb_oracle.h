@@ -1,7 +1,7 @@
 /*
  * hb_oracle.h -- CPU restatement of the reference HyperBall path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * TEST INFRASTRUCTURE ONLY.  The tests check the HIP path against this library: only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
  * product path (sparkx_amd/libsparkx_hb.so) never links or calls it.
  *

@@ -1,6 +1,6 @@
 """ctypes front-end of the CPU oracle (libhb_oracle.so).
 
-TEST INFRASTRUCTURE ONLY: the checker for the HIP path.  Only tests/,
+TEST INFRASTRUCTURE ONLY: the tests check the HIP path against it.  Only tests/,
 __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module.
 
 Each function mirrors a reference routine (paths relative to the reference

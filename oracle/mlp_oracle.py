@@ -1,8 +1,8 @@
 """CPU reference of the surrogate MLP (6 -> 64 -> 64 -> 1, GELU).
 
-TEST INFRASTRUCTURE ONLY: the checker of the HIP training and inference kernels
-(sparkx_amd/csrc/hb_mlp.hip).  No reference counterpart: the reference has no
-learned component; the model is the one the build's specification names.
+TEST INFRASTRUCTURE ONLY: the tests check the HIP training and inference kernels
+(sparkx_amd/csrc/hb_mlp.hip) against it.  No reference counterpart: the reference has no
+learned component; the model is the surrogate network those kernels train.
 
 Arithmetic of the GPU model, restated here with float64 accumulation:
     xn = (x - mean) / std                       (f32)

@@ -3,8 +3,10 @@
 - HyperBallMany: one engine context (own stream) per (graph, p), all advanced at once
   by hb_run_many from a pool of host threads; every graph's results are exactly its
   own single-graph run's.
-- HyperBallBatch: the persistent work-queue kernel over (graph, degree-bin) tiles
-  (hb_batch_*), switched off in this build (hb_batch_run returns HB_ESTATE).
+- HyperBallBatch: the batched engine (hb_batch_*): every graph's CSR and registers in one
+  set of device arrays, advanced by persistent kernels per (precision, degree bin) over
+  the running graphs' tiles, one hipGraph replay per iteration; every graph's results are
+  exactly its own single-graph run's.
 - batched_workload: the seeded 64-graph workload (half Erdos-Renyi, half R-MAT, n
   log-uniform in [2^12, 2^18], average degree uniform in [4, 32], p uniform in {4..8}).
 """

@@ -10,7 +10,7 @@ centralities `hyperballcloseness_<seq>_<id>.mmap`, `hyperballharmonic_<seq>_<id>
 (seq 1: normalize=false, seq 2: normalize=true) and `hyperballlin_0_<id>.mmap`
 (src/centralities/hyper_ball.rs:203-246).  The counters file `hyperball_<id>.mmap`
 holds hyperloglog-rs structs whose byte layout is not available offline, so it is
-neither written nor read: the registers go to a file of this build's own,
+neither written nor read: the registers go to a file of this library's own,
 `hyperballregisters_<p>_<id>.mmap` (u8, node-major, the buffer the reference's
 counters file would hold), which the reference never opens.  The accumulator and
 centrality files interoperate; the reference's own get_or_compute (hyper_ball.rs:
@@ -76,7 +76,7 @@ def write_centrality(cache_dir: str, kind: str, gid: str, normalize: bool, value
 
 def write_results(hb, cache_dir: str, gid: str) -> list:
     """Writes the accumulators and every centrality `hb` has computed under the
-    reference's file names, and the registers under this build's own name; returns
+    reference's file names, and the registers under this library's own name; returns
     the paths written."""
     os.makedirs(cache_dir, exist_ok=True)
     dist, inv = hb._accumulators()

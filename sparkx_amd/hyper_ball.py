@@ -76,7 +76,7 @@ class HyperBall:
     def get_or_compute(cls, g: Graph, cache_dir: str, gid: str, p: Optional[int] = None, **kw) -> "HyperBall":
         """HyperBallInner::get_or_compute (hyper_ball.rs:165-201).  When cache_dir holds
         the registers and both accumulators of graph `gid` (hyperballregisters_<p>_<id>,
-        this build's own file -- the reference's counters file holds hyperloglog-rs
+        this library's own file -- the reference's counters file holds hyperloglog-rs
         structs whose layout is not available, so it is neither written nor read --
         and hyperballdistances_<id> / hyperballinvdistances_<id>, the reference's
         names), the run is rebuilt from them on the device and every cached

@@ -1,4 +1,4 @@
-"""CPU tests: pin the oracle (the checker of the HIP path) against the
+"""CPU tests: pin the oracle (what the HIP path is checked against) against the
 reference's fixtures, published numbers and the mathematics, and check the
 host-side logic.  No GPU calls."""
 import functools
