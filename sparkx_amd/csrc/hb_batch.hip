@@ -1,8 +1,11 @@
 // hb_batch.hip -- many independent HyperBall runs (graphs of different sizes and
 // precisions p = 4..8) advanced together.  One iteration of the whole batch is
-//     k_batch_begin -> k_batch_union<4..8> -> k_batch_merge<p> (precisions with hubs)
-//     -> k_batch_finalize
-// captured once in a hipGraph and replayed until every graph has stopped.
+//     k_batch_begin -> k_batch_compact -> per precision P, as parallel branches (the
+//     precisions touch disjoint graphs): k_batch_nodes<P, T> for the three bins,
+//     k_batch_chunk<P>, k_batch_merge<P> -> k_batch_finalize
+// captured in a hipGraph and replayed until every graph has stopped; a precision whose
+// graphs have all stopped is left out of the replay (one captured iteration per set of
+// running precisions, kept until the schedule changes).
 // The work of one precision P comes in four classes: node tiles of the three degree
 // bins (team width G, 4G or 64 lanes per node, G = 2^P / 16) and chunks of hub
 // neighbour lists, whose partial unions go to scratch and are merged into their hubs
