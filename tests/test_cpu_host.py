@@ -75,6 +75,8 @@ def test_create_reports_errors_without_crashing():
     assert L.hb_run(None, 6, 0, 0, None) == _lib.HB_EINVAL
     assert L.hb_stage_csr(None, None, 8, 0, None, 8, 0) == _lib.HB_EINVAL
     assert L.hb_read_registers(None, 0, None) == _lib.HB_EINVAL
+    assert L.hb_restore(None, 6, None, None, None) == _lib.HB_EINVAL
+    assert b"hb_restore" in L.hb_last_error()
     assert L.hb_num_nodes(None) == 0
     L.hb_destroy(None)
 

@@ -619,6 +619,42 @@ def test_results_written_under_reference_cache_names(hb, oracle, tmp_path):
     assert np.array_equal(cache.read_result(str(tmp_path), "harmonic", "g1", False), hh)
 
 
+def test_restore_errors_and_repeats(hb, oracle):
+    """hb_restore rejects a precision outside 4..16, missing buffers and a context with
+    no graph, leaving the context usable; restoring twice on one context gives the
+    saved run both times."""
+    import ctypes as C
+    from sparkx_amd import _lib
+    L = _lib.load()
+    g = hb.Graph.rmat(11, 8, seed=17)
+    p = 6
+    ref = _oracle_run(oracle, g, p)
+    regs = ref.regs_ref.copy()
+    dist, inv = ref.dist.copy(), ref.inv.copy()
+    ctx = C.c_void_p()
+    _lib.check(L.hb_create(C.byref(ctx), 0))
+    try:
+        assert L.hb_restore(ctx, p, _lib.ptr(regs, C.c_uint8), _lib.ptr(dist, C.c_double),
+                            _lib.ptr(inv, C.c_double)) == _lib.HB_ESTATE  # nothing staged
+    finally:
+        L.hb_destroy(ctx)
+    with hb.HyperBall(g, p, _run=False) as h:
+        assert L.hb_restore(h._ctx, 3, _lib.ptr(regs, C.c_uint8), _lib.ptr(dist, C.c_double),
+                            _lib.ptr(inv, C.c_double)) == _lib.HB_EINVAL
+        assert L.hb_restore(h._ctx, 17, _lib.ptr(regs, C.c_uint8), _lib.ptr(dist, C.c_double),
+                            _lib.ptr(inv, C.c_double)) == _lib.HB_EINVAL
+        assert L.hb_restore(h._ctx, p, None, _lib.ptr(dist, C.c_double), _lib.ptr(inv, C.c_double)) == _lib.HB_EINVAL
+        assert L.hb_restore(h._ctx, p, _lib.ptr(regs, C.c_uint8), None, _lib.ptr(inv, C.c_double)) == _lib.HB_EINVAL
+        for _ in range(2):
+            _lib.check(L.hb_restore(h._ctx, p, _lib.ptr(regs, C.c_uint8), _lib.ptr(dist, C.c_double),
+                                    _lib.ptr(inv, C.c_double)))
+            h._ran(None)
+            assert np.array_equal(h.registers(_lib.HB_REGS_REFERENCE).ravel(), ref.regs_ref)
+            d, i = h._accumulators()
+            assert np.array_equal(d, ref.dist) and np.array_equal(i, ref.inv)
+            assert np.array_equal(h.estimates(), oracle.estimates(p, ref.regs_ref))
+
+
 @pytest.mark.parametrize("layout", ["auto", "input"])
 def test_get_or_compute_reopens_cached_results(hb, oracle, tmp_path, layout):
     """get_or_compute (hyper_ball.rs:165-201): the first call runs and saves registers
