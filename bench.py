@@ -83,7 +83,12 @@ def algorithmic_bytes(slot, arcs, nodes, items, p, gathered, first_nodes=0):
     return b
 
 
-def kernel_name(slot, p, team):
+def kernel_name(slot, p, team, layout):
+    """The kernel that does a slot's work in the dense iterations.  In the chain layout at
+    p = 5 and 6 the single bin of one-copy teams (an id range from node 0) is run by
+    k_union_rows after the first iteration, which k_union still runs."""
+    if slot == 0 and layout == "chain" and p in (5, 6) and team == (1 << p) // 16:
+        return f"hb::k_union_rows<{p}>"
     if slot < 3:
         return f"hb::k_union<{p}, {team}>"
     return f"hb::k_chunk<{p}>" if slot == 3 else f"hb::k_merge<{p}>"
@@ -666,8 +671,8 @@ def main():
                 # algorithmic bytes per second; it can exceed the HBM peak where gathers hit
                 # in L2 / the memory-side cache (the hub chunks do)
                 "GBps_algorithmic": round(slot_bytes[s] / (slot_ms[s] * 1e-3) / 1e9, 1) if slot_ms[s] else None}
-    dom_kernel = kernel_name(dom, p, int(st.kernel_team[dom]))
     layout_used = {v: k for k, v in HyperBall.LAYOUTS.items()}[int(st.layout)]
+    dom_kernel = kernel_name(dom, p, int(st.kernel_team[dom]), layout_used)
     traffic, traffic_src = measured_traffic(dom_kernel, args.config, p, layout_used)
     all_ms = sum(slot_ms)
     iter_gbs = sum(slot_bytes) / (all_ms * 1e-3) / 1e9 if all_ms else 0.0

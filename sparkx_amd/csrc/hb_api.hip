@@ -793,6 +793,7 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
     d.lc_thr = kLcThreshold[p];
     d.frontier = (flags & HB_FLAG_DENSE) ? 0 : 1;
     d.window = c->relabelled() && c->layout_used == HB_LAYOUT_CHAIN ? 1 : 0;
+    d.rows = d.window && (p == 5 || p == 6) ? 1 : 0;
 
     HbIterState st0{};
     st0.iter = 1;

@@ -88,6 +88,7 @@ struct HbDev {
     double lc_thr;         // HLL++ linear-counting threshold for p
     int frontier;
     int window;            // dense bin-0 iterations read in-window neighbours from LDS (chain layout)
+    int rows;              // ... and, at p = 5 or 6, run one node per lane (k_union_rows)
 };
 
 // ---------------------------------------------------------------------------
@@ -333,6 +334,49 @@ __device__ __forceinline__ float hb_estimate_copy(const HbTables &tb, uint4 acc)
             for (int q = 0; q < 4; ++q)
 #pragma unroll
                 for (int b = 0; b < 4; ++b) S = __fadd_rn(S, hb_pow2neg((w[q] >> (8 * b)) & 0xffu));
+        }
+    }
+    float est = (float)((double)tb.am / (double)S);
+    if (est <= tb.five_m) est = (float)((double)est - hb_bias_lookup(tb.bias, est));
+    return est;
+}
+
+// The same estimate of a whole sketch held by one lane (W = 2^P/16 words, registers in
+// order).  k sums at most 16 W terms of at most 2^24, so it cannot wrap, and k < 2^24
+// exactly when the saturating sum of hb_estimate_copy is below 2^24: the same bits.
+template <int W>
+__device__ __forceinline__ float hb_estimate_row(const HbTables &tb, const uint4 *acc)
+{
+    uint32_t z = 0, mx = 0;
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+        const uint4 a = acc[q];
+        z += hb_zero_bytes(a.x) + hb_zero_bytes(a.y) + hb_zero_bytes(a.z) + hb_zero_bytes(a.w);
+        mx = max(mx, hb_max_byte(hb_bmax(hb_bmax(a.x, a.y), hb_bmax(a.z, a.w))));
+    }
+    if (z > 0) {
+        const float lc = tb.lc[z];
+        if ((double)lc <= tb.lc_thr) return lc;
+    }
+    uint32_t k = 0;
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+        const uint4 a = acc[q];
+        k += hb_scaled_terms(a.x, mx) + hb_scaled_terms(a.y, mx) + hb_scaled_terms(a.z, mx) + hb_scaled_terms(a.w, mx);
+    }
+    float S;
+    if (k < (1u << 24)) {
+        S = ldexpf((float)k, -(int)mx);
+    } else {
+        S = 0.0f;  // the reference's sequential f32 loop over registers 0..m-1
+#pragma unroll
+        for (int q = 0; q < W; ++q) {
+            const uint4 a = acc[q];
+            const uint32_t w[4] = {a.x, a.y, a.z, a.w};
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) S = __fadd_rn(S, hb_pow2neg((w[i] >> (8 * b)) & 0xffu));
         }
     }
     float est = (float)((double)tb.am / (double)S);

@@ -9,6 +9,7 @@
 // Launch sequence per iteration, all reading the device-resident HbIterState so
 // that one captured hipGraph serves every iteration:
 //     k_begin (clear next changed-bitmap)  ->  k_union<P,T> per degree bin
+//     (k_union_rows<P> for the dense iterations of a chain-layout id range at p = 5, 6)
 //     ->  k_chunk<P> (hub neighbour chunks) -> k_merge<P> (hubs) -> [k_nf] -> k_finalize
 // Frontier: if no neighbour of u changed at t-1 then c_t(u) = c_{t-1}(u) (union
 // of unchanged sets), so u is skipped; when u itself changed at t-1 its row is
@@ -23,6 +24,7 @@
 namespace hb {
 
 static constexpr int kBlock = 256;
+
 
 // block-wide sums of per-lane counters, one atomic per counter per block
 __device__ __forceinline__ void block_add5(unsigned long long a, unsigned long long b, unsigned long long c,
@@ -291,6 +293,177 @@ __global__ __launch_bounds__(kBlock) void k_arc_fingerprint(const uint32_t *off,
 }
 
 // ---------------------------------------------------------------------------
+// dense iterations of an id-range bin in the chain layout at p = 5 or 6: one lane per
+// node, its whole 2^p-byte row in the lane's registers (W = 2^p / 16 words)
+// ---------------------------------------------------------------------------
+// With a team of G = 2^p / 16 lanes per node (union_node_window) a wavefront holds 16
+// nodes at p = 6, and the lanes of a team repeat each other's per-node work: offsets,
+// addresses, the estimate.  On a graph of long low-degree chains that union is bound by
+// each node's chain of dependent memory accesses, not by bytes, so what counts is how
+// many nodes are in flight.  Here a wavefront holds 64 consecutive nodes lo..lo+63:
+//   1. their rows c_{t-1} enter the wavefront's LDS window by coalesced 16-byte loads;
+//   2. each lane takes its node's row from the window and merges its neighbours' rows,
+//      from the window when the neighbour is one of the 64, else from memory;
+//   3. the new rows go back through the window and out by coalesced stores;
+//   4. estimate and accumulators per lane; the changed bits of the 64 nodes are ORed
+//      into the bitmap with one atomic per 32-bit word.
+// Same registers, estimates and accumulators as union_node.
+
+// an iteration whose id-range bins k_union_rows runs instead of k_union: dense (no
+// frontier gathers, no push list) and after the first
+__device__ __forceinline__ bool rows_iteration(const HbDev &d, const HbIterState &st)
+{
+    return st.iter > 1 && !st.push && !(d.frontier && !st.skip_scan);
+}
+
+// The window of a wavefront is W planes of 64 uint4: plane q holds chunk q (bytes
+// 16q .. 16q + 15) of the 64 rows, row w at position w ^ row_perm(q).  The permutation
+// keeps a wavefront's structured accesses free of LDS bank conflicts: a lane reading
+// chunk q of its own row, and the coalesced copies (chunk q of rows 64/W apart per
+// lane group), land in distinct 16-byte bank groups for 8 and for 16 lanes per cycle.
+template <int W>
+__device__ __forceinline__ uint32_t row_perm(uint32_t q)
+{
+    return W == 4 ? (0xCA60u >> (4 * q)) & 15u : (0xC0u >> (4 * q)) & 15u;  // {0,6,10,12} / {0,12}
+}
+
+template <int W>
+__device__ __forceinline__ uint32_t win_index(uint32_t w, uint32_t q)
+{
+    return q * HB_WAVE + (w ^ row_perm<W>(q));
+}
+
+// row c_{t-1}(x): from the window when x is one of its nodes lo .. lo + span - 1
+template <int P>
+__device__ __forceinline__ void row_fetch(uint4 *r, uint32_t x, uint32_t lo, uint32_t span, const uint4 *win,
+                                          const uint8_t *__restrict__ in)
+{
+    constexpr int W = (1 << P) / 16;
+    if (x - lo < span) {
+#pragma unroll
+        for (int q = 0; q < W; ++q) r[q] = win[win_index<W>(x - lo, q)];
+    } else {
+        const uint4 *g = (const uint4 *)(in + ((uint64_t)x << P));
+#pragma unroll
+        for (int q = 0; q < W; ++q) r[q] = g[q];
+    }
+}
+
+// Six wavefronts per SIMD (80 VGPRs) is the most that runs without spills at p = 6: 384
+// nodes in flight per SIMD against 128 for union_node_window at eight.  On the k-mer graph
+// (p = 6) a launch takes 0.367 ms against 0.583 for union_node_window; at seven (4 VGPRs
+// spilled) 0.362 ms, at eight (16 spilled) 0.460 ms.  The bin's base must be a multiple of 64.
+template <int P>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_union_rows(
+    HbDev d, uint32_t base, uint32_t count, int slot_id)
+{
+    constexpr int W = (1 << P) / 16;
+    static_assert(W == 2 || W == 4, "p = 5 or 6");
+    const HbIterState *st = d.st;
+    if (st->done || !rows_iteration(d, *st)) return;
+    HB_TABLES_LDS(P, d, tb)
+    __shared__ uint4 s_win[kBlock * W];  // 64 rows per wavefront
+    const uint32_t it = st->iter;
+    const uint32_t lane = threadIdx.x % HB_WAVE;
+    uint4 *win = s_win + (threadIdx.x / HB_WAVE) * (HB_WAVE * W);
+    HbView v{};
+    v.in = st->parity ? d.regs1 : d.regs0;
+    v.out = st->parity ? d.regs0 : d.regs1;
+    v.dist = d.dist;
+    v.inv = d.inv;
+    v.t = st->t;
+    v.inv_t = st->inv_t;
+    const uint8_t *__restrict__ in = v.in;
+    const uint32_t *__restrict__ nb = d.nbr;
+    uint32_t *bitsW = (it & 1u) ? d.bits1 : d.bits0;
+    // chunk r * 64 + lane of the wavefront's rows (a coalesced copy) is chunk lane % W of
+    // row r * 64 / W + lane / W, at window index cidx + r * 64 / W (the permutation only
+    // touches the low 4 bits of a row, below 64 / W)
+    constexpr uint32_t RPC = HB_WAVE / W;  // rows per coalesced copy instruction
+    const uint32_t lw = lane / W;
+    const uint32_t cidx = win_index<W>(lw, lane % W);
+    unsigned long long n_inc = 0, n_arcs = 0, n_nodes = 0, n_chg = 0;  // the wavefront's counts
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / HB_WAVE);
+    const uint32_t stride = __builtin_amdgcn_readfirstlane(gridDim.x) * kBlock;
+    for (uint32_t w0 = (blockIdx.x * (kBlock / HB_WAVE) + wave) * HB_WAVE; w0 < count; w0 += stride) {
+        const uint32_t lo = base + w0, span = min((uint32_t)HB_WAVE, count - w0);
+        const bool valid = lane < span;
+        const uint32_t u = lo + lane;
+        uint32_t e0 = 0, e1 = 0;
+        float prev = 0.0f;
+        if (valid) {
+            e0 = d.off[u];
+            e1 = d.off[u + 1];
+            prev = d.est[u];
+        }
+        const uint4 *src = (const uint4 *)(in + ((uint64_t)lo << P));
+#pragma unroll
+        for (int r = 0; r < W; ++r)
+            if ((int)lw < (int)span - r * (int)RPC) win[cidx + r * RPC] = src[r * HB_WAVE + lane];
+        // LDS operations of one wavefront complete in order; the barrier and fence keep
+        // the compiler from moving window accesses across each other
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        uint4 acc[W];
+#pragma unroll
+        for (int q = 0; q < W; ++q) acc[q] = valid ? win[win_index<W>(lane, q)] : make_uint4(0, 0, 0, 0);
+        uint32_t e = e0;
+        for (; e < e1; ++e) {
+            uint4 r0[W];
+            row_fetch<P>(r0, nb[e], lo, span, win, in);
+#pragma unroll
+            for (int q = 0; q < W; ++q) acc[q] = hb_bmax4(acc[q], r0[q]);
+        }
+        double dist0 = 0.0, inv0 = 0.0;
+        if (valid) {
+            dist0 = d.dist[u];
+            inv0 = d.inv[u];
+        }
+        __builtin_amdgcn_wave_barrier();  // every lane has read the window
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        uint32_t diffw = 0;
+#pragma unroll
+        for (int q = 0; q < W; ++q) {
+            const uint4 o = win[win_index<W>(lane, q)];
+            diffw |= (acc[q].x != o.x) | (acc[q].y != o.y) | (acc[q].z != o.z) | (acc[q].w != o.w);
+            win[win_index<W>(lane, q)] = acc[q];
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        uint4 *dst = (uint4 *)(v.out + ((uint64_t)lo << P));
+#pragma unroll
+        for (int r = 0; r < W; ++r)
+            if ((int)lw < (int)span - r * (int)RPC) dst[r * HB_WAVE + lane] = win[cidx + r * RPC];
+        const float curr = hb_estimate_row<W>(tb, acc);
+        const bool changed = valid && diffw != 0;
+        bool inc = false;
+        if (valid) {
+            inc = accumulate(v, u, prev, curr, dist0, inv0);
+            d.est[u] = curr;
+        }
+        // counts: nodes lo .. lo + span - 1 hold arcs off[lo] .. off[lo + span] - 1
+        n_nodes += span;
+        n_arcs += (uint32_t)__builtin_amdgcn_readlane((int)e1, (int)span - 1) -
+                  (uint32_t)__builtin_amdgcn_readfirstlane((int)e0);
+        n_inc += __popcll(__ballot(inc));
+        // the changed bits of nodes lo .. lo + 63 (lo a multiple of 64): two bitmap words,
+        // ORed atomically since the last may also hold bits of nodes past the bin
+        const unsigned long long mask = __ballot(changed);
+        n_chg += __popcll(mask);
+        if (lane < 2) {
+            const uint32_t bits = (uint32_t)(lane ? mask >> 32 : mask);
+            if (bits) atomicOr(bitsW + (lo >> 5) + lane, bits);
+        }
+        // the next trip's window writes follow this trip's reads
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    }
+    const bool first_lane = lane == 0;
+    block_add5(first_lane ? n_inc : 0, first_lane ? n_arcs : 0, first_lane ? n_nodes : 0, first_lane ? n_chg : 0,
+               first_lane ? n_arcs : 0, d.rec + it % HB_REC_RING, slot_id);
+}
+
+// ---------------------------------------------------------------------------
 // union over the neighbours of listed nodes; a team of T lanes per node
 // (T = G: one sketch copy; T > G: S = T/G copies split the neighbours)
 // ---------------------------------------------------------------------------
@@ -307,6 +480,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(P >= 7 ?
 {
     const HbIterState *st = d.st;
     if (st->done) return;
+    if constexpr (T == (1 << P) / 16 && (P == 5 || P == 6))
+        if (d.rows && !list && (base & 63u) == 0 && rows_iteration(d, *st)) return;  // k_union_rows runs it
     HB_TABLES_LDS(P, d, tb)
     const uint32_t it = st->iter;
     if (st->push) {  // only the bin's nodes with a changed neighbour (k_push_compact)
@@ -336,9 +511,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(P >= 7 ?
     const uint32_t tl = threadIdx.x % T;
     const uint32_t teams_per_grid = gridDim.x * (kBlock / T);
     HbCounts c;
-    if constexpr (T == (1 << P) / 16) {
+    if constexpr (T == (1 << P) / 16 && P != 5 && P != 6) {
         // dense iteration over an id-range bin: each wavefront's teams hold consecutive
-        // nodes, and neighbours among them come from the wavefront's LDS window
+        // nodes, and neighbours among them come from the wavefront's LDS window (at p = 5
+        // and 6 k_union_rows runs these iterations)
         if (d.window && !list && !v.first && !v.gather_bits) {
             uint4 *win = reinterpret_cast<uint4 *>(s_first) + (threadIdx.x / HB_WAVE) * HB_WAVE;
             const uint32_t in_wave = (threadIdx.x % HB_WAVE) / T;
@@ -858,6 +1034,15 @@ hipError_t launch_iteration(const HbDev &d, const HbSchedule &sc, bool nf, hipSt
                 e = hipGetLastError();
             });
         } else {
+            // k_union_rows and k_union each return at once in the iterations the other runs
+            if (d.rows && !sc.bin_list[b] && (sc.bin_base[b] & 63u) == 0 &&
+                sc.bin_team[b] == (int)((1u << d.p) / 16)) {
+                const unsigned grid = grid_for(sc.bin_count[b], sc.max_blocks);
+                if (d.p == 5) k_union_rows<5><<<grid, kBlock, 0, s>>>(d, sc.bin_base[b], sc.bin_count[b], b);
+                else k_union_rows<6><<<grid, kBlock, 0, s>>>(d, sc.bin_base[b], sc.bin_count[b], b);
+                e = hipGetLastError();
+                if (e != hipSuccess) return e;
+            }
             HB_DISPATCH_P(d.p, e = launch_union_T<P>(d, sc.bin_list[b], sc.bin_base[b], sc.bin_count[b],
                                                      sc.bin_team[b], sc.max_blocks, b, s));
         }

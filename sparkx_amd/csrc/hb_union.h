@@ -239,7 +239,8 @@ __device__ __forceinline__ void union_node(const HbView &v, const HbTables &tb, 
 // neighbour inside [lo, hi) is read from there instead of memory: in a numbering that
 // keeps chains together (the chain layout) most neighbours are in the window, and the
 // union is bound by the rate of memory requests, not by bytes.  Same registers,
-// estimates and accumulators as union_node.
+// estimates and accumulators as union_node.  (At p = 5 and 6 the single-graph engine runs
+// such iterations with one node per lane instead: k_union_rows.)
 template <int P, int T>
 __device__ __forceinline__ void union_node_window(const HbView &v, const HbTables &tb, uint32_t u, uint32_t tl,
                                                   HbCounts &c, uint32_t lo, uint32_t hi, uint4 *win)

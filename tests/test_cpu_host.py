@@ -315,9 +315,23 @@ def test_bench_traffic_files_match_kernel_workload_and_layout():
     t, src = bench.measured_traffic("hb::k_union<7, 8>", "rmat24", 7, "degree")
     assert t > 0 and src.endswith("k_union_p7_t8_rmat24_traffic.txt")
     assert bench.measured_traffic("hb::k_union<7, 8>", "rmat24", 7, "chain") == (None, None)
-    t_chain, src = bench.measured_traffic("hb::k_union<6, 4>", "kmer", 6, "chain")
+    t_chain, src = bench.measured_traffic("hb::k_union_rows<6>", "kmer", 6, "chain")
     t_degree, _ = bench.measured_traffic("hb::k_union<6, 4>", "kmer", 6, "degree")
-    assert src.endswith("kmer_chain_traffic.txt") and 0 < t_chain < t_degree
+    assert src.endswith("k_union_rows_p6_kmer_chain_traffic.txt") and 0 < t_chain < t_degree
+    assert bench.measured_traffic("hb::k_union<6, 4>", "kmer", 6, "chain") == (None, None)
+
+
+def test_bench_names_the_kernel_that_runs_the_dense_iterations():
+    """In the chain layout at p = 5 and 6 the one-copy bin is run by k_union_rows after
+    the first iteration; everywhere else the slot's kernel is k_union<p, team>."""
+    sys.path.insert(0, ROOT)
+    import bench
+    assert bench.kernel_name(0, 6, 4, "chain") == "hb::k_union_rows<6>"
+    assert bench.kernel_name(0, 5, 2, "chain") == "hb::k_union_rows<5>"
+    assert bench.kernel_name(0, 6, 4, "degree") == "hb::k_union<6, 4>"
+    assert bench.kernel_name(0, 7, 8, "chain") == "hb::k_union<7, 8>"
+    assert bench.kernel_name(0, 6, 8, "chain") == "hb::k_union<6, 8>"
+    assert bench.kernel_name(3, 7, 64, "degree") == "hb::k_chunk<7>"
 
 
 def test_bench_refuses_more_than_one_gpu():

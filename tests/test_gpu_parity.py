@@ -811,8 +811,9 @@ def _chain_graph(hb, kind):
 @pytest.mark.parametrize("p", [4, 5, 6, 7, 8, 10])
 def test_chain_layout_window_matches_oracle(hb, oracle, kind, p):
     """Chain numbering, dense iterations: the union reads neighbours inside each
-    wavefront's window of consecutive nodes from LDS (team widths 1..64).  Default and
-    dense schedules, hipGraph replay included, equal the oracle bit for bit."""
+    wavefront's window of consecutive nodes from LDS (team widths 1..64; at p = 5 and 6
+    one node per lane, k_union_rows, with a 64-node window).  Default and dense
+    schedules, hipGraph replay included, equal the oracle bit for bit."""
     g = _chain_graph(hb, kind)
     say("chain window", kind, p, g.size(), g.width())
     ref = _oracle_run(oracle, g, p, max_iters=300)
