@@ -33,7 +33,8 @@ typedef enum {
                                  neighbours changed, with identical results      */
 #define HB_FLAG_GRAPH 0x2u    /* capture one iteration in a hipGraph and replay it   */
 #define HB_FLAG_NF    0x4u    /* record the neighbourhood function N(t) each iteration */
-#define HB_FLAG_TIMING 0x8u   /* per-iteration device timing (adds a sync per iteration) */
+#define HB_FLAG_TIMING 0x8u   /* per-iteration device timing with events (read at the
+                                 host's usual checks: no extra sync)             */
 #define HB_FLAG_KTIMING 0x10u /* per-kernel device timing with events (implies TIMING;
                                  not combinable with HB_FLAG_GRAPH)              */
 

@@ -821,12 +821,18 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
         (void)hipEventDestroy(e2);
         return fail(HB_EINVAL, "hb_run: HB_FLAG_KTIMING cannot be combined with HB_FLAG_GRAPH");
     }
-    hipEvent_t kev[2 * HB_NSLOT] = {};
+    // timing: each launch of a check's batch has its own pair of events around the
+    // iteration and, with HB_FLAG_KTIMING, its own events around each kernel slot; they
+    // are read after the check, so timing adds no host round trip per iteration
+    constexpr uint32_t kMaxBatch = 8;  // launches between two checks
+    const uint32_t per_it = 2 + (ktiming ? 2 * HB_NSLOT : 0);
+    std::vector<hipEvent_t> tev(timing ? (size_t)kMaxBatch * per_it : 0, nullptr);
     double kms[HB_NSLOT] = {};
     uint64_t klaunch[HB_NSLOT] = {};
-    if (ktiming)
-        for (auto &ev : kev) (void)hipEventCreate(&ev);
     do {
+        bool events_ok = true;
+        for (auto &ev : tev) events_ok = events_ok && hipEventCreate(&ev) == hipSuccess;
+        if (!events_ok) { rc = fail(HB_EHIP, "event create failed"); break; }
         if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fail(HB_EHIP, "event record failed"); break; }
         hipError_t e = hb::launch_init(d, c->stream);
         if (e != hipSuccess) { rc = fail(HB_EHIP, "init kernel: %s", hipGetErrorString(e)); break; }
@@ -856,27 +862,20 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
         HbIterRec ring[HB_REC_RING];
         c->h_rec.assign(1, HbIterRec{});
         for (;;) {
-            const uint32_t batch = timing ? 1u : (launched < 8 ? 2u : 8u);
+            const uint32_t k = launched < 8 ? 2u : kMaxBatch;
             if (launched >= bound + 16) {
                 rc = fail(HB_ESTATE, "hb_run: still changing after %llu iterations", (unsigned long long)launched);
                 break;
             }
-            const uint32_t k = batch;
-            hipEvent_t ib = nullptr, ie = nullptr;
-            if (timing) {
-                (void)hipEventCreate(&ib);
-                (void)hipEventCreate(&ie);
-                (void)hipEventRecord(ib, c->stream);
-            }
             for (uint32_t i = 0; i < k && rc == HB_OK; ++i) {
+                hipEvent_t *ev = timing ? &tev[(size_t)i * per_it] : nullptr;
+                if (ev) (void)hipEventRecord(ev[0], c->stream);
                 e = gexec ? hipGraphLaunch(gexec, c->stream)
-                          : hb::launch_iteration(d, c->sched, nf, c->stream, ktiming ? kev : nullptr);
+                          : hb::launch_iteration(d, c->sched, nf, c->stream, ktiming ? ev + 2 : nullptr);
                 if (e != hipSuccess) rc = fail(HB_EHIP, "iteration launch: %s", hipGetErrorString(e));
+                if (ev) (void)hipEventRecord(ev[1], c->stream);
             }
             launched += k;
-            if (timing) {
-                (void)hipEventRecord(ie, c->stream);
-            }
             if (rc != HB_OK) break;
             if (hipMemcpyAsync(&hs, c->st.ptr, sizeof hs, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
                 hipMemcpyAsync(ring, c->rec.ptr, sizeof ring, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
@@ -888,19 +887,21 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
             for (uint32_t t = drained + 1; t <= finished; ++t) c->h_rec.push_back(ring[t % HB_REC_RING]);
             drained = finished;
             if (timing) {
-                float ms = 0.0f;
-                (void)hipEventElapsedTime(&ms, ib, ie);
-                const bool ran_now = !hs.done || hs.executed == launched;
-                if (ran_now) iter_ms.push_back(ms);
-                (void)hipEventDestroy(ib);
-                (void)hipEventDestroy(ie);
-                if (ktiming && ran_now) {
-                    const uint32_t counts[HB_NSLOT] = {c->sched.bin_count[0], c->sched.bin_count[1],
-                                                       c->sched.bin_count[2], c->sched.ntasks, c->sched.nhubs};
+                const uint32_t counts[HB_NSLOT] = {c->sched.bin_count[0], c->sched.bin_count[1],
+                                                   c->sched.bin_count[2], c->sched.ntasks, c->sched.nhubs};
+                for (uint32_t i = 0; i < k; ++i) {
+                    // launch number launched - k + i + 1 ran iteration launched - k + i + 1
+                    // unless the run had stopped before it (it then returned at once)
+                    if (hs.done && launched - k + i + 1 > hs.executed) break;
+                    const hipEvent_t *ev = &tev[(size_t)i * per_it];
+                    float ms = 0.0f;
+                    (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+                    iter_ms.push_back(ms);
+                    if (!ktiming) continue;
                     for (int sl = 0; sl < HB_NSLOT; ++sl) {
                         if (!counts[sl]) continue;
                         float kms1 = 0.0f;
-                        (void)hipEventElapsedTime(&kms1, kev[2 * sl], kev[2 * sl + 1]);
+                        (void)hipEventElapsedTime(&kms1, ev[2 + 2 * sl], ev[3 + 2 * sl]);
                         kms[sl] += kms1;
                         klaunch[sl] += 1;
                     }
@@ -917,8 +918,8 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
     } while (0);
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (graph) (void)hipGraphDestroy(graph);
-    if (ktiming)
-        for (auto &ev : kev) (void)hipEventDestroy(ev);
+    for (auto &ev : tev)
+        if (ev) (void)hipEventDestroy(ev);
     float ms_init = 0.0f, ms_loop = 0.0f;
     if (rc == HB_OK) {
         (void)hipEventElapsedTime(&ms_init, e0, e1);
