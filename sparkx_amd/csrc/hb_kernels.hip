@@ -349,12 +349,12 @@ __device__ __forceinline__ void row_fetch(uint4 *r, uint32_t x, uint32_t lo, uin
     }
 }
 
-// Six wavefronts per SIMD (80 VGPRs) is the most that runs without spills at p = 6: 384
-// nodes in flight per SIMD against 128 for union_node_window at eight.  On the k-mer graph
-// (p = 6) a launch takes 0.367 ms against 0.583 for union_node_window; at seven (4 VGPRs
-// spilled) 0.362 ms, at eight (16 spilled) 0.460 ms.  The bin's base must be a multiple of 64.
+// Seven wavefronts per SIMD (71 VGPRs at p = 6, no spill): 448 nodes in flight per SIMD
+// against 128 for union_node_window at eight wavefronts of 16.  On the k-mer graph (p = 6) a
+// launch takes 0.362 ms against 0.583 for union_node_window; at eight (64 VGPRs, spills
+// outside the node loop only) 0.365 ms.  The bin's base must be a multiple of 64.
 template <int P>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_union_rows(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) void k_union_rows(
     HbDev d, uint32_t base, uint32_t count, int slot_id)
 {
     constexpr int W = (1 << P) / 16;
@@ -414,11 +414,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
 #pragma unroll
             for (int q = 0; q < W; ++q) acc[q] = hb_bmax4(acc[q], r0[q]);
         }
-        double dist0 = 0.0, inv0 = 0.0;
-        if (valid) {
-            dist0 = d.dist[u];
-            inv0 = d.inv[u];
-        }
         __builtin_amdgcn_wave_barrier();  // every lane has read the window
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         uint32_t diffw = 0;
@@ -434,26 +429,25 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
 #pragma unroll
         for (int r = 0; r < W; ++r)
             if ((int)lw < (int)span - r * (int)RPC) dst[r * HB_WAVE + lane] = win[cidx + r * RPC];
-        const float curr = hb_estimate_row<W>(tb, acc);
-        const bool changed = valid && diffw != 0;
-        bool inc = false;
-        if (valid) {
-            inc = accumulate(v, u, prev, curr, dist0, inv0);
-            d.est[u] = curr;
-        }
         // counts: nodes lo .. lo + span - 1 hold arcs off[lo] .. off[lo + span] - 1
         n_nodes += span;
         n_arcs += (uint32_t)__builtin_amdgcn_readlane((int)e1, (int)span - 1) -
                   (uint32_t)__builtin_amdgcn_readfirstlane((int)e0);
+        const float curr = hb_estimate_row<W>(tb, acc);
+        const bool changed = valid && diffw != 0;
+        bool inc = false;
+        if (valid) {
+            inc = accumulate(v, u, prev, curr);  // reads the accumulators only if they change
+            d.est[u] = curr;
+        }
         n_inc += __popcll(__ballot(inc));
         // the changed bits of nodes lo .. lo + 63 (lo a multiple of 64): two bitmap words,
         // ORed atomically since the last may also hold bits of nodes past the bin
         const unsigned long long mask = __ballot(changed);
         n_chg += __popcll(mask);
-        if (lane < 2) {
-            const uint32_t bits = (uint32_t)(lane ? mask >> 32 : mask);
-            if (bits) atomicOr(bitsW + (lo >> 5) + lane, bits);
-        }
+        const uint32_t bits_lo = (uint32_t)mask, bits_hi = (uint32_t)(mask >> 32);
+        if (lane == 0 && bits_lo) atomicOr(bitsW + (lo >> 5), bits_lo);
+        if (lane == 1 && bits_hi) atomicOr(bitsW + (lo >> 5) + 1, bits_hi);
         // the next trip's window writes follow this trip's reads
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
