@@ -213,8 +213,9 @@ __device__ __forceinline__ void batch_chunk(const HbBatchDev &b, const HbView &v
 #define HB_BATCH_TABLES(P, b, name)                                                        \
     __shared__ float name##_lc[(1 << (P)) + 1];                                            \
     __shared__ HbBiasTable name##_bias;                                                    \
+    constexpr uint32_t name##_off = lc_off(P); /* folded here: a call would give the kernel a stack */ \
     for (uint32_t i_ = threadIdx.x; i_ <= (1u << (P)); i_ += blockDim.x)                   \
-        name##_lc[i_] = (b).lc[lc_off(P) + i_];                                            \
+        name##_lc[i_] = (b).lc[name##_off + i_];                                           \
     for (uint32_t i_ = threadIdx.x; i_ < sizeof(HbBiasTable) / 8; i_ += blockDim.x)       \
         ((uint64_t *)&name##_bias)[i_] = ((const uint64_t *)((b).bias + ((P) - kBatchPMin)))[i_]; \
     __syncthreads();
