@@ -1,8 +1,9 @@
 // hb_batch.hip -- many independent HyperBall runs (graphs of different sizes and
 // precisions p = 4..8) advanced together.  One iteration of the whole batch is
-//     k_batch_begin -> k_batch_compact -> per precision P, as parallel branches (the
-//     precisions touch disjoint graphs): k_batch_nodes<P, T> for the three bins,
-//     k_batch_chunk<P>, k_batch_merge<P> -> k_batch_finalize
+//     k_batch_begin -> k_batch_compact -> per precision P (the precisions touch
+//     disjoint graphs): k_batch_nodes<P, T> for the three bins, k_batch_chunk<P>,
+//     k_batch_merge<P>, the precisions' chains packed by their work into at most four
+//     parallel branches, one per hardware queue -> k_batch_finalize
 // captured in a hipGraph and replayed until every graph has stopped; a precision whose
 // graphs have all stopped is left out of the replay (one captured iteration per set of
 // running precisions, kept until the schedule changes).
@@ -48,6 +49,9 @@ namespace hb {
 
 static constexpr int kBatchBlock = 256;
 static constexpr int kBatchPMin = 4, kBatchPMax = 8, kBatchNP = kBatchPMax - kBatchPMin + 1;
+// side streams for the precisions' chains: one per hardware queue of the process (HIP's
+// default is 4); a fifth stream would share a queue with another chain and run after it
+static constexpr int kSide = 4;
 static constexpr uint32_t kRing = 32;    // per-graph iteration records on the device
 static constexpr uint32_t kCheck = 8;    // launches between host checks (< kRing)
 static constexpr uint32_t kScrRow = 256; // scratch bytes per hub chunk (2^p, p <= 8)
@@ -477,11 +481,11 @@ bool trace_on()
 struct hb_batch {
     int device = 0;
     hipStream_t stream = nullptr;
-    // the precisions touch disjoint graphs, so each precision's kernels run on a stream
-    // of their own between a fork after the tile compaction and a join before finalize
-    // (in the captured graph: parallel branches)
-    hipStream_t side[hb::kBatchNP] = {};
-    hipEvent_t fork = nullptr, join[hb::kBatchNP] = {};
+    // the precisions touch disjoint graphs, so their kernels run as up to kSide chains on
+    // streams of their own between a fork after the tile compaction and a join before
+    // finalize (in the captured graph: parallel branches)
+    hipStream_t side[hb::kSide] = {};
+    hipEvent_t fork = nullptr, join[hb::kSide] = {};
     // captured iterations, one per (precisions still running, dense flag); they depend
     // only on the batch's buffers and tiles, so they are kept until the tiles change
     std::vector<std::pair<uint32_t, hipGraphExec_t>> execs;
@@ -521,7 +525,7 @@ struct hb_batch {
     ~hb_batch()
     {
         drop_execs();
-        for (int i = 0; i < hb::kBatchNP; ++i) {
+        for (int i = 0; i < hb::kSide; ++i) {
             if (join[i]) (void)hipEventDestroy(join[i]);
             if (side[i]) (void)hipStreamDestroy(side[i]);
         }
@@ -674,7 +678,7 @@ int hb_batch_create(hb_batch **out, int device, const hb_graph_desc *graphs, int
     b->device = device;
     HB_BTRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     HB_BTRY(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
-    for (int i = 0; i < hb::kBatchNP; ++i) {
+    for (int i = 0; i < hb::kSide; ++i) {
         HB_BTRY(hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking));
         HB_BTRY(hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming));
     }
@@ -877,6 +881,10 @@ int hb_batch_run(hb_batch *b, uint32_t flags, hb_batch_stats *stats)
         has_work[pi] = hub_lists[pi].count != 0;
         for (int c = 0; c < hb::kClasses; ++c) has_work[pi] = has_work[pi] || lists[pi * hb::kClasses + c].count;
     }
+    // bytes of rows a dense iteration gathers per precision, to balance the side streams
+    uint64_t pwork[hb::kBatchNP] = {};
+    for (int i = 0; i < ng; ++i)
+        pwork[b->hg[i].p - hb::kBatchPMin] += ((uint64_t)b->h_off[i].back() + b->hg[i].n) << b->hg[i].p;
     // `running`: bit pi set while a graph of precision kBatchPMin + pi may still run.  A
     // stopped graph never runs again and its kernels would return at once, so leaving
     // out a precision whose graphs have all stopped changes nothing.
@@ -884,15 +892,24 @@ int hb_batch_run(hb_batch *b, uint32_t flags, hb_batch_stats *stats)
         hb::k_batch_begin<<<(unsigned)std::min(ng, 1024), hb::kBatchBlock, 0, b->stream>>>(d);
         hb::k_batch_compact<<<hb::kQueues, 1024, 0, b->stream>>>(d, classes);
         hipError_t e = hipEventRecord(b->fork, b->stream);
-        // per precision, on its own stream: the three node bins (team widths G, 4G capped
-        // at 64, and 64), the hub chunks, then the hubs
+        // the running precisions' chains (the three node bins of team widths G, 4G capped at
+        // 64, and 64, the hub chunks, then the hubs), heaviest first, each on the side
+        // stream with the least work so far
+        int order[hb::kBatchNP], side_of[hb::kBatchNP], np = 0;
+        for (int pi = 0; pi < hb::kBatchNP; ++pi)
+            if (has_work[pi] && ((running >> pi) & 1u)) order[np++] = pi;
+        std::stable_sort(order, order + np, [&](int x, int y) { return pwork[x] > pwork[y]; });
+        uint64_t load[hb::kSide] = {};
+        for (int k = 0; k < np; ++k) {
+            const int si = (int)(std::min_element(load, load + hb::kSide) - load);
+            side_of[order[k]] = si;
+            load[si] += pwork[order[k]];
+        }
 #define HB_BATCH_LAUNCH(q, kernel)                                                                     \
         if (lists[q].count) kernel<<<b->grid[q], hb::kBatchBlock, 0, s>>>(d, lists[q]);
 #define HB_BATCH_P(P)                                                                                  \
-        if (has_work[P - hb::kBatchPMin] && ((running >> (P - hb::kBatchPMin)) & 1u) && e == hipSuccess) { \
+        case P: {                                                                                      \
             constexpr int G_ = (1 << P) / 16, q0 = (P - hb::kBatchPMin) * hb::kClasses;               \
-            const hipStream_t s = b->side[P - hb::kBatchPMin];                                         \
-            e = hipStreamWaitEvent(s, b->fork, 0);                                                     \
             HB_BATCH_LAUNCH(q0, (hb::k_batch_nodes<P, G_>))                                            \
             HB_BATCH_LAUNCH(q0 + 1, (hb::k_batch_nodes<P, (4 * G_ < 64 ? 4 * G_ : 64)>))              \
             HB_BATCH_LAUNCH(q0 + 2, (hb::k_batch_nodes<P, 64>))                                        \
@@ -900,10 +917,21 @@ int hb_batch_run(hb_batch *b, uint32_t flags, hb_batch_stats *stats)
             if (hub_lists[P - hb::kBatchPMin].count)                                                   \
                 hb::k_batch_merge<P><<<(hub_lists[P - hb::kBatchPMin].count + 3) / 4, hb::kBatchBlock, 0, s>>>( \
                     d, hub_lists[P - hb::kBatchPMin]);                                                 \
-            if (e == hipSuccess) e = hipEventRecord(b->join[P - hb::kBatchPMin], s);                   \
-            if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->join[P - hb::kBatchPMin], 0);    \
+        } break;
+        for (int si = 0; si < hb::kSide && e == hipSuccess; ++si) {
+            const hipStream_t s = b->side[si];
+            bool forked = false;
+            for (int k = 0; k < np && e == hipSuccess; ++k) {
+                if (side_of[order[k]] != si) continue;
+                if (!forked) {
+                    e = hipStreamWaitEvent(s, b->fork, 0);
+                    forked = true;
+                }
+                switch (order[k] + hb::kBatchPMin) { HB_BATCH_P(4) HB_BATCH_P(5) HB_BATCH_P(6) HB_BATCH_P(7) HB_BATCH_P(8) }
+            }
+            if (forked && e == hipSuccess) e = hipEventRecord(b->join[si], s);
+            if (forked && e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->join[si], 0);
         }
-        HB_BATCH_P(4) HB_BATCH_P(5) HB_BATCH_P(6) HB_BATCH_P(7) HB_BATCH_P(8)
 #undef HB_BATCH_P
 #undef HB_BATCH_LAUNCH
         hb::k_batch_finalize<<<(ng + 63) / 64, 64, 0, b->stream>>>(d);
