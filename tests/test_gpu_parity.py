@@ -817,10 +817,17 @@ def test_chain_layout_window_matches_oracle(hb, oracle, kind, p):
     g = _chain_graph(hb, kind)
     say("chain window", kind, p, g.size(), g.width())
     ref = _oracle_run(oracle, g, p, max_iters=300)
+    linked = int((g.degrees() > 0).sum())
     for kw in (dict(), dict(dense=True), dict(dense=True, graph_capture=True)):
         with hb.HyperBall(g, p, layout="chain", max_iters=300, **kw) as h:
             assert h.stats.layout == 2
             _assert_same(h, ref, oracle, p, f"{kind} p={p} {kw}")
+            if kw.get("dense"):
+                # every iteration recomputes every linked node over every arc, and the
+                # kernels' node and arc counts say so
+                it = min(h.stats.iterations, len(h.stats.iter_nodes))
+                assert list(h.stats.iter_nodes[:it]) == [linked] * it, kw
+                assert list(h.stats.iter_arcs[:it]) == [g.width()] * it, kw
 
 
 def test_layout_switch_and_snapshots(hb, oracle):
