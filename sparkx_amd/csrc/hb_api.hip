@@ -735,7 +735,7 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
     HB_TRY(c->bias.ensure(1));
     if (c->snap_k) HB_TRY(c->snaps.ensure((size_t)c->snap_k * c->n));
     const bool push_ok = c->sym && p <= 10;
-    if (push_ok) HB_TRY(c->active.ensure(words));
+    if (push_ok) HB_TRY(c->active.ensure(2 * (size_t)words));  // two, by iteration parity
     {
         int rc = ensure_layout(c);  // normally built at staging already
         default_schedule(c, p);
@@ -803,6 +803,8 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
     HB_TRY(hipMemcpyAsync(c->st.ptr, &st0, sizeof st0, hipMemcpyHostToDevice, c->stream));
     HB_TRY(hb::launch_fill(c->bits0.ptr, words, 0u, c->stream));
     HB_TRY(hb::launch_fill(c->bits1.ptr, words, 0u, c->stream));
+    // iteration t marks the active bitmap iteration t - 1 cleared; both start clear
+    if (push_ok) HB_TRY(hb::launch_fill(c->active.ptr, 2 * (uint64_t)words, 0u, c->stream));
 
     hipEvent_t e0, e1, e2;
     HB_TRY(hipEventCreate(&e0));

@@ -73,7 +73,7 @@ struct HbDev {
     uint8_t *scratch;      // partial sketches of hub chunks
     uint32_t *chunk_flag;  // 1 if the chunk was recomputed this iteration
     // push iterations (symmetric graphs, p <= 10): nodes with a changed neighbour
-    uint32_t *active;      // bitmap
+    uint32_t *active;      // two bitmaps of (n + 31) / 32 words, by iteration parity (hb_active)
     uint32_t *dyn_list;    // their ids per degree bin, bin b at dyn_base[b]
     uint32_t *dyn_count;   // [3]
     uint32_t dyn_base[3];
@@ -90,6 +90,14 @@ struct HbDev {
     int window;            // dense bin-0 iterations read in-window neighbours from LDS (chain layout)
     int rows;              // ... and, at p = 5 or 6, run one node per lane (k_union_rows)
 };
+
+// The active bitmap of iteration t.  There are two, by parity: iteration t marks its own
+// in a push iteration and clears the other for iteration t + 1 (k_begin), so the marking
+// does not wait for a clear.
+__device__ __forceinline__ uint32_t *hb_active(const HbDev &d, uint32_t t)
+{
+    return d.active + (size_t)(t & 1u) * ((d.n + 31u) / 32u);
+}
 
 // ---------------------------------------------------------------------------
 // SipHash-1-3 of an 8-byte little-endian message, key (0,0)

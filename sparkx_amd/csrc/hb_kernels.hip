@@ -129,38 +129,40 @@ __global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t *p, uint64_t count
         p[i] = value;
 }
 
-// start of iteration t: clear the bitmap iteration t writes (bits[t & 1]); in a push
-// iteration also the active bitmap and the per-bin list counts
-__global__ __launch_bounds__(kBlock) void k_begin(HbDev d, uint32_t words)
-{
-    if (d.st->done) return;
-    uint32_t *bw = (d.st->iter & 1u) ? d.bits1 : d.bits0;
-    const bool push = d.st->push != 0;
-    if (push && blockIdx.x == 0 && threadIdx.x < 3) d.dyn_count[threadIdx.x] = 0u;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {  // this iteration's ring record, cleared
-        HbIterRec *r = d.rec + d.st->iter % HB_REC_RING;
-        *r = HbIterRec{};
-        if (push) r->push = 1ull;
-    }
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += gridDim.x * blockDim.x) {
-        bw[i] = 0u;
-        if (push) d.active[i] = 0u;
-    }
-}
-
 // ---------------------------------------------------------------------------
 // push iterations (symmetric graph, few changes at t-1): u must be recomputed
 // iff one of its neighbours changed, i.e. iff u is a neighbour of a changed node.
-// k_push_mark sets those bits from the changed nodes' lists (a wavefront per
-// bitmap word); k_push_compact lists them per degree bin with wavefront-aggregated
-// appends and copies c_{t-1}(u) into the output buffer for every non-hub node that
-// changed at t-1 but has no changed neighbour (hubs: k_merge copies).
+// k_begin sets those bits from the changed nodes' lists (a wavefront per bitmap
+// word); k_push_compact lists them per degree bin with wavefront-aggregated appends
+// and copies c_{t-1}(u) into the output buffer for every non-hub node that changed
+// at t-1 but has no changed neighbour (hubs: k_merge copies).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_push_mark(HbDev d, uint32_t words)
+// Start of iteration t: clear the bitmap iteration t writes (bits[t & 1]) and, where
+// push iterations can occur, the active bitmap of iteration t + 1; clear this
+// iteration's ring record; in a push iteration also clear the per-bin list counts and
+// mark this iteration's active bitmap.  The marking reads bits[(t - 1) & 1] and writes
+// hb_active(t), neither of which the clearing touches.
+__global__ __launch_bounds__(kBlock) void k_begin(HbDev d, uint32_t words)
 {
     const HbIterState *st = d.st;
-    if (st->done || !st->push) return;
-    const uint32_t *bitsR = (st->iter & 1u) ? d.bits0 : d.bits1;
+    if (st->done) return;
+    const uint32_t it = st->iter;
+    uint32_t *bw = (it & 1u) ? d.bits1 : d.bits0;
+    const bool push = st->push != 0;
+    if (push && blockIdx.x == 0 && threadIdx.x < 3) d.dyn_count[threadIdx.x] = 0u;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {  // this iteration's ring record, cleared
+        HbIterRec *r = d.rec + it % HB_REC_RING;
+        *r = HbIterRec{};
+        if (push) r->push = 1ull;
+    }
+    uint32_t *next_active = d.active ? hb_active(d, it + 1) : nullptr;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < words; i += gridDim.x * blockDim.x) {
+        bw[i] = 0u;
+        if (next_active) next_active[i] = 0u;
+    }
+    if (!push) return;
+    uint32_t *act = hb_active(d, it);
+    const uint32_t *bitsR = (it & 1u) ? d.bits0 : d.bits1;
     const uint32_t lane = threadIdx.x & (HB_WAVE - 1);
     const uint32_t waves = gridDim.x * (kBlock / HB_WAVE);
     for (uint32_t wd = blockIdx.x * (kBlock / HB_WAVE) + threadIdx.x / HB_WAVE; wd < words; wd += waves) {
@@ -171,7 +173,7 @@ __global__ __launch_bounds__(kBlock) void k_push_mark(HbDev d, uint32_t words)
             const uint32_t e1 = d.off[w + 1];
             for (uint32_t e = d.off[w] + lane; e < e1; e += HB_WAVE) {
                 const uint32_t u = d.nbr[e], bit = 1u << (u & 31u);
-                if (!(d.active[u >> 5] & bit)) atomicOr(d.active + (u >> 5), bit);
+                if (!(act[u >> 5] & bit)) atomicOr(act + (u >> 5), bit);
             }
         }
     }
@@ -185,13 +187,14 @@ __global__ __launch_bounds__(kBlock) void k_push_compact(HbDev d, uint32_t words
     const uint32_t *bitsR = (st->iter & 1u) ? d.bits0 : d.bits1;
     const uint8_t *in = st->parity ? d.regs1 : d.regs0;
     uint8_t *out = st->parity ? d.regs0 : d.regs1;
+    const uint32_t *act = hb_active(d, st->iter);
     const uint32_t lane = threadIdx.x & (HB_WAVE - 1);
     // the trip count is uniform across a wavefront, so every lane takes part in the ballots
     for (uint32_t base = blockIdx.x * blockDim.x; base < words; base += gridDim.x * blockDim.x) {
         const uint32_t wd = base + threadIdx.x;
         uint32_t a = 0, c = 0;
         if (wd < words) {
-            a = d.active[wd];
+            a = act[wd];
             c = bitsR[wd] & ~a;
         }
         while (c) {
@@ -560,7 +563,7 @@ __global__ __launch_bounds__(kBlock) void k_chunk(HbDev d, const uint32_t *__res
         const uint32_t *__restrict__ nb = d.nbr + eb;
         const uint32_t *__restrict__ nbh = d.orig + eb;
         const uint32_t len = ee - eb;
-        bool active = !(push && !bit_test(d.active, tnode[task]));
+        bool active = !(push && !bit_test(hb_active(d, it), tnode[task]));
         if (use_frontier && active) {
             uint32_t any = 0;
             for (uint32_t j = tl; j < len; j += T) any |= bit_test(bitsR, nb[j]);
@@ -1009,11 +1012,13 @@ hipError_t launch_iteration(const HbDev &d, const HbSchedule &sc, bool nf, hipSt
                             hipEvent_t *ev)
 {
     const uint32_t words = (d.n + 31) / 32;
-    k_begin<<<grid_for(words, 2048), kBlock, 0, s>>>(d, words);
+    const bool push_possible = d.sym && d.p <= 10 && d.frontier;
+    // where push iterations can occur k_begin also marks, one wavefront per bitmap word
+    k_begin<<<push_possible ? grid_for((uint64_t)words * HB_WAVE, 8192) : grid_for(words, 2048), kBlock, 0, s>>>(
+        d, words);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if (d.sym && d.p <= 10 && d.frontier) {  // return at once unless this is a push iteration
-        k_push_mark<<<grid_for((uint64_t)words * HB_WAVE, 8192), kBlock, 0, s>>>(d, words);
+    if (push_possible) {  // returns at once unless this is a push iteration
         HB_DISPATCH_P(d.p, k_push_compact<P><<<grid_for(words, 4096), kBlock, 0, s>>>(d, words));
         e = hipGetLastError();
         if (e != hipSuccess) return e;
