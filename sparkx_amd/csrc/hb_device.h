@@ -103,17 +103,31 @@ __device__ __forceinline__ uint32_t *hb_active(const HbDev &d, uint32_t t)
 // SipHash-1-3 of an 8-byte little-endian message, key (0,0)
 // (Aumasson & Bernstein; Rust's `usize: Hash` writes the value's 8 LE bytes)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint64_t hb_rotl(uint64_t x, int b) { return (x << b) | (x >> (64 - b)); }
+// 64-bit rotate left by B (0 < B <= 32) as two 32-bit funnel shifts (v_alignbit_b32):
+// the high word is the low 32 bits of (hi:lo) >> (32 - B), the low word of (lo:hi) >> (32 - B)
+template <int B>
+__device__ __forceinline__ uint64_t hb_rotl(uint64_t x)
+{
+    static_assert(B > 0 && B <= 32, "rotation");
+    const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    if constexpr (B == 32) {
+        return ((uint64_t)lo << 32) | hi;
+    } else {
+        const uint32_t nhi = __builtin_amdgcn_alignbit(hi, lo, 32 - B);
+        const uint32_t nlo = __builtin_amdgcn_alignbit(lo, hi, 32 - B);
+        return ((uint64_t)nhi << 32) | nlo;
+    }
+}
 
 __device__ __forceinline__ uint64_t hb_siphash13_u64(uint64_t msg)
 {
     uint64_t v0 = 0x736f6d6570736575ULL, v1 = 0x646f72616e646f6dULL;
     uint64_t v2 = 0x6c7967656e657261ULL, v3 = 0x7465646279746573ULL;
 #define HB_SIPROUND                                                              \
-    v0 += v1; v1 = hb_rotl(v1, 13); v1 ^= v0; v0 = hb_rotl(v0, 32);             \
-    v2 += v3; v3 = hb_rotl(v3, 16); v3 ^= v2;                                    \
-    v0 += v3; v3 = hb_rotl(v3, 21); v3 ^= v0;                                    \
-    v2 += v1; v1 = hb_rotl(v1, 17); v1 ^= v2; v2 = hb_rotl(v2, 32);
+    v0 += v1; v1 = hb_rotl<13>(v1); v1 ^= v0; v0 = hb_rotl<32>(v0);             \
+    v2 += v3; v3 = hb_rotl<16>(v3); v3 ^= v2;                                    \
+    v0 += v3; v3 = hb_rotl<21>(v3); v3 ^= v0;                                    \
+    v2 += v1; v1 = hb_rotl<17>(v1); v1 ^= v2; v2 = hb_rotl<32>(v2);
     v3 ^= msg;
     HB_SIPROUND
     v0 ^= msg;
