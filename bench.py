@@ -72,10 +72,10 @@ def algorithmic_bytes(slot, arcs, nodes, items, p, gathered, first_nodes=0):
     """Bytes a kernel slot must move: per recomputed node its offsets (8), own
     registers read + new registers written (2R), estimate read + write (8), dist/inv
     read + write (32); per neighbour id scanned 4; per neighbour register block read R
-    (`gathered`: none in iteration 1, which hashes ids, and only the neighbours that
+    (`gathered`: none in an iteration 1 that hashes ids, and only the neighbours that
     changed outside dense iterations).  Hub chunks add one partial sketch write per
-    task.  In iteration 1 (p <= 10) the node's own block is not read either
-    (first_nodes)."""
+    task.  Where iteration 1 hashes (p <= 10, every bin but one that k_union_rows runs)
+    the node's own block is not read either (first_nodes)."""
     R = 1 << p
     b = nodes * (8 + 2 * R + 8 + 32) + arcs * 4 + gathered * R - first_nodes * R
     if slot == 3:

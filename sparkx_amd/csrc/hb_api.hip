@@ -960,7 +960,9 @@ extern "C" int hb_run(hb_ctx *c, int p, int max_iters, uint32_t flags, hb_stats 
         if (p <= 10 && executed >= 1)
             for (int sl = 0; sl < HB_NSLOT; ++sl) {
                 // bins hash the ids of the neighbours and of the node itself, hub chunks
-                // those of the neighbours; the hub merge reads register blocks as always
+                // those of the neighbours; the hub merge reads register blocks as always,
+                // and so does a bin that k_union_rows runs (it gathers the c_0 rows)
+                if (sl < HB_SLOT_CHUNK && hb::rows_bin(c->dev, c->sched, sl)) continue;
                 if (sl <= HB_SLOT_CHUNK) stats->kernel_first_arcs[sl] = c->h_rec[1].arcs[sl];
                 if (sl < HB_SLOT_CHUNK) stats->kernel_first_nodes[sl] = c->h_rec[1].nodes[sl];
             }

@@ -25,6 +25,15 @@ struct HbSchedule {
     unsigned max_blocks = 8192;        // grid cap of the persistent loops
 };
 
+// Bin b is run by k_union_rows in its dense iterations (iteration 1 included) and by
+// k_union in the others: the chain layout at p = 5 or 6, an id range from a multiple of
+// 64, one sketch copy per node.
+inline bool rows_bin(const HbDev &d, const HbSchedule &sc, int b)
+{
+    return d.rows && !sc.bin_list[b] && (sc.bin_base[b] & 63u) == 0 &&
+           sc.bin_team[b] == (int)((1u << d.p) / 16);
+}
+
 // thread-local message returned by hb_last_error() (set by every C-ABI entry point)
 void set_error(const char *msg);
 // host-built HLL++ bias windows of precision p (4..16), built once per process

@@ -313,10 +313,12 @@ __global__ __launch_bounds__(kBlock) void k_arc_fingerprint(const uint32_t *off,
 // Same registers, estimates and accumulators as union_node.
 
 // an iteration whose id-range bins k_union_rows runs instead of k_union: dense (no
-// frontier gathers, no push list) and after the first
+// frontier gathers, no push list).  That includes the first: k_init wrote every node's
+// c_0 row into the buffer iteration 1 reads, and gathering those rows, most of them from
+// the LDS window, is cheaper here than hashing every neighbour id as k_union does.
 __device__ __forceinline__ bool rows_iteration(const HbDev &d, const HbIterState &st)
 {
-    return st.iter > 1 && !st.push && !(d.frontier && !st.skip_scan);
+    return !st.push && (st.iter == 1 || !(d.frontier && !st.skip_scan));
 }
 
 // The window of a wavefront is W planes of 64 uint4: plane q holds chunk q (bytes
@@ -1034,8 +1036,7 @@ hipError_t launch_iteration(const HbDev &d, const HbSchedule &sc, bool nf, hipSt
             });
         } else {
             // k_union_rows and k_union each return at once in the iterations the other runs
-            if (d.rows && !sc.bin_list[b] && (sc.bin_base[b] & 63u) == 0 &&
-                sc.bin_team[b] == (int)((1u << d.p) / 16)) {
+            if (rows_bin(d, sc, b)) {
                 const unsigned grid = grid_for(sc.bin_count[b], sc.max_blocks);
                 if (d.p == 5) k_union_rows<5><<<grid, kBlock, 0, s>>>(d, sc.bin_base[b], sc.bin_count[b], b);
                 else k_union_rows<6><<<grid, kBlock, 0, s>>>(d, sc.bin_base[b], sc.bin_count[b], b);
