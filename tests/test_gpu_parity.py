@@ -137,6 +137,16 @@ def test_schedules_frontier_dense_and_graph_capture_agree(hb, oracle):
             _assert_same(h, ref, oracle, p, str(kw))
             if "schedule" in kw and kw["schedule"][1] < g.degrees().max():
                 assert h.stats.kernel_items[3] > 0  # hub chunks were used
+            if kw.get("ktiming"):
+                # every executed iteration was timed once, each slot once per iteration it
+                # launched, from events read at the host's usual checks
+                st = h.stats
+                it = st.iterations
+                assert all(st.iter_ms[t] > 0 for t in range(it)) and st.iter_ms[it] == 0
+                assert sum(st.iter_ms[:it]) <= st.ms_total * 1.001
+                for s in range(5):
+                    assert st.kernel_launches[s] == (it if st.kernel_items[s] else 0), s
+                    assert (st.kernel_ms[s] > 0) == (st.kernel_items[s] > 0), s
 
 
 def test_push_iterations_on_symmetric_graphs(hb, oracle):
