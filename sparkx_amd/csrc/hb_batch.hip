@@ -251,8 +251,11 @@ __device__ __forceinline__ uint32_t wave_count() { return gridDim.x * (kBatchBlo
 
 // Union over the active node tiles of one (precision P, team width T) class.  Every
 // loop has a trip count fixed by the active list and its tiles (at most 4096 nodes).
+// Six wavefronts per SIMD from p = 6 up (80 VGPRs: the one-copy bins spill 2-5 where
+// seven spilled 9-19): with the precisions' chains overlapping, the p = 7 chain is the
+// longest, and the 64-graph run takes 14.1 ms against 14.6 ms at seven.
 template <int P, int T>
-__global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(P >= 7 ? 7 : 8))) void k_batch_nodes(
+__global__ __launch_bounds__(kBatchBlock) __attribute__((amdgpu_waves_per_eu(P >= 6 ? 6 : 8))) void k_batch_nodes(
     HbBatchDev b, HbBatchList l)
 {
     const uint32_t count = b.active_count[l.queue];
