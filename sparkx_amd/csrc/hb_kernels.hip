@@ -468,15 +468,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) voi
 // ---------------------------------------------------------------------------
 // Wavefronts per SIMD: the gathers are latency-bound, so more wavefronts in flight
 // help until the register bound makes the compiler spill.  8 (at most 64 VGPRs) took the
-// k-mer graph (p = 6, four-lane teams in the degree layout) from 59.5 to 49.5 ms per run;
-// at p >= 7 the register rows are wider and 7 (at most 72 VGPRs, no spill in the degree
-// <= 256 bin) is faster: R-MAT-24 (p = 7) 41.8 ms at 8, 41.0 at 7.  The teams of 32 and 64
-// lanes spill 10-12 VGPRs at 72; at 6 (80 VGPRs) they do not, and R-MAT-24's degree
-// 257..4096 bins run 1-2% faster.  At p <= 6 one-copy teams run at 7 (no spill, where 8
+// k-mer graph (p = 6, four-lane teams in the degree layout) from 59.5 to 49.5 ms per run.
+// At p >= 7 the one-copy bins run at 8 too since their per-lane counters are 32-bit
+// (HbCounts): 64 VGPRs without a spill, R-MAT-24's degree <= 256 bin 2.145 -> 2.130 ms per
+// launch against 7 (with 64-bit counters 8 spilled: 41.8 ms per run against 41.0 at 7).
+// The teams of 32 and 64 lanes spill 10-12 VGPRs at 72; at 6 (80 VGPRs) they do not, and
+// R-MAT-24's degree 257..4096 bins run 1-2% faster.  At p <= 6 one-copy teams run at 7 (no spill, where 8
 // spilled 7 VGPRs): ER (n = 65,536, p = 6) 44.7 -> 43.5 us per launch; the k-mer graph's
 // dense iterations run k_union_rows instead.
 template <int P, int T>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(P >= 7 ? (T >= 32 ? 6 : 7) : (T == (1 << P) / 16 ? 7 : 8)))) void k_union(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(P >= 7 ? (T >= 32 ? 6 : 8) : (T == (1 << P) / 16 ? 7 : 8)))) void k_union(
     HbDev d, const uint32_t *__restrict__ list, uint32_t base, uint32_t count, int slot_id)
 {
     const HbIterState *st = d.st;

@@ -30,9 +30,12 @@ struct HbView {
     bool use_frontier;
 };
 
+// One lane's counts over its share of a launch.  32 bits suffice (a lane's sums are at
+// most the graph's node or arc count, both below 2^32 in the u32 CSR) and keep five
+// registers free in the union kernels; the block and batch sums widen them to 64 bits.
 struct HbCounts {
-    unsigned long long inc = 0, arcs = 0, nodes = 0, chg = 0;
-    unsigned long long gathered = 0;  // neighbour register blocks read
+    uint32_t inc = 0, arcs = 0, nodes = 0, chg = 0;
+    uint32_t gathered = 0;  // neighbour register blocks read
 };
 
 __device__ __forceinline__ bool bit_test(const uint32_t *bits, uint32_t v)
