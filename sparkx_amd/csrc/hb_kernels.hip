@@ -472,8 +472,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(7))) voi
 // At p >= 7 the one-copy bins run at 8 too since their per-lane counters are 32-bit
 // (HbCounts): 64 VGPRs without a spill, R-MAT-24's degree <= 256 bin 2.145 -> 2.130 ms per
 // launch against 7 (with 64-bit counters 8 spilled: 41.8 ms per run against 41.0 at 7).
-// The teams of 32 and 64 lanes spill 10-12 VGPRs at 72; at 6 (80 VGPRs) they do not, and
-// R-MAT-24's degree 257..4096 bins run 1-2% faster.  At p <= 6 one-copy teams run at 7 (no spill, where 8
+// The teams of 32 and 64 lanes spill 10-12 VGPRs at 72 (6 with the 32-bit counters); at 6
+// (80 VGPRs) they do not, and R-MAT-24's degree 257..4096 bins run 1-2% faster (with the
+// 32-bit counters, 7 still leaves the run 0.2-0.3% slower).  At p <= 6 one-copy teams run at 7 (no spill, where 8
 // spilled 7 VGPRs): ER (n = 65,536, p = 6) 44.7 -> 43.5 us per launch; the k-mer graph's
 // dense iterations run k_union_rows instead.
 template <int P, int T>
